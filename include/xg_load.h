@@ -1,0 +1,208 @@
+/*
+ * xg_load.h — C ABI of the gfx950 device-load library (libxg_load.so).
+ *
+ * The library evaluates Xyce MOSFET level-1 and BSIM3 (level 9/49) instances
+ * on an MI355X and assembles their residual vectors and Jacobian stamps into
+ * a fixed CSR pattern.  It is the drop-in target of a Device "Master"
+ * (src/DeviceModelPKG/Core/include/N_DEV_Device.h:93-270): the three
+ * per-Newton calls map one-to-one onto
+ *
+ *   Device::updateState(solVec, staVec, stoVec)            N_DEV_Device.h:214
+ *       -> xg_update_state
+ *   Device::loadDAEVectors(solVec, fVec, qVec, ...)        N_DEV_Device.h:248
+ *       -> xg_load_vectors
+ *   Device::loadDAEMatrices(dFdx, dQdx)                    N_DEV_Device.h:265
+ *       -> xg_load_matrices
+ *
+ * Conventions (N_LAS_Builder.C:336-338, N_TIA_Assembler.C:156-162):
+ *  - every vector has n_nodes + 1 entries; the last one is the ground slot
+ *    (GID -1).  Writes to it happen as in the reference and are ignored by
+ *    the caller;
+ *  - the Jacobian is an overlap-with-ground CSR matrix with n_nodes + 1 rows,
+ *    each row's columns sorted by local id with the ground column last;
+ *  - outputs are accumulated with "+=": the caller zeroes them first, as
+ *    the reference assembler does.  Contributions to one output entry are
+ *    added in the reference's order (device type, instance insertion order,
+ *    statement order within the instance), so results are bitwise
+ *    reproducible run to run;
+ *  - all pointers passed to the xg_update_state / xg_load_* calls are
+ *    device (HBM) pointers; the xg_*_host variants take host pointers and
+ *    stage through pinned buffers;
+ *  - every call returns an int status: XG_OK (0) or a negative XG_E_* code.
+ *    The library never aborts.  xg_last_error() gives a message.
+ *
+ * One handle serves one stream; calls on a handle are not re-entrant.
+ * Work is enqueued asynchronously; xg_sync() waits for it.
+ */
+#ifndef XG_LOAD_H
+#define XG_LOAD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XG_OK 0
+#define XG_E_ARG (-1)       /* bad argument / unknown parameter name        */
+#define XG_E_STATE (-2)     /* call out of order (e.g. before finalize)     */
+#define XG_E_HIP (-3)       /* HIP runtime error                            */
+#define XG_E_PATTERN (-4)   /* supplied CSR pattern lacks a stamp entry     */
+#define XG_E_NOMEM (-5)     /* host or device allocation failed             */
+#define XG_E_PARAM (-6)     /* parameter value the model rejects            */
+
+typedef struct xg_circuit xg_circuit; /* setup-time builder                 */
+typedef struct xg_handle xg_handle;   /* finalized, device-resident load    */
+
+/* Per-Newton-iteration flags; mirrors the fields of SolverState that the
+   MOSFET1/BSIM3 loads branch on (N_DEV_SolverState.h:75-200, set by
+   DeviceMgr::setupSolverInfo_, N_DEV_DeviceMgr.C:3473-3634). */
+typedef struct xg_solver_state {
+  int32_t dcopFlag;
+  int32_t initJctFlag;
+  int32_t initFixFlag;
+  int32_t initTranFlag;
+  int32_t newtonIter;
+  int32_t tranopFlag;
+  int32_t acopFlag;
+  int32_t transientFlag;
+} xg_solver_state;
+
+/* Load statistics of the last xg_update_state (device-side counts). */
+typedef struct xg_stats {
+  int64_t n_limited;   /* instances whose limiter fired (isConverged()=false) */
+  int64_t n_nonfinite; /* instances that produced a non-finite value          */
+} xg_stats;
+
+const char *xg_last_error(void);
+
+/* ---- setup ------------------------------------------------------------- */
+
+/* New circuit with n_nodes non-ground unknowns.  State and store vectors
+   have n_state / n_store entries; devices are given their base offsets.
+   Replaces the LID/state/store registration done by Topology
+   (N_TOP_CktGraphBasic.C:1092-1145, DeviceInstance::registerStateLIDs). */
+int xg_circuit_new(xg_circuit **c, int32_t n_nodes, int32_t n_state,
+                   int32_t n_store);
+void xg_circuit_free(xg_circuit *c);
+
+/* Device options (N_DEV_DeviceOptions.C:82-147): gmin (default 1e-12),
+   voltage limiter on/off (default on), circuit temperature in kelvin
+   (default 300.15) and TNOM default (300.15). */
+int xg_circuit_set_options(xg_circuit *c, double gmin, int32_t limiter,
+                           double temp, double tnom);
+
+/* MOSFET level 1 model card: name/value pairs as on a .MODEL line
+   (parameter table N_DEV_MOSFET1.C:117-263); dtype = +1 NMOS, -1 PMOS.
+   Replaces MOSFET1::Model::Model + processParams (N_DEV_MOSFET1.C:3786-3980). */
+int xg_add_mos1_model(xg_circuit *c, int32_t dtype, int32_t n,
+                      const char *const *keys, const double *vals,
+                      int32_t *model_id);
+
+/* MOSFET level 1 instance (N_DEV_MOSFET1.C:76-110 parameters; constructor
+   and updateTemperature N_DEV_MOSFET1.C:313-1260, 3437-3680).  lids =
+   {d, g, s, b, d', s'}; d'/s' are read only when the instance has a drain /
+   source resistance (else they collapse onto d / s).  -1 means ground.
+   The instance owns 8 state slots at sta_base and 5 store slots at
+   sto_base in the order of registerStateLIDs / registerStoreLIDs
+   (N_DEV_MOSFET1.C:1486-1566). */
+int xg_add_mos1_instance(xg_circuit *c, int32_t model_id, int32_t n,
+                         const char *const *keys, const double *vals,
+                         const int32_t lids[6], int32_t sta_base,
+                         int32_t sto_base, int32_t *inst_id);
+
+/* BSIM3 v3.2 model card (parameter table N_DEV_MOSFET_B3.C:132-1820;
+   Model::processParams N_DEV_MOSFET_B3.C:8488-8596). */
+int xg_add_bsim3_model(xg_circuit *c, int32_t dtype, int32_t n,
+                       const char *const *keys, const double *vals,
+                       int32_t *model_id);
+
+/* BSIM3 instance (N_DEV_MOSFET_B3.C:1845-2785, updateTemperature and the
+   size-dependent parameters N_DEV_MOSFET_B3.C:3611-4518).  lids =
+   {d, g, s, b, d', s'}; 12 state slots at sta_base and 5 store slots at
+   sto_base (N_DEV_MOSFET_B3.C:2360-2364). */
+int xg_add_bsim3_instance(xg_circuit *c, int32_t model_id, int32_t n,
+                          const char *const *keys, const double *vals,
+                          const int32_t lids[6], int32_t sta_base,
+                          int32_t sto_base, int32_t *inst_id);
+
+/* Stamp of a device the library does not evaluate (resistor, capacitor,
+   source ...), so that the CSR pattern is the whole circuit's.
+   gids[n_nodes_dev] are the device's solution ids (-1 = ground); entry e is
+   (rows[e], cols[e]) in device-local numbering, as in jacobianStamp(). */
+int xg_add_foreign_stamp(xg_circuit *c, int32_t n_dev_nodes,
+                         const int32_t *gids, int32_t n_entries,
+                         const int32_t *rows, const int32_t *cols);
+
+/* Use a CSR pattern built elsewhere (e.g. by the reference topology) instead
+   of building one.  It must contain every stamp entry. */
+int xg_circuit_set_pattern(xg_circuit *c, const int32_t *rowptr,
+                           const int32_t *colidx);
+
+/* Build pattern, offsets and the ordered-reduction program; upload all
+   device data.  The circuit may be freed afterwards. */
+int xg_finalize(xg_circuit *c, xg_handle **h);
+
+/* Same setup work as xg_finalize (pattern, offsets, per-instance fields,
+   reduction program) without touching the GPU: the handle answers the
+   xg_get_* / xg_pattern_size queries only and every load call on it fails
+   with XG_E_STATE.  For checking setup on hosts without a device. */
+int xg_finalize_plan(xg_circuit *c, xg_handle **h);
+void xg_destroy(xg_handle *h);
+
+/* Pattern of the finalized handle: rowptr has n_nodes + 2 entries. */
+int xg_pattern_size(const xg_handle *h, int32_t *n_rows_with_ground,
+                    int64_t *nnz);
+int xg_get_pattern(const xg_handle *h, int32_t *rowptr, int32_t *colidx);
+
+/* CSR positions of every instance's stamp entries, in the reference's
+   registerJacLIDs order (MOSFET1: the 22 offsets of
+   N_DEV_MOSFET1.C:1650-1673, BSIM3: N_DEV_MOSFET_B3.C:3150-3364).
+   Offsets are within the row, as matrixGlobalToLocal produces them. */
+int xg_get_mos1_offsets(const xg_handle *h, int32_t *off /* n_inst x 22 */);
+int xg_get_bsim3_offsets(const xg_handle *h, int32_t *off /* n_inst x 24 */);
+
+/* Per-instance temperature/geometry fields the load reads, for checking the
+   setup against other implementations (layout: xg_mos1_field_names). */
+int xg_get_mos1_fields(const xg_handle *h, double *out, int32_t *n_fields);
+int xg_get_bsim3_fields(const xg_handle *h, double *out, int32_t *n_fields);
+
+/* ---- per Newton iteration (device pointers) ----------------------------- */
+
+int xg_set_solver_state(xg_handle *h, const xg_solver_state *ss);
+
+/* Master::updateState (N_DEV_MOSFET1.C:4103, N_DEV_MOSFET_B3.C:9229): reads
+   x_next, sta_curr, sto_curr (and sto_next when newtonIter > 0), writes
+   sta_next / sto_next (and sta_curr at initTran for BSIM3), and computes
+   every stamp value of the subsequent vector and matrix loads. */
+int xg_update_state(xg_handle *h, const double *x_next, double *sta_curr,
+                    const double *sto_curr, double *sta_next, double *sto_next);
+
+/* Master::loadDAEVectors (N_DEV_MOSFET1.C:4202, N_DEV_MOSFET_B3.C:9306). */
+int xg_load_vectors(xg_handle *h, double *F, double *Q, double *dFdxdVp,
+                    double *dQdxdVp);
+
+/* Master::loadDAEMatrices (N_DEV_MOSFET1.C:4441, N_DEV_MOSFET_B3.C:9654):
+   values of the two CSR matrices (same pattern). */
+int xg_load_matrices(xg_handle *h, double *dFdx_vals, double *dQdx_vals);
+
+/* All three in one enqueue (same results; fewer launches). */
+int xg_load_all(xg_handle *h, const double *x_next, double *sta_curr,
+                const double *sto_curr, double *sta_next, double *sto_next,
+                double *F, double *Q, double *dFdxdVp, double *dQdxdVp,
+                double *dFdx_vals, double *dQdx_vals);
+
+int xg_get_stats(xg_handle *h, xg_stats *st);
+int xg_sync(xg_handle *h);
+
+/* Stream the handle enqueues on (hipStream_t; 0 = default stream). */
+int xg_set_stream(xg_handle *h, void *stream);
+
+/* Sizes the library was finalized with. */
+int xg_sizes(const xg_handle *h, int32_t *n_nodes, int32_t *n_state,
+             int32_t *n_store, int32_t *n_mos1, int32_t *n_bsim3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+"""ctypes binding of libxg_load.so (the C ABI declared in include/xg_load.h).
+
+The library must have been built (``python -m xyce_amd.build``); importing
+this module without it raises immediately: there is no fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxg_load.so")
+
+XG_OK = 0
+ERRORS = {
+    -1: "XG_E_ARG", -2: "XG_E_STATE", -3: "XG_E_HIP", -4: "XG_E_PATTERN",
+    -5: "XG_E_NOMEM", -6: "XG_E_PARAM",
+}
+
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_char_pp = ctypes.POINTER(ctypes.c_char_p)
+
+
+class SolverState(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "dcopFlag", "initJctFlag", "initFixFlag", "initTranFlag", "newtonIter",
+        "tranopFlag", "acopFlag", "transientFlag")]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_limited", ctypes.c_int64), ("n_nonfinite", ctypes.c_int64)]
+
+
+# (name, restype, argtypes) for every exported symbol of include/xg_load.h
+SIGNATURES = [
+    ("xg_last_error", ctypes.c_char_p, []),
+    ("xg_circuit_new", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("xg_circuit_free", None, [ctypes.c_void_p]),
+    ("xg_circuit_set_options", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_double]),
+    ("xg_add_mos1_model", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, c_double_p, c_int32_p]),
+    ("xg_add_mos1_instance", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, c_double_p, c_int32_p, ctypes.c_int32, ctypes.c_int32, c_int32_p]),
+    ("xg_add_bsim3_model", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, c_double_p, c_int32_p]),
+    ("xg_add_bsim3_instance", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, c_double_p, c_int32_p, ctypes.c_int32, ctypes.c_int32, c_int32_p]),
+    ("xg_add_foreign_stamp", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_int32_p, ctypes.c_int32, c_int32_p, c_int32_p]),
+    ("xg_circuit_set_pattern", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p]),
+    ("xg_finalize", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("xg_finalize_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("xg_destroy", None, [ctypes.c_void_p]),
+    ("xg_pattern_size", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p]),
+    ("xg_get_pattern", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p]),
+    ("xg_get_mos1_offsets", ctypes.c_int, [ctypes.c_void_p, c_int32_p]),
+    ("xg_get_bsim3_offsets", ctypes.c_int, [ctypes.c_void_p, c_int32_p]),
+    ("xg_get_mos1_fields", ctypes.c_int, [ctypes.c_void_p, c_double_p, c_int32_p]),
+    ("xg_get_bsim3_fields", ctypes.c_int, [ctypes.c_void_p, c_double_p, c_int32_p]),
+    ("xg_set_solver_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverState)]),
+    ("xg_update_state", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
+    ("xg_load_vectors", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    ("xg_load_matrices", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 2),
+    ("xg_load_all", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 11),
+    ("xg_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    ("xg_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("xg_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),
+]
+
+
+class XgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libxg_load.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise XgError("libxg_load.so is not built: run `python -m xyce_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != XG_OK:
+        msg = lib().xg_last_error().decode(errors="replace")
+        raise XgError("%s failed: %s (%s)" % (what, ERRORS.get(rc, rc), msg))
+
+
+def keys_vals(params):
+    """dict -> (n, char**, double*) for the name/value parameter lists."""
+    items = list(params.items())
+    n = len(items)
+    keys = (ctypes.c_char_p * max(n, 1))(*[k.encode() for k, _ in items])
+    vals = (ctypes.c_double * max(n, 1))(*[float(v) for _, v in items])
+    return n, keys, vals

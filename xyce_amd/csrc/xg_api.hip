@@ -1,0 +1,472 @@
+// C ABI of libxg_load.so: circuit setup, device-resident handles and the
+// per-Newton load entry points.  See include/xg_load.h for the contract.
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "xg_bsim3.h"
+#include "xg_internal.h"
+#include "xg_setup.h"
+
+struct xg_circuit {
+  xg::Circuit c;
+};
+
+namespace xg {
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Handle {
+  Plan plan;
+  hipStream_t stream = nullptr;
+  SolverStateDev ss{};
+  std::vector<std::unique_ptr<DevBuf>> bufs;
+  // MOSFET1 device data
+  Mos1Args m1{};
+  // BSIM3 device data
+  Bsim3Dev b3{};
+  // reduction programs
+  ReduceDev red_vec[4]{}, red_mat[2]{};
+  double *slots = nullptr;
+  int64_t *count_buf = nullptr;
+  int32_t *limited_all = nullptr;  // m1 then b3 eval slots
+  int32_t n_limited_slots = 0;
+  // host staging for the *_host variants
+  double *pinned = nullptr;
+  bool plan_only = false;
+
+  template <class T>
+  T *alloc(size_t n) {
+    auto b = std::make_unique<DevBuf>();
+    if (n == 0) n = 1;
+    if (hipMalloc(&b->p, n * sizeof(T)) != hipSuccess) return nullptr;
+    T *r = (T *)b->p;
+    bufs.push_back(std::move(b));
+    return r;
+  }
+  template <class T>
+  T *upload(const std::vector<T> &v) {
+    T *d = alloc<T>(v.size());
+    if (!d) return nullptr;
+    if (!v.empty() &&
+        hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return nullptr;
+    return d;
+  }
+  ~Handle() {
+    if (pinned) (void)hipHostFree(pinned);
+  }
+};
+
+}  // namespace xg
+
+struct xg_handle {
+  xg::Handle h;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return XG_E_HIP;
+}
+}  // namespace
+
+extern "C" {
+
+const char *xg_last_error(void) { return g_err.c_str(); }
+
+int xg_circuit_new(xg_circuit **c, int32_t n_nodes, int32_t n_state,
+                   int32_t n_store) {
+  if (!c || n_nodes < 0 || n_state < 0 || n_store < 0)
+    return fail(XG_E_ARG, "xg_circuit_new: bad argument");
+  auto *p = new (std::nothrow) xg_circuit;
+  if (!p) return fail(XG_E_NOMEM, "xg_circuit_new: out of memory");
+  p->c.n_nodes = n_nodes;
+  p->c.n_state = n_state;
+  p->c.n_store = n_store;
+  *c = p;
+  return XG_OK;
+}
+
+void xg_circuit_free(xg_circuit *c) { delete c; }
+
+int xg_circuit_set_options(xg_circuit *c, double gmin, int32_t limiter,
+                           double temp, double tnom) {
+  if (!c || !(temp > 0) || !(tnom > 0) || !(gmin >= 0))
+    return fail(XG_E_ARG, "xg_circuit_set_options: bad argument");
+  if (!c->c.mos1_models.empty() || !c->c.b3_models.empty())
+    return fail(XG_E_STATE, "xg_circuit_set_options: set options before models");
+  c->c.opt.gmin = gmin;
+  c->c.opt.limiter = limiter ? 1 : 0;
+  c->c.opt.temp = temp;
+  c->c.opt.tnom = tnom;
+  return XG_OK;
+}
+
+int xg_add_mos1_model(xg_circuit *c, int32_t dtype, int32_t n,
+                      const char *const *keys, const double *vals,
+                      int32_t *model_id) {
+  if (!c || (dtype != 1 && dtype != -1) || !model_id)
+    return fail(XG_E_ARG, "xg_add_mos1_model: bad argument");
+  xg::Mos1Model m;
+  m.dtype = dtype;
+  if (xg::parse_params(n, keys, vals, m.p))
+    return fail(XG_E_ARG, "xg_add_mos1_model: bad parameter list");
+  std::string err;
+  int rc = xg::mos1_model_process(m, c->c.opt, err);
+  if (rc) return fail(rc, err);
+  *model_id = (int32_t)c->c.mos1_models.size();
+  c->c.mos1_models.push_back(std::move(m));
+  return XG_OK;
+}
+
+int xg_add_mos1_instance(xg_circuit *c, int32_t model_id, int32_t n,
+                         const char *const *keys, const double *vals,
+                         const int32_t lids[6], int32_t sta_base,
+                         int32_t sto_base, int32_t *inst_id) {
+  if (!c || !lids || !inst_id || model_id < 0 ||
+      model_id >= (int32_t)c->c.mos1_models.size())
+    return fail(XG_E_ARG, "xg_add_mos1_instance: bad argument");
+  xg::Mos1Inst in;
+  in.model = model_id;
+  if (xg::parse_params(n, keys, vals, in.p))
+    return fail(XG_E_ARG, "xg_add_mos1_instance: bad parameter list");
+  for (int k = 0; k < 6; ++k) in.lids[k] = lids[k] < 0 ? -1 : lids[k];
+  in.sta_base = sta_base;
+  in.sto_base = sto_base;
+  std::string err;
+  int rc = xg::mos1_instance_setup(c->c.mos1_models[model_id], in, c->c.opt,
+                                   err);
+  if (rc) return fail(rc, err);
+  *inst_id = (int32_t)c->c.mos1.size();
+  c->c.mos1.push_back(std::move(in));
+  return XG_OK;
+}
+
+int xg_add_bsim3_model(xg_circuit *c, int32_t dtype, int32_t n,
+                       const char *const *keys, const double *vals,
+                       int32_t *model_id) {
+  if (!c || (dtype != 1 && dtype != -1) || !model_id)
+    return fail(XG_E_ARG, "xg_add_bsim3_model: bad argument");
+  xg::ParamSet p;
+  if (xg::parse_params(n, keys, vals, p))
+    return fail(XG_E_ARG, "xg_add_bsim3_model: bad parameter list");
+  std::string err;
+  int rc = xg::bsim3_add_model(c->c, dtype, p, model_id, err);
+  return rc ? fail(rc, err) : XG_OK;
+}
+
+int xg_add_bsim3_instance(xg_circuit *c, int32_t model_id, int32_t n,
+                          const char *const *keys, const double *vals,
+                          const int32_t lids[6], int32_t sta_base,
+                          int32_t sto_base, int32_t *inst_id) {
+  if (!c || !lids || !inst_id)
+    return fail(XG_E_ARG, "xg_add_bsim3_instance: bad argument");
+  xg::ParamSet p;
+  if (xg::parse_params(n, keys, vals, p))
+    return fail(XG_E_ARG, "xg_add_bsim3_instance: bad parameter list");
+  std::string err;
+  int rc = xg::bsim3_add_instance(c->c, model_id, p, lids, sta_base, sto_base,
+                                  inst_id, err);
+  return rc ? fail(rc, err) : XG_OK;
+}
+
+int xg_add_foreign_stamp(xg_circuit *c, int32_t n_dev_nodes,
+                         const int32_t *gids, int32_t n_entries,
+                         const int32_t *rows, const int32_t *cols) {
+  if (!c || n_dev_nodes <= 0 || n_entries < 0 || !gids ||
+      (n_entries && (!rows || !cols)))
+    return fail(XG_E_ARG, "xg_add_foreign_stamp: bad argument");
+  xg::ForeignStamp fs;
+  for (int32_t k = 0; k < n_dev_nodes; ++k) {
+    if (gids[k] >= c->c.n_nodes)
+      return fail(XG_E_ARG, "xg_add_foreign_stamp: node id out of range");
+    fs.gids.push_back(gids[k] < 0 ? -1 : gids[k]);
+  }
+  for (int32_t e = 0; e < n_entries; ++e) {
+    if (rows[e] < 0 || rows[e] >= n_dev_nodes || cols[e] < 0 ||
+        cols[e] >= n_dev_nodes)
+      return fail(XG_E_ARG, "xg_add_foreign_stamp: entry out of range");
+    fs.rows.push_back(rows[e]);
+    fs.cols.push_back(cols[e]);
+  }
+  c->c.foreign.push_back(std::move(fs));
+  return XG_OK;
+}
+
+int xg_circuit_set_pattern(xg_circuit *c, const int32_t *rowptr,
+                           const int32_t *colidx) {
+  if (!c || !rowptr || !colidx)
+    return fail(XG_E_ARG, "xg_circuit_set_pattern: bad argument");
+  const int32_t nr = c->c.n_nodes + 1;
+  if (rowptr[0] != 0) return fail(XG_E_ARG, "rowptr[0] != 0");
+  for (int32_t r = 0; r < nr; ++r) {
+    if (rowptr[r + 1] < rowptr[r]) return fail(XG_E_ARG, "rowptr not monotone");
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+      if (colidx[j] < 0 || colidx[j] >= nr)
+        return fail(XG_E_ARG, "column index out of range");
+      if (j > rowptr[r] && colidx[j] <= colidx[j - 1])
+        return fail(XG_E_ARG, "row columns not strictly increasing");
+    }
+  }
+  c->c.rowptr.assign(rowptr, rowptr + nr + 1);
+  c->c.colidx.assign(colidx, colidx + rowptr[nr]);
+  c->c.have_pattern = true;
+  return XG_OK;
+}
+
+int xg_finalize(xg_circuit *c, xg_handle **out) {
+  if (!c || !out) return fail(XG_E_ARG, "xg_finalize: bad argument");
+  auto *H = new (std::nothrow) xg_handle;
+  if (!H) return fail(XG_E_NOMEM, "xg_finalize: out of memory");
+  xg::Handle &h = H->h;
+  std::string err;
+  int rc = xg::build_plan(c->c, h.plan, err);
+  if (rc) {
+    delete H;
+    return fail(rc, err);
+  }
+  const xg::Plan &P = h.plan;
+  bool ok = true;
+  auto chk = [&](const void *p) { ok = ok && p != nullptr; };
+  // MOSFET1
+  h.m1.n_slots = P.m1_slots;
+  chk(h.m1.inst_of = h.upload(P.m1_inst_of));
+  chk(h.m1.wave_card = h.upload(P.m1_wave_card));
+  chk(h.m1.cards = h.upload(P.m1_cards));
+  chk(h.m1.fields = h.upload(P.m1_fields));
+  chk(h.m1.lid4 = h.upload(P.m1_lid4));
+  chk(h.m1.lid_int = h.upload(P.m1_lid_int));
+  chk(h.m1.sta_base = h.upload(P.m1_sta));
+  chk(h.m1.sto_base = h.upload(P.m1_sto));
+  chk(h.m1.flags = h.upload(P.m1_flags));
+  // BSIM3
+  if (ok) {
+    rc = xg::bsim3_upload(P, h.b3, [&](const std::vector<double> &v) {
+      return h.upload(v);
+    }, [&](const std::vector<int32_t> &v) { return h.upload(v); });
+    ok = rc == 0;
+  }
+  // slots, flags
+  chk(h.slots = h.alloc<double>((size_t)P.n_slot_values));
+  h.n_limited_slots = P.m1_slots + xg::bsim3_eval_slots(P);
+  chk(h.limited_all = h.alloc<int32_t>((size_t)h.n_limited_slots));
+  chk(h.count_buf = h.alloc<int64_t>(64));
+  if (ok) {
+    h.m1.slots = h.slots;
+    h.m1.limited = h.limited_all;
+    xg::bsim3_bind(h.b3, h.slots, h.limited_all + P.m1_slots);
+    if (hipMemset(h.limited_all, 0, sizeof(int32_t) * (h.n_limited_slots ? h.n_limited_slots : 1)) != hipSuccess)
+      ok = false;
+    if (P.n_slot_values &&
+        hipMemset(h.slots, 0, sizeof(double) * P.n_slot_values) != hipSuccess)
+      ok = false;
+  }
+  auto up_prog = [&](const xg::ReduceProgram &rp, xg::ReduceDev &rd) {
+    rd.n_work = (int32_t)rp.tgt.size();
+    chk(rd.tgt = h.upload(rp.tgt));
+    chk(rd.ptr = h.upload(rp.ptr));
+    chk(rd.src = h.upload(rp.src));
+  };
+  for (int k = 0; k < 4; ++k) up_prog(P.red_vec[k], h.red_vec[k]);
+  for (int k = 0; k < 2; ++k) up_prog(P.red_mat[k], h.red_mat[k]);
+  if (!ok) {
+    delete H;
+    return fail(XG_E_NOMEM, "xg_finalize: device allocation/upload failed");
+  }
+  *out = H;
+  return XG_OK;
+}
+
+int xg_finalize_plan(xg_circuit *c, xg_handle **out) {
+  if (!c || !out) return fail(XG_E_ARG, "xg_finalize_plan: bad argument");
+  auto *H = new (std::nothrow) xg_handle;
+  if (!H) return fail(XG_E_NOMEM, "xg_finalize_plan: out of memory");
+  std::string err;
+  int rc = xg::build_plan(c->c, H->h.plan, err);
+  if (rc) {
+    delete H;
+    return fail(rc, err);
+  }
+  H->h.plan_only = true;
+  *out = H;
+  return XG_OK;
+}
+
+void xg_destroy(xg_handle *h) {
+  if (h) {
+    if (!h->h.plan_only) (void)hipDeviceSynchronize();
+    delete h;
+  }
+}
+
+int xg_pattern_size(const xg_handle *h, int32_t *n_rows, int64_t *nnz) {
+  if (!h || !n_rows || !nnz) return fail(XG_E_ARG, "xg_pattern_size");
+  *n_rows = h->h.plan.n_nodes + 1;
+  *nnz = (int64_t)h->h.plan.colidx.size();
+  return XG_OK;
+}
+
+int xg_get_pattern(const xg_handle *h, int32_t *rowptr, int32_t *colidx) {
+  if (!h || !rowptr || !colidx) return fail(XG_E_ARG, "xg_get_pattern");
+  const auto &P = h->h.plan;
+  std::memcpy(rowptr, P.rowptr.data(), P.rowptr.size() * sizeof(int32_t));
+  std::memcpy(colidx, P.colidx.data(), P.colidx.size() * sizeof(int32_t));
+  return XG_OK;
+}
+
+int xg_get_mos1_offsets(const xg_handle *h, int32_t *off) {
+  if (!h || !off) return fail(XG_E_ARG, "xg_get_mos1_offsets");
+  const auto &v = h->h.plan.m1_offsets;
+  std::memcpy(off, v.data(), v.size() * sizeof(int32_t));
+  return XG_OK;
+}
+
+int xg_get_bsim3_offsets(const xg_handle *h, int32_t *off) {
+  if (!h || !off) return fail(XG_E_ARG, "xg_get_bsim3_offsets");
+  return xg::bsim3_get_offsets(h->h.plan, off);
+}
+
+int xg_get_mos1_fields(const xg_handle *h, double *out, int32_t *n_fields) {
+  if (!h || !n_fields) return fail(XG_E_ARG, "xg_get_mos1_fields");
+  *n_fields = xg::M1F_N;
+  if (out) {
+    const auto &v = h->h.plan.m1_fields_ref;
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  }
+  return XG_OK;
+}
+
+int xg_get_bsim3_fields(const xg_handle *h, double *out, int32_t *n_fields) {
+  if (!h || !n_fields) return fail(XG_E_ARG, "xg_get_bsim3_fields");
+  return xg::bsim3_get_fields(h->h.plan, out, n_fields);
+}
+
+int xg_set_solver_state(xg_handle *H, const xg_solver_state *s) {
+  if (!H || !s) return fail(XG_E_ARG, "xg_set_solver_state");
+  auto &d = H->h.ss;
+  d.dcop = s->dcopFlag;
+  d.initJct = s->initJctFlag;
+  d.initFix = s->initFixFlag;
+  d.initTran = s->initTranFlag;
+  d.newtonIter = s->newtonIter;
+  d.tranop = s->tranopFlag;
+  d.acop = s->acopFlag;
+  d.transient = s->transientFlag;
+  return XG_OK;
+}
+
+int xg_set_stream(xg_handle *H, void *stream) {
+  if (!H) return fail(XG_E_ARG, "xg_set_stream");
+  H->h.stream = (hipStream_t)stream;
+  return XG_OK;
+}
+
+int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
+                    const double *sto_curr, double *sta_next,
+                    double *sto_next) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !x_next || !sta_curr || !sto_curr || !sta_next || !sto_next)
+    return fail(XG_E_ARG, "xg_update_state: null pointer");
+  xg::Handle &h = H->h;
+  xg::LoadPtrs p{x_next, sta_curr, sto_curr, sta_next, sto_next};
+  hipError_t e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "mos1 eval");
+  e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
+  return XG_OK;
+}
+
+int xg_load_vectors(xg_handle *H, double *F, double *Q, double *dFdxdVp,
+                    double *dQdxdVp) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !F || !Q || !dFdxdVp || !dQdxdVp)
+    return fail(XG_E_ARG, "xg_load_vectors: null pointer");
+  xg::Handle &h = H->h;
+  double *outs[4] = {F, Q, dFdxdVp, dQdxdVp};
+  hipError_t e = xg::launch_reduce(4, h.red_vec, outs, h.slots, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "vector reduction");
+  return XG_OK;
+}
+
+int xg_load_matrices(xg_handle *H, double *dFdx, double *dQdx) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !dFdx || !dQdx)
+    return fail(XG_E_ARG, "xg_load_matrices: null pointer");
+  xg::Handle &h = H->h;
+  double *outs[2] = {dFdx, dQdx};
+  hipError_t e = xg::launch_reduce(2, h.red_mat, outs, h.slots, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "matrix reduction");
+  return XG_OK;
+}
+
+int xg_load_all(xg_handle *H, const double *x_next, double *sta_curr,
+                const double *sto_curr, double *sta_next, double *sto_next,
+                double *F, double *Q, double *dFdxdVp, double *dQdxdVp,
+                double *dFdx, double *dQdx) {
+  int rc = xg_update_state(H, x_next, sta_curr, sto_curr, sta_next, sto_next);
+  if (rc) return rc;
+  rc = xg_load_vectors(H, F, Q, dFdxdVp, dQdxdVp);
+  if (rc) return rc;
+  return xg_load_matrices(H, dFdx, dQdx);
+}
+
+int xg_get_stats(xg_handle *H, xg_stats *st) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !st) return fail(XG_E_ARG, "xg_get_stats");
+  xg::Handle &h = H->h;
+  st->n_limited = 0;
+  st->n_nonfinite = 0;
+  if (h.n_limited_slots == 0) return XG_OK;
+  hipError_t e = xg::launch_count_flags(h.limited_all, h.n_limited_slots,
+                                        h.count_buf, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "count");
+  int64_t part[64];
+  e = hipMemcpyAsync(part, h.count_buf, sizeof(part), hipMemcpyDeviceToHost,
+                     h.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h.stream);
+  if (e != hipSuccess) return hip_fail(e, "count copy");
+  for (int k = 0; k < 64; ++k) st->n_limited += part[k];
+  return XG_OK;
+}
+
+int xg_sync(xg_handle *H) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H) return fail(XG_E_ARG, "xg_sync");
+  hipError_t e = hipStreamSynchronize(H->h.stream);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "xg_sync");
+}
+
+int xg_sizes(const xg_handle *H, int32_t *n_nodes, int32_t *n_state,
+             int32_t *n_store, int32_t *n_mos1, int32_t *n_bsim3) {
+  if (!H) return fail(XG_E_ARG, "xg_sizes");
+  const auto &P = H->h.plan;
+  if (n_nodes) *n_nodes = P.n_nodes;
+  if (n_state) *n_state = P.n_state;
+  if (n_store) *n_store = P.n_store;
+  if (n_mos1) *n_mos1 = P.m1_n;
+  if (n_bsim3) *n_bsim3 = xg::bsim3_count(P);
+  return XG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Internal declarations shared by the host setup (xg_setup.cpp), the C ABI
+// (xg_api.hip) and the kernels (xg_mos1.hip, xg_bsim3.hip, xg_reduce.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/xg_load.h"
+
+namespace xg {
+
+// ---------------------------------------------------------------------------
+// MOSFET level 1.
+//
+// Card: quantities that depend only on the model and the temperature; one
+// card per (model, temperature) pair.  Staged in LDS by the eval kernel.
+enum Mos1Card : int {
+  M1C_DTYPE = 0, M1C_GAMMA, M1C_LAMBDA, M1C_MJ, M1C_MJSW, M1C_TPHI, M1C_TVBI,
+  M1C_TVTO, M1C_VT, M1C_TDEPCAP, M1C_TBULKPOT, M1C_GMIN, M1C_LIMITER,
+  M1C_MJ_EQ_HALF,    // 1.0 when MJ == 0.5 (sqrt fast path)
+  M1C_MJSW_EQ_HALF,  // 1.0 when MJSW == 0.5
+  M1C_MJ_EQ_MJSW,    // 1.0 when MJ == MJSW
+  M1C_N = 16
+};
+
+// Per-instance fields (structure of arrays, one array per field, indexed by
+// evaluation slot).  These are the reference instance members that
+// updateTemperature / the constructor leave behind and the load reads.
+enum Mos1Field : int {
+  M1F_BETA = 0, M1F_DSATCUR, M1F_SSATCUR, M1F_SVCRIT, M1F_DVCRIT, M1F_CBS,
+  M1F_CBSSW, M1F_CBD, M1F_CBDSW, M1F_F2S, M1F_F3S, M1F_F4S, M1F_F2D, M1F_F3D,
+  M1F_F4D, M1F_OXIDECAP, M1F_GSO, M1F_GDO, M1F_GBO, M1F_GD, M1F_GS, M1F_M,
+  M1F_ICVDS, M1F_ICVGS, M1F_ICVBS,
+  M1F_N = 25
+};
+
+// Per-instance flag bits.
+enum : int32_t { XF_OFF = 1, XF_IC = 2, XF_HAS_DP = 4, XF_HAS_SP = 8 };
+
+// Stamp-value slots written per MOSFET1 instance, in the order the reference
+// Master executes its "+=" statements.  F/Q/limiter entries first, then the
+// 17 dF/dx and 14 dQ/dx entries (N_DEV_MOSFET1.C:4202-4430, 4441-4722).
+enum Mos1Slot : int {
+  M1S_F_D = 0, M1S_F_G, M1S_F_S, M1S_F_B, M1S_F_DP, M1S_F_SP,
+  M1S_Q_G, M1S_Q_B, M1S_Q_DP, M1S_Q_SP,
+  M1S_FL_B, M1S_FL_DP, M1S_FL_SP,
+  M1S_QL_G, M1S_QL_B, M1S_QL_DP, M1S_QL_SP,
+  M1S_JF0,          // 17 dF/dx entries
+  M1S_JQ0 = M1S_JF0 + 17,  // 14 dQ/dx entries
+  M1S_N = M1S_JQ0 + 14
+};
+
+// ---------------------------------------------------------------------------
+// Ordered reduction program for one output array: for work item w,
+//   out[tgt[w]] = ((out[tgt[w]] + s[src[p0]]) + s[src[p0+1]]) + ...
+// with p in [ptr[w], ptr[w+1]).  src == -1 means "+ 0.0" (a structurally
+// zero reference statement whose only effect is to turn -0.0 into +0.0).
+struct ReduceProgram {
+  std::vector<int32_t> tgt;
+  std::vector<int32_t> ptr;
+  std::vector<int32_t> src;
+};
+
+// Device-side view of a reduction program.
+struct ReduceDev {
+  int32_t n_work = 0;
+  int32_t *tgt = nullptr;
+  int32_t *ptr = nullptr;
+  int32_t *src = nullptr;
+};
+
+struct SolverStateDev {
+  int32_t dcop, initJct, initFix, initTran, newtonIter, tranop, acop,
+      transient;
+};
+
+// Arguments of the MOSFET1 fused eval kernel.
+struct Mos1Args {
+  int32_t n_slots;          // padded evaluation slots (multiple of 64)
+  const int32_t *inst_of;   // eval slot -> reference instance index, -1 pad
+  const int32_t *wave_card; // wave index -> card index
+  const double *cards;      // n_cards x M1C_N
+  const double *fields;     // M1F_N x n_slots
+  const int32_t *lid4;      // 4 x n_slots (d, g, s, b)
+  const int32_t *lid_int;   // 2 x n_slots (d', s')
+  const int32_t *sta_base;  // n_slots
+  const int32_t *sto_base;  // n_slots
+  const int32_t *flags;     // n_slots
+  double *slots;            // M1S_N x n_slots
+  int32_t *limited;         // n_slots (0/1)
+};
+
+struct LoadPtrs {
+  const double *x;
+  double *sta_curr;
+  const double *sto_curr;
+  double *sta_next;
+  double *sto_next;
+};
+
+// Kernel launchers (defined in the .hip files).
+hipError_t launch_mos1_eval(const Mos1Args &a, const SolverStateDev &ss,
+                            const LoadPtrs &p, hipStream_t s);
+hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
+                         const double *slots, hipStream_t s);
+hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
+                              hipStream_t s);
+
+}  // namespace xg

@@ -1,0 +1,134 @@
+// Host-side setup: circuit description -> structure-of-arrays device data,
+// fixed CSR pattern, per-instance stamp offsets and the ordered-reduction
+// programs.  Everything here runs once per circuit.
+#pragma once
+
+#include <array>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "xg_internal.h"
+
+namespace xg {
+
+struct ParamSet {
+  std::map<std::string, double> v;  // upper-case name -> value (given only)
+  bool given(const char *k) const { return v.count(k) != 0; }
+  double get(const char *k, double dflt) const {
+    auto it = v.find(k);
+    return it == v.end() ? dflt : it->second;
+  }
+};
+
+struct Options {
+  double gmin = 1.0e-12;
+  int limiter = 1;
+  double temp = 300.15;
+  double tnom = 300.15;
+};
+
+// Processed MOSFET1 model (after processParams).
+struct Mos1Model {
+  int dtype = 1;
+  ParamSet p;
+  // resolved values
+  double model_l, model_w, tnom, latDiff, jctSatCurDensity, jctSatCur, rd, rs,
+      rsh, kp, cgso, cgdo, cgbo, oxideCapFactor, vt0, capBD, capBS, cj, cjsw,
+      pb, mj, mjsw, fc, phi, gamma, lambda, nsub, nss, tox, uo;
+  int tpg;
+  bool capBDGiven, capBSGiven, cjGiven, cjswGiven, rshGiven;
+  double fact1, vtnom, egfet1, pbfact1;
+};
+
+struct Mos1Inst {
+  int model;
+  ParamSet p;
+  std::array<int32_t, 6> lids;  // d g s b d' s'  (-1 = ground)
+  int32_t sta_base, sto_base;
+  // derived
+  double temp;
+  std::array<double, M1F_N> f;
+  int32_t flags;
+  int card;
+  double gd, gs;  // drain / source conductance
+};
+
+struct ForeignStamp {
+  std::vector<int32_t> gids;
+  std::vector<int32_t> rows, cols;
+};
+
+// BSIM3 pieces live in xg_bsim3_setup.cpp.
+struct Bsim3Model;
+struct Bsim3Inst;
+struct Bsim3Plan;
+
+struct Circuit {
+  int32_t n_nodes = 0, n_state = 0, n_store = 0;
+  Options opt;
+  std::vector<Mos1Model> mos1_models;
+  std::vector<Mos1Inst> mos1;
+  std::vector<Bsim3Model *> b3_models;
+  std::vector<Bsim3Inst *> b3;
+  std::vector<ForeignStamp> foreign;
+  bool have_pattern = false;
+  std::vector<int32_t> rowptr, colidx;
+  ~Circuit();
+};
+
+// A device-ready plan (host copies of everything that gets uploaded).
+struct Plan {
+  int32_t n_nodes, n_state, n_store;
+  std::vector<int32_t> rowptr, colidx;
+  // MOSFET1
+  int32_t m1_n = 0, m1_slots = 0;
+  std::vector<int32_t> m1_inst_of, m1_wave_card, m1_lid4, m1_lid_int,
+      m1_sta, m1_sto, m1_flags;
+  std::vector<double> m1_cards, m1_fields;
+  std::vector<int32_t> m1_offsets;  // n x 22, reference order
+  std::vector<double> m1_fields_ref;  // n x M1F_N, reference order
+  int64_t m1_slot_base = 0;
+  // BSIM3
+  Bsim3Plan *b3 = nullptr;
+  // slot buffer and reduction programs
+  int64_t n_slot_values = 0;
+  ReduceProgram red_vec[4];  // F, Q, dFdxdVp, dQdxdVp
+  ReduceProgram red_mat[2];  // dFdx, dQdx
+  ~Plan();
+};
+
+// One "+=" statement of the reference: out[out] += slot[src]
+// (src == -1: "+= 0.0").
+struct Contrib {
+  int32_t out;
+  int32_t src;
+};
+
+int parse_params(int32_t n, const char *const *keys, const double *vals,
+                 ParamSet &out);
+int mos1_model_process(Mos1Model &m, const Options &opt, std::string &err);
+int mos1_instance_setup(const Mos1Model &m, Mos1Inst &in, const Options &opt,
+                        std::string &err);
+int build_plan(Circuit &c, Plan &plan, std::string &err);
+
+// BSIM3 hooks (xg_bsim3_setup.cpp).
+int bsim3_add_model(Circuit &c, int dtype, const ParamSet &p, int32_t *id,
+                    std::string &err);
+int bsim3_add_instance(Circuit &c, int model, const ParamSet &p,
+                       const int32_t lids[6], int32_t sta, int32_t sto,
+                       int32_t *id, std::string &err);
+void bsim3_free(Circuit &c);
+// Instance setup (size-dependent parameters, temperature), cards, checks.
+int bsim3_prepare(Circuit &c, std::string &err);
+// (row, col) of every BSIM3 stamp entry, ground = n_nodes.
+void bsim3_pattern_entries(const Circuit &c,
+                           std::vector<std::array<int32_t, 2>> &entries);
+// Evaluation order, SoA data, offsets and reduction contributions.
+int bsim3_build(Circuit &c, Plan &plan, int64_t slot_base,
+                std::vector<Contrib> *vec, std::vector<Contrib> *mat,
+                std::string &err);
+int64_t bsim3_slot_count(const Plan &plan);
+void bsim3_plan_free(Plan &plan);
+
+}  // namespace xg

@@ -1,0 +1,164 @@
+"""Host-side mirror of the reference's device-load interface over the C ABI.
+
+``DeviceLoad`` plays the role of the MOSFET1/BSIM3 ``Master`` objects behind
+DeviceMgr (src/DeviceModelPKG/Core/src/N_DEV_DeviceMgr.C:2201-2661): it is
+built once from a laid-out Netlist, then per Newton iteration
+
+    set_solver_state(...)                 DeviceMgr::setupSolverInfo_
+    update_state(x, sta_c, sto_c, sta_n, sto_n)   Device::updateState
+    load_vectors(F, Q, dFdxdVp, dQdxdVp)          Device::loadDAEVectors
+    load_matrices(dFdx_vals, dQdx_vals)           Device::loadDAEMatrices
+
+Arguments are torch tensors resident on the GPU (float64, contiguous); torch
+is used only to own device memory and streams.
+"""
+import ctypes
+import numpy as np
+
+from . import _lib
+from ._lib import check, keys_vals, lib
+
+
+def _ptr(t):
+    if not t.is_cuda or str(t.dtype) != "torch.float64" or not t.is_contiguous():
+        raise ValueError("expected a contiguous float64 CUDA tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class DeviceLoad:
+    def __init__(self, nl, gmin=1e-12, limiter=True, temp=300.15, tnom=300.15,
+                 pattern=None, plan_only=False):
+        nl.layout()
+        L = lib()
+        self.nl = nl
+        c = ctypes.c_void_p()
+        check(L.xg_circuit_new(ctypes.byref(c), nl.n_nodes, nl.n_state, nl.n_store),
+              "xg_circuit_new")
+        try:
+            check(L.xg_circuit_set_options(c, gmin, int(limiter), temp, tnom),
+                  "xg_circuit_set_options")
+            mid = {}
+            for name, m in nl.models.items():
+                n, k, v = keys_vals(m.params)
+                out = ctypes.c_int32()
+                fn = L.xg_add_mos1_model if m.kind == "mos1" else L.xg_add_bsim3_model
+                check(fn(c, m.dtype, n, k, v, ctypes.byref(out)), "add model " + name)
+                mid[name] = out.value
+            for rec in nl.mos1:
+                n, k, v = keys_vals(rec["params"])
+                lids = (ctypes.c_int32 * 6)(*rec["lids"])
+                out = ctypes.c_int32()
+                check(L.xg_add_mos1_instance(c, mid[rec["model"]], n, k, v, lids,
+                                             rec["sta"], rec["sto"], ctypes.byref(out)),
+                      "add MOSFET1 " + rec["name"])
+            for rec in nl.bsim3:
+                n, k, v = keys_vals(rec["params"])
+                lids = (ctypes.c_int32 * 6)(*rec["lids"])
+                out = ctypes.c_int32()
+                check(L.xg_add_bsim3_instance(c, mid[rec["model"]], n, k, v, lids,
+                                              rec["sta"], rec["sto"], ctypes.byref(out)),
+                      "add BSIM3 " + rec["name"])
+            for gids, rows, cols in nl.foreign_stamps():
+                g = (ctypes.c_int32 * len(gids))(*gids)
+                r = (ctypes.c_int32 * len(rows))(*rows)
+                cc = (ctypes.c_int32 * len(cols))(*cols)
+                check(L.xg_add_foreign_stamp(c, len(gids), g, len(rows), r, cc),
+                      "add stamp")
+            if pattern is not None:
+                rp, ci = (np.ascontiguousarray(a, dtype=np.int32) for a in pattern)
+                check(L.xg_circuit_set_pattern(
+                    c, rp.ctypes.data_as(_lib.c_int32_p), ci.ctypes.data_as(_lib.c_int32_p)),
+                    "set pattern")
+            h = ctypes.c_void_p()
+            fin = L.xg_finalize_plan if plan_only else L.xg_finalize
+            check(fin(c, ctypes.byref(h)), "xg_finalize")
+        finally:
+            L.xg_circuit_free(c)
+        self.h = h
+        nr = ctypes.c_int32()
+        nnz = ctypes.c_int64()
+        check(L.xg_pattern_size(h, ctypes.byref(nr), ctypes.byref(nnz)), "pattern size")
+        self.n_rows = nr.value
+        self.nnz = nnz.value
+        self.n_mos1 = len(nl.mos1)
+        self.n_bsim3 = len(nl.bsim3)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- setup queries
+    def pattern(self):
+        rp = np.zeros(self.n_rows + 1, np.int32)
+        ci = np.zeros(max(self.nnz, 1), np.int32)
+        check(lib().xg_get_pattern(self.h, rp.ctypes.data_as(_lib.c_int32_p),
+                                   ci.ctypes.data_as(_lib.c_int32_p)), "get pattern")
+        return rp, ci[: self.nnz]
+
+    def mos1_offsets(self):
+        off = np.zeros((max(self.n_mos1, 1), 22), np.int32)
+        check(lib().xg_get_mos1_offsets(self.h, off.ctypes.data_as(_lib.c_int32_p)), "offsets")
+        return off[: self.n_mos1]
+
+    def bsim3_offsets(self):
+        off = np.zeros((max(self.n_bsim3, 1), 24), np.int32)
+        check(lib().xg_get_bsim3_offsets(self.h, off.ctypes.data_as(_lib.c_int32_p)), "offsets")
+        return off[: self.n_bsim3]
+
+    def mos1_fields(self):
+        nf = ctypes.c_int32()
+        check(lib().xg_get_mos1_fields(self.h, None, ctypes.byref(nf)), "fields")
+        out = np.zeros((max(self.n_mos1, 1), nf.value))
+        check(lib().xg_get_mos1_fields(self.h, out.ctypes.data_as(_lib.c_double_p),
+                                       ctypes.byref(nf)), "fields")
+        return out[: self.n_mos1]
+
+    def bsim3_fields(self):
+        nf = ctypes.c_int32()
+        check(lib().xg_get_bsim3_fields(self.h, None, ctypes.byref(nf)), "fields")
+        out = np.zeros((max(self.n_bsim3, 1), max(nf.value, 1)))
+        check(lib().xg_get_bsim3_fields(self.h, out.ctypes.data_as(_lib.c_double_p),
+                                        ctypes.byref(nf)), "fields")
+        return out[: self.n_bsim3, : nf.value]
+
+    # ---- per Newton iteration
+    def set_solver_state(self, dcop=0, initJct=0, initFix=0, initTran=0, newtonIter=0,
+                         tranop=0, acop=0, transient=0):
+        ss = _lib.SolverState(dcop, initJct, initFix, initTran, newtonIter, tranop,
+                              acop, transient)
+        check(lib().xg_set_solver_state(self.h, ctypes.byref(ss)), "set solver state")
+
+    def set_stream(self, stream_handle):
+        check(lib().xg_set_stream(self.h, ctypes.c_void_p(stream_handle)), "set stream")
+
+    def update_state(self, x, sta_curr, sto_curr, sta_next, sto_next):
+        check(lib().xg_update_state(self.h, _ptr(x), _ptr(sta_curr), _ptr(sto_curr),
+                                    _ptr(sta_next), _ptr(sto_next)), "xg_update_state")
+
+    def load_vectors(self, F, Q, dFdxdVp, dQdxdVp):
+        check(lib().xg_load_vectors(self.h, _ptr(F), _ptr(Q), _ptr(dFdxdVp),
+                                    _ptr(dQdxdVp)), "xg_load_vectors")
+
+    def load_matrices(self, dFdx, dQdx):
+        check(lib().xg_load_matrices(self.h, _ptr(dFdx), _ptr(dQdx)), "xg_load_matrices")
+
+    def load_all(self, x, sta_curr, sto_curr, sta_next, sto_next, F, Q, dFdxdVp, dQdxdVp,
+                 dFdx, dQdx):
+        check(lib().xg_load_all(self.h, *[_ptr(t) for t in (
+            x, sta_curr, sto_curr, sta_next, sto_next, F, Q, dFdxdVp, dQdxdVp, dFdx, dQdx)]),
+            "xg_load_all")
+
+    def stats(self):
+        st = _lib.Stats()
+        check(lib().xg_get_stats(self.h, ctypes.byref(st)), "stats")
+        return st.n_limited, st.n_nonfinite
+
+    def sync(self):
+        check(lib().xg_sync(self.h), "xg_sync")
