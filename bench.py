@@ -1,0 +1,202 @@
+"""Device-load benchmark: BSIM3 instance-evaluations per second on one MI355X.
+
+Workload: the seeded random CMOS logic netlist of 5,000 2-input NAND/NOR
+gates (20,000 BSIM3 transistors, W/L within +-20% of 0.5u/0.13u, RC
+interconnect of 1-3 segments per net).  A step is one full device load of
+every BSIM3 instance, as the reference's Newton iteration performs it:
+zero the outputs (the assembler's job), updateState, loadDAEVectors,
+loadDAEMatrices (Master loops of N_DEV_MOSFET_B3.C:9222-9928), at a
+transient iteration (charges, capacitances and limiter terms all live) with
+seeded node voltages from U[-0.2, 1.5] V.  All vectors stay resident in HBM.
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build measures one
+GPU (the load of one circuit is sharded across GPUs by graph partition,
+which is not part of it).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic HBM bytes of one BSIM3 instance evaluation (full load):
+# node voltages 4x8, previous store 5x8, store+state writes 10x8, stamp
+# values 50x8 (F 3, Q 4, limiter 7, dF/dx 18, dQ/dx 16, rounded as counted
+# for the reference data contract), indices (4 nodes + 34 stamp slots) x4,
+# and the per-instance size-dependent + junction record (B3R_N + B3I_N
+# doubles, read once per instance when every instance has its own record).
+B3_SDP_FIELDS = 84
+B3_GEO_FIELDS = 15
+B3_BYTES_PER_EVAL = 32 + 40 + 80 + 400 + 152 + 8 * (B3_SDP_FIELDS + B3_GEO_FIELDS)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--gates", type=int, default=5000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="bound on the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(nl, x, sta, sto, budget_s):
+    """The oracle's restatement of the reference Master loops (array of
+    structs, pointer read-modify-write stamping), serial on one core."""
+    from oracle.oracle import RefLoad
+    ref = RefLoad(nl)
+    ref.set_solver_state(transient=1, newtonIter=1)
+    n = len(nl.bsim3)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        ref.full_load(x, sta.copy(), sto.copy(), np.zeros(nl.n_state), sto.copy())
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    ref.close()
+    return dict(value=n * reps / el, unit="BSIM3 instance-evals/s", cores=1, kind="port",
+                sample="%d full loads of the %d-instance workload in %.1f s, one host core "
+                       "(oracle restatement of the reference Master loops, gcc -O2)"
+                       % (reps, n, el))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; the device load of one circuit would shard by "
+              "graph partition across GPUs, which this build does not implement "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd.loader import DeviceLoad
+
+    nl = circuits.random_logic_bsim3(n_gates=args.gates, seed=2)
+    nl.layout()
+    n_b3 = len(nl.bsim3)
+    n1 = nl.n_nodes + 1
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-0.2, 1.5, n1)
+    x[-1] = 0.0
+    sta = rng.uniform(-1e-14, 1e-14, nl.n_state)
+    sto = rng.uniform(-0.2, 1.5, nl.n_store)
+
+    dev = DeviceLoad(nl)
+    stream = torch.cuda.current_stream()
+    dev.set_stream(stream.cuda_stream)
+    dev.set_solver_state(transient=1, newtonIter=1)
+    cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    X, SC, TC = cu(x), cu(sta), cu(sto)
+    SN, TN = cu(np.zeros(nl.n_state)), cu(sto)
+    vec = torch.zeros(4, n1, dtype=torch.float64, device="cuda")
+    mat = torch.zeros(2, dev.nnz, dtype=torch.float64, device="cuda")
+    F, Q, FL, QL = vec[0], vec[1], vec[2], vec[3]
+    JF, JQ = mat[0], mat[1]
+
+    def step():
+        vec.zero_()
+        mat.zero_()
+        dev.load_all(X, SC, TC, SN, TN, F, Q, FL, QL, JF, JQ)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    def timed(fn, steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(torch.cuda.current_stream())
+        for _ in range(steps):
+            fn()
+        e1.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps, time.perf_counter() - t0
+
+    ms_eager, _ = timed(step, args.steps)
+
+    # The same step captured once in a hipGraph and replayed: every kernel of
+    # the step still runs on every replay; only the per-launch host cost goes.
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        dev.set_stream(torch.cuda.current_stream().cuda_stream)
+        step()
+    dev.set_stream(stream.cuda_stream)
+    for _ in range(args.warmup):
+        graph.replay()
+    ms_step, wall = timed(graph.replay, args.steps)
+
+    # per-kernel timing of the dominant kernel (the fused BSIM3 evaluation)
+    k_ms = []
+    for _ in range(min(args.steps, 50)):
+        vec.zero_()
+        mat.zero_()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        dev.update_state(X, SC, TC, SN, TN)
+        b.record(stream)
+        dev.load_vectors(F, Q, FL, QL)
+        dev.load_matrices(JF, JQ)
+        k_ms.append((a, b))
+    torch.cuda.synchronize()
+    r_ms = []
+    for _ in range(min(args.steps, 50)):
+        vec.zero_()
+        mat.zero_()
+        dev.update_state(X, SC, TC, SN, TN)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        dev.load_vectors(F, Q, FL, QL)
+        dev.load_matrices(JF, JQ)
+        b.record(stream)
+        r_ms.append((a, b))
+    torch.cuda.synchronize()
+    eval_ms = float(np.mean([a.elapsed_time(b) for a, b in k_ms]))
+    reduce_ms = float(np.mean([a.elapsed_time(b) for a, b in r_ms]))
+    achieved = B3_BYTES_PER_EVAL * n_b3 / (eval_ms * 1e-3) / 1e9
+
+    value = n_b3 / (ms_step * 1e-3)
+    out = {
+        "metric": "BSIM3 instance-evals/s",
+        "value": value,
+        "unit": "instance-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "random CMOS logic, %d NAND2/NOR2 gates, %d BSIM3 "
+                               "instances, RC interconnect (seed 2); full device load "
+                               "at a transient Newton iteration" % (args.gates, n_b3),
+                   "n_nodes": nl.n_nodes, "nnz": dev.nnz, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "bsim3_eval_kernel", "kernel_ms": eval_ms,
+                     "bytes_per_eval": B3_BYTES_PER_EVAL},
+        "wall_s_timed_region": wall,
+        "launch": "hipGraph replay of the whole step (zeroing, eval, ordered reduction)",
+        "ms_per_step_eager": ms_eager,
+        "reduction_ms_eager": reduce_ms,
+    }
+    if not args.no_cpu:
+        sta_h, sto_h = sta.copy(), sto.copy()
+        out["cpu_baseline"] = cpu_baseline(nl, x, sta_h, sto_h, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -157,10 +157,11 @@ int xg_get_pattern(const xg_handle *h, int32_t *rowptr, int32_t *colidx);
 
 /* CSR positions of every instance's stamp entries, in the reference's
    registerJacLIDs order (MOSFET1: the 22 offsets of
-   N_DEV_MOSFET1.C:1650-1673, BSIM3: N_DEV_MOSFET_B3.C:3150-3364).
-   Offsets are within the row, as matrixGlobalToLocal produces them. */
+   N_DEV_MOSFET1.C:1650-1673; BSIM3 without IC=/NQSMOD: the same 22 in the
+   same order, N_DEV_MOSFET_B3.C:3150-3330).  Offsets are within the row,
+   as matrixGlobalToLocal produces them. */
 int xg_get_mos1_offsets(const xg_handle *h, int32_t *off /* n_inst x 22 */);
-int xg_get_bsim3_offsets(const xg_handle *h, int32_t *off /* n_inst x 24 */);
+int xg_get_bsim3_offsets(const xg_handle *h, int32_t *off /* n_inst x 22 */);
 
 /* Per-instance temperature/geometry fields the load reads, for checking the
    setup against other implementations (layout: xg_mos1_field_names). */

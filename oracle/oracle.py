@@ -109,15 +109,16 @@ def pattern_and_offsets(nl):
             st[a].append(b)
         rows.append((gids, st))
     rowptr, colidx = stamp_ref.build_pattern(nl.n_nodes, rows)
+    rowmaps = {}
     m1_off = []
     for gids, st, mp, m2 in m1:
-        jl = stamp_ref.stamp_offsets(rowptr, colidx, nl.n_nodes, gids, st)
+        jl = stamp_ref.stamp_offsets(rowptr, colidx, nl.n_nodes, gids, st, rowmaps)
         m1_off.append(stamp_ref.mos1_register_offsets(jl, mp, m2))
     b3_off = []
     if b3:
         from . import bsim3_bind
         for gids, st, mp, m2 in b3:
-            jl = stamp_ref.stamp_offsets(rowptr, colidx, nl.n_nodes, gids, st)
+            jl = stamp_ref.stamp_offsets(rowptr, colidx, nl.n_nodes, gids, st, rowmaps)
             b3_off.append(bsim3_bind.register_offsets(jl, mp, m2))
     return (np.array(rowptr, np.int32), np.array(colidx, np.int32),
             np.array(m1_off, np.int32).reshape(-1, 22),

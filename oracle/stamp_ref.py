@@ -232,14 +232,21 @@ def build_pattern(n_rows, device_rows):
     return rowptr, colidx
 
 
-def stamp_offsets(rowptr, colidx, n_rows, gids, stamp):
-    """Indexor::matrixGlobalToLocal: stamp entry -> offset in its CSR row."""
+def stamp_offsets(rowptr, colidx, n_rows, gids, stamp, rowmaps=None):
+    """Indexor::matrixGlobalToLocal: stamp entry -> offset in its CSR row.
+    ``rowmaps`` (a dict owned by the caller, one per pattern) caches each
+    row's column->offset map, as the reference's accelerated matrix index
+    does."""
     gl = lambda g: n_rows if g == -1 else g
+    if rowmaps is None:
+        rowmaps = {}
     out = []
     for i, row in enumerate(stamp):
         r = gl(gids[i])
-        rc = colidx[rowptr[r]:rowptr[r + 1]]
-        pos = {c: k for k, c in enumerate(rc)}
+        pos = rowmaps.get(r)
+        if pos is None:
+            rc = colidx[rowptr[r]:rowptr[r + 1]]
+            pos = rowmaps[r] = {c: k for k, c in enumerate(rc)}
         out.append([pos[gl(gids[c])] for c in row])
     return out
 

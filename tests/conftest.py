@@ -36,6 +36,27 @@ def assert_close(got, ref, rtol=RTOL, floor_frac=1e-3, what=""):
                                 err.flat[i] / max(denom.flat[i], 1e-300)))
 
 
+def ground_masks(rowptr, colidx, n_nodes):
+    """Boolean keep-masks (vector, CSR values) that drop the ground slot and
+    the ground row and column: the reference writes them but N_LAS discards
+    them before the linear solve (N_LAS_Builder.C:424-434), and the load
+    library does not write them."""
+    vkeep = np.ones(n_nodes + 1, bool)
+    vkeep[n_nodes] = False
+    rows = np.repeat(np.arange(n_nodes + 1), np.diff(rowptr))
+    mkeep = (rows != n_nodes) & (np.asarray(colidx) != n_nodes)
+    return vkeep, mkeep
+
+
+def compare_loads(got, ref, rowptr, colidx, n_nodes):
+    """got/ref = (F, Q, dFdxdVp, dQdxdVp, dFdx, dQdx)."""
+    vkeep, mkeep = ground_masks(rowptr, colidx, n_nodes)
+    names = ["F", "Q", "dFdxdVp", "dQdxdVp", "dFdx", "dQdx"]
+    for k, (nm, a, b) in enumerate(zip(names, got, ref)):
+        keep = vkeep if k < 4 else mkeep
+        assert_close(np.asarray(a)[keep], np.asarray(b)[keep], what=nm)
+
+
 @pytest.fixture(scope="session")
 def oracle_lib():
     from oracle import oracle

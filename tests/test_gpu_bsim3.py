@@ -1,0 +1,136 @@
+"""GPU parity of the BSIM3 load (C ABI on cuda:0) against the oracle.
+
+Every solver state the reference visits in a transient is exercised (DCOP
+with initJct, DCOP iterations with and without charge computation, the
+first transient iteration at initTran, later iterations), on a netlist that
+covers mobMod 1/2/3, the three charge partitions, sheet resistance, IJTH,
+NGATE, binning and off-nominal temperatures, and on the random-logic
+workload.  All outputs and the next state/store vectors are compared with
+the FP64 rule of conftest.assert_close.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_close, compare_loads
+
+pytestmark = pytest.mark.gpu
+
+SOLVER_STATES = [
+    dict(dcop=1, tranop=1, initJct=1, newtonIter=0),
+    dict(dcop=1, tranop=1, newtonIter=1),
+    dict(dcop=1, newtonIter=2),                      # no charge computation
+    dict(transient=1, initTran=1, newtonIter=0),
+    dict(transient=1, newtonIter=0),
+    dict(transient=1, newtonIter=3),
+]
+
+
+def run_pair(nl, ss, seed, torch, vhi=1.5):
+    from oracle.oracle import RefLoad
+    from xyce_amd.loader import DeviceLoad
+    rng = np.random.default_rng(seed)
+    nl.layout()
+    n1 = nl.n_nodes + 1
+    x = rng.uniform(-0.2, vhi, n1)
+    x[-1] = 0.0
+    sta_c = rng.uniform(-1e-14, 1e-14, nl.n_state)
+    sto_c = rng.uniform(-0.5, vhi, nl.n_store)
+    sto_n0 = rng.uniform(-0.5, vhi, nl.n_store)
+    ref = RefLoad(nl)
+    ref.set_solver_state(**ss)
+    sta_c_r = sta_c.copy()
+    sta_n_r = np.zeros(nl.n_state)
+    sto_n_r = sto_n0.copy()
+    out_r = ref.full_load(x, sta_c_r, sto_c.copy(), sta_n_r, sto_n_r)
+    lim_r, _ = ref.b3.flags()
+
+    dev = DeviceLoad(nl)
+    cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    X, SC, TC = cu(x), cu(sta_c), cu(sto_c)
+    SN, TN = cu(np.zeros(nl.n_state)), cu(sto_n0)
+    F, Q, FL, QL = (cu(np.zeros(n1)) for _ in range(4))
+    JF, JQ = cu(np.zeros(dev.nnz)), cu(np.zeros(dev.nnz))
+    dev.set_solver_state(**ss)
+    dev.update_state(X, SC, TC, SN, TN)
+    dev.load_vectors(F, Q, FL, QL)
+    dev.load_matrices(JF, JQ)
+    dev.sync()
+    out_g = [t.cpu().numpy() for t in (F, Q, FL, QL, JF, JQ)]
+    lim_g, _ = dev.stats()
+    st_g = (SN.cpu().numpy(), TN.cpu().numpy(), SC.cpu().numpy())
+    return (out_r, out_g, (sta_n_r, sto_n_r, sta_c_r), st_g, int(lim_r.sum()), lim_g,
+            (ref.rowptr, ref.colidx, nl.n_nodes))
+
+
+def check(out_r, out_g, st_r, st_g, lim_r, lim_g, pat):
+    compare_loads(out_g, out_r, *pat)
+    assert_close(st_g[0], st_r[0], what="next state")
+    assert_close(st_g[1], st_r[1], what="next store")
+    assert_close(st_g[2], st_r[2], what="current state (initTran copy)")
+    assert lim_g == lim_r
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("si", range(len(SOLVER_STATES)))
+def test_bsim3_variety_parity(seed, si):
+    import torch
+    from xyce_amd import circuits
+    nl = circuits.bsim3_variety(seed=seed)
+    ss = SOLVER_STATES[si]
+    print("case variety", seed, ss, flush=True)
+    check(*run_pair(nl, ss, 100 * seed + si, torch))
+
+
+@pytest.mark.parametrize("si", [0, 5])
+def test_bsim3_random_logic_parity(si):
+    import torch
+    from xyce_amd import circuits
+    nl = circuits.random_logic_bsim3(n_gates=300, seed=2)
+    ss = SOLVER_STATES[si]
+    print("case random logic 300 gates", ss, flush=True)
+    check(*run_pair(nl, ss, 7 + si, torch, vhi=1.3))
+
+
+def test_bsim3_random_logic_full_size_parity():
+    """The benchmark workload itself (5,000 gates, 20,000 BSIM3 instances):
+    the supply rail collects contributions from 10,000 devices, which is the
+    wavefront-folded long-segment path of the reduction."""
+    import torch
+    from xyce_amd import circuits
+    nl = circuits.random_logic_bsim3(n_gates=5000, seed=2)
+    print("case random logic 5000 gates", flush=True)
+    check(*run_pair(nl, SOLVER_STATES[5], 11, torch, vhi=1.5))
+
+
+def test_bsim3_monte_carlo_cards_parity():
+    """Per-device model cards (VTH0 / TOX perturbed) share one LDS card per
+    polarity; results still match the oracle's per-model evaluation."""
+    import torch
+    from xyce_amd import circuits
+    nl = circuits.ring_oscillator_bsim3(stages=31, seed=3, dvth0=0.03, dtox=0.02)
+    check(*run_pair(nl, SOLVER_STATES[5], 9, torch, vhi=1.3))
+
+
+def test_bsim3_bitwise_deterministic():
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd.loader import DeviceLoad
+    nl = circuits.bsim3_variety(seed=4)
+    nl.layout()
+    rng = np.random.default_rng(5)
+    n1 = nl.n_nodes + 1
+    cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    x = rng.uniform(-0.2, 1.5, n1)
+    x[-1] = 0
+    dev = DeviceLoad(nl)
+    dev.set_solver_state(transient=1, newtonIter=1)
+    outs = []
+    for _ in range(3):
+        X, SC, TC = cu(x), cu(np.zeros(nl.n_state)), cu(np.zeros(nl.n_store))
+        SN, TN = cu(np.zeros(nl.n_state)), cu(np.ones(nl.n_store))
+        F, Q, FL, QL = (cu(np.zeros(n1)) for _ in range(4))
+        JF, JQ = cu(np.zeros(dev.nnz)), cu(np.zeros(dev.nnz))
+        dev.load_all(X, SC, TC, SN, TN, F, Q, FL, QL, JF, JQ)
+        dev.sync()
+        outs.append(np.concatenate([t.cpu().numpy() for t in (F, Q, FL, QL, JF, JQ, SN, TN)]))
+    assert outs[0].tobytes() == outs[1].tobytes() == outs[2].tobytes()

@@ -9,7 +9,7 @@ is compared with the FP64 parity rule of conftest.assert_close.
 import numpy as np
 import pytest
 
-from conftest import assert_close
+from conftest import assert_close, compare_loads
 
 pytestmark = pytest.mark.gpu
 
@@ -61,7 +61,7 @@ def run_pair(nl, ss, seed, torch):
     lim_g, _ = dev.stats()
     lim_r, _ = ref.mos1_flags()
     return out_r, out_g, (sta_n_r, sto_n_r), (SN.cpu().numpy(), TN.cpu().numpy()), \
-        int(lim_r.sum()), lim_g
+        int(lim_r.sum()), lim_g, (ref.rowptr, ref.colidx, nl.n_nodes)
 
 
 @pytest.mark.parametrize("kind", ["ring", "variety"])
@@ -71,10 +71,8 @@ def test_mos1_load_parity(kind, si):
     nl = _mos1_circuit(kind, seed=11 + si)
     ss = SOLVER_STATES[si]
     print("case", kind, ss, flush=True)
-    out_r, out_g, st_r, st_g, lim_r, lim_g = run_pair(nl, ss, 100 + si, torch)
-    names = ["F", "Q", "dFdxdVp", "dQdxdVp", "dFdx", "dQdx"]
-    for nm, a, b in zip(names, out_g, out_r):
-        assert_close(a, b, what=nm)
+    out_r, out_g, st_r, st_g, lim_r, lim_g, pat = run_pair(nl, ss, 100 + si, torch)
+    compare_loads(out_g, out_r, *pat)
     assert_close(st_g[0], st_r[0], what="state")
     assert_close(st_g[1], st_r[1], what="store")
     assert lim_g == lim_r

@@ -40,9 +40,9 @@ DEVICE = [
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJDIR, src + ".o")
-    cmd = [HIPCC] + COMMON
+    cmd = [HIPCC] + COMMON + DEVICE
     if src.endswith(".hip"):
-        cmd += ["-x", "hip"] + DEVICE
+        cmd += ["-x", "hip"]
     cmd += ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -240,6 +240,170 @@ def mos1_variety(seed=0, n_nodes=24, n_dev=96):
     return nl
 
 
+# 130 nm-class BSIM3v3 cards (level 9): every parameter not listed keeps the
+# BSIM3 default of the model's parameter table.
+BSIM3_NMOS_130 = dict(TOX=3.2e-9, VTH0=0.36, U0=350.0, VSAT=1.1e5, K1=0.52, K2=-0.02,
+                      NCH=2.6e17, XJ=1.0e-7, RDSW=180.0, PRWG=0.1, PRWB=-0.1,
+                      DVT0=2.5, DVT1=0.6, DVT2=-0.03, ETA0=0.06, ETAB=-0.05,
+                      PCLM=1.1, PDIBLC1=0.3, PDIBLC2=0.004, PSCBE1=4.0e8, PSCBE2=1.0e-5,
+                      ALPHA0=1.0e-6, BETA0=25.0, DELTA=0.01, A0=1.2, AGS=0.2, KETA=-0.04,
+                      CJ=1.0e-3, CJSW=2.2e-10, CJSWG=3.0e-10, PB=0.8, PBSW=0.8, MJ=0.42,
+                      MJSW=0.31, CGSO=2.4e-10, CGDO=2.4e-10, CGBO=1.0e-12, CGSL=1.0e-10,
+                      CGDL=1.0e-10, CKAPPA=0.6, LINT=1.0e-8, WINT=5.0e-9, JS=1.0e-6,
+                      JSW=1.0e-12, NFACTOR=1.2, VOFF=-0.09, XPART=0.5)
+BSIM3_PMOS_130 = dict(TOX=3.2e-9, VTH0=-0.34, U0=110.0, VSAT=8.0e4, K1=0.50, K2=-0.01,
+                      NCH=2.2e17, XJ=1.0e-7, RDSW=400.0, PRWG=0.05, PRWB=-0.05,
+                      DVT0=2.2, DVT1=0.55, DVT2=-0.03, ETA0=0.05, ETAB=-0.04,
+                      PCLM=0.9, PDIBLC1=0.25, PDIBLC2=0.003, PSCBE1=4.5e8, PSCBE2=1.0e-5,
+                      ALPHA0=5.0e-7, BETA0=28.0, DELTA=0.01, A0=1.1, AGS=0.25, KETA=-0.03,
+                      CJ=1.1e-3, CJSW=2.0e-10, CJSWG=2.8e-10, PB=0.85, PBSW=0.85, MJ=0.45,
+                      MJSW=0.33, CGSO=2.2e-10, CGDO=2.2e-10, CGBO=1.0e-12, CGSL=1.0e-10,
+                      CGDL=1.0e-10, CKAPPA=0.6, LINT=1.0e-8, WINT=5.0e-9, JS=1.0e-6,
+                      JSW=1.0e-12, NFACTOR=1.3, VOFF=-0.1, XPART=0.5)
+
+
+def _diffusion(w, ext=0.3e-6):
+    """Drain/source diffusion geometry of a transistor of width w."""
+    return dict(AD=w * ext, AS=w * ext, PD=2 * (w + ext), PS=2 * (w + ext))
+
+
+def bsim3_variety(seed=0, n_nodes=30, n_dev=128):
+    """Parity workload covering the BSIM3 options a card can set: mobMod
+    1/2/3, xpart 0 / 0.4 / 0.5 / 1 (the three charge partitions), RSH sheet
+    resistance (internal d'/s'), IJTH junction linearisation, NGATE
+    poly depletion, K1/K2 given vs. derived from NCH/GAMMA1/GAMMA2/VBX,
+    VFB given, length/width binning (L*, W*, P* coefficients), DWG/DWB,
+    TNOM and instance TEMP away from nominal, diffusion perimeters smaller
+    than Weff, OFF, M=, both polarities and grounded terminals."""
+    rng = np.random.default_rng(seed)
+    nl = Netlist("BSIM3 variety")
+    nl.add_model("N1", "bsim3", 1, **BSIM3_NMOS_130)
+    nl.add_model("P1", "bsim3", -1, **BSIM3_PMOS_130)
+    n2 = dict(BSIM3_NMOS_130, MOBMOD=2, XPART=0.0, IJTH=0.05, RSH=5.0, NGATE=5e19,
+              LVTH0=2e-3, WU0=3.0, PVSAT=1e3, DWG=-2e-9, DWB=1e-9, TNOM=35.0)
+    nl.add_model("N2", "bsim3", 1, **n2)
+    p3 = dict(BSIM3_PMOS_130, MOBMOD=3, XPART=1.0, VFB=-0.6, LK1=0.01, WETA0=1e-3,
+              TCJ=1e-3, TPB=1e-3, IJTH=0.2)
+    nl.add_model("P3", "bsim3", -1, **p3)
+    n4 = {k: v for k, v in BSIM3_NMOS_130.items() if k not in ("K1", "K2", "VTH0")}
+    n4.update(GAMMA1=0.6, GAMMA2=0.45, VBX=-1.5, XPART=0.4, CGSO=0.0, CGDO=0.0)
+    nl.add_model("N4", "bsim3", 1, **n4)
+    names = list(nl.models)
+    nodes = ["n%d" % k for k in range(n_nodes)] + ["0"]
+    for i in range(n_dev):
+        d, g, s, b = (nodes[k] for k in rng.integers(0, len(nodes), 4))
+        mdl = names[i % len(names)]
+        w = float(rng.choice([0.4e-6, 0.5e-6, 1.0e-6, float(rng.uniform(0.3e-6, 3e-6))]))
+        l = float(rng.choice([0.13e-6, float(rng.uniform(0.12e-6, 0.5e-6))]))
+        p = dict(W=w, L=l)
+        if rng.random() < 0.7:
+            ext = float(rng.uniform(0.05e-6, 0.4e-6))
+            p.update(_diffusion(w, ext))
+            if rng.random() < 0.3:
+                p.update(PD=float(rng.uniform(0.05e-6, w)), PS=float(rng.uniform(0.05e-6, w)))
+        if rng.random() < 0.3:
+            p.update(NRD=float(rng.uniform(0.5, 3)), NRS=float(rng.uniform(0.5, 3)))
+        if rng.random() < 0.2:
+            p["M"] = float(rng.integers(2, 4))
+        if rng.random() < 0.06:
+            p["OFF"] = 1.0
+        if rng.random() < 0.1:
+            p["TEMP"] = float(rng.choice([50.0, 85.0]))
+        nl.add_mos("M%d" % i, d, g, s, b, mdl, **p)
+    return nl
+
+
+def ring_oscillator_bsim3(stages=11, seed=3, vdd=1.2, dvth0=0.0, dtox=0.0,
+                          rng=None, prefix=""):
+    """Ring oscillator of BSIM3 inverters.  With dvth0 / dtox nonzero every
+    device gets its own model card with VTH0 += N(0, dvth0) and
+    TOX *= U[1 - dtox, 1 + dtox] (a Monte-Carlo sample)."""
+    rng = rng if rng is not None else np.random.default_rng(seed)
+    nl = Netlist("BSIM3 ring oscillator, %d stages" % stages)
+    per_device = dvth0 != 0.0 or dtox != 0.0
+    if not per_device:
+        nl.add_model("NB", "bsim3", 1, **BSIM3_NMOS_130)
+        nl.add_model("PB", "bsim3", -1, **BSIM3_PMOS_130)
+    nl.add_vsrc("VDD", "vdd", "0", vdd)
+    caps = rng.uniform(1e-15, 3e-15, stages)
+    for i in range(stages):
+        inp = "out%d" % ((i - 1) % stages)
+        out = "out%d" % i
+        wn, wp = 0.5e-6, 1.0e-6
+        mn, mp = "NB", "PB"
+        if per_device:
+            mn, mp = "N%s%d" % (prefix, i), "P%s%d" % (prefix, i)
+            for name, base, dt in ((mn, BSIM3_NMOS_130, 1), (mp, BSIM3_PMOS_130, -1)):
+                card = dict(base)
+                card["VTH0"] = base["VTH0"] + float(rng.normal(0.0, dvth0))
+                card["TOX"] = base["TOX"] * float(rng.uniform(1 - dtox, 1 + dtox))
+                nl.add_model(name, "bsim3", dt, **card)
+        nl.add_mos("MN%d" % i, out, inp, "0", "0", mn, W=wn, L=0.13e-6, **_diffusion(wn))
+        nl.add_mos("MP%d" % i, out, inp, "vdd", "vdd", mp, W=wp, L=0.13e-6, **_diffusion(wp))
+        nl.add_cap("C%d" % i, out, "0", caps[i])
+    return nl
+
+
+def random_logic_bsim3(n_gates=5000, seed=2, vdd=1.2, n_inputs=64):
+    """Seeded random CMOS logic: n_gates 2-input NAND / NOR gates (50/50),
+    i.e. 4 * n_gates BSIM3 transistors.  Each gate output fans out to 1-4
+    gate inputs; W and L are drawn independently from U[0.8, 1.2] x
+    (0.5u, 0.13u); every net is an RC line of 1-3 segments with R from
+    U[10, 200] ohm and C from U[1, 20] fF per segment.  Gate inputs that no
+    output reaches are driven by n_inputs primary-input sources."""
+    rng = np.random.default_rng(seed)
+    nl = Netlist("random BSIM3 logic, %d gates" % n_gates)
+    nl.add_model("NB", "bsim3", 1, **BSIM3_NMOS_130)
+    nl.add_model("PB", "bsim3", -1, **BSIM3_PMOS_130)
+    nl.add_vsrc("VDD", "vdd", "0", vdd)
+    for k in range(n_inputs):
+        nl.add_vsrc("VIN%d" % k, "in%d" % k, "0", 0.0,
+                    Pulse(0.0, vdd, float(rng.uniform(0, 1e-9)), 50e-12, 50e-12,
+                          float(rng.uniform(0.3e-9, 1e-9)), float(rng.uniform(1.5e-9, 3e-9))))
+    fanout = rng.integers(1, 5, n_gates)
+    slots = np.repeat(np.arange(n_gates), fanout)
+    rng.shuffle(slots)
+    n_in = 2 * n_gates
+    # driver of every gate input: a gate output, else a primary input
+    drivers = [("g", int(slots[k])) if k < len(slots) else ("i", int(rng.integers(0, n_inputs)))
+               for k in range(n_in)]
+    order = rng.permutation(n_in)
+    drivers = [drivers[k] for k in order]
+    # RC line per gate output net: out -> seg1 -> ... -> end (fan-out point)
+    end_node = {}
+    for gidx in range(n_gates):
+        prev = "g%d" % gidx
+        nseg = int(rng.integers(1, 4))
+        for s in range(nseg):
+            node = "g%d_w%d" % (gidx, s)
+            nl.add_res("R%d_%d" % (gidx, s), prev, node, float(rng.uniform(10, 200)))
+            nl.add_cap("C%d_%d" % (gidx, s), node, "0", float(rng.uniform(1e-15, 20e-15)))
+            prev = node
+        end_node[gidx] = prev
+    for gidx in range(n_gates):
+        a_kind, a_idx = drivers[2 * gidx]
+        b_kind, b_idx = drivers[2 * gidx + 1]
+        a = end_node[a_idx] if a_kind == "g" else "in%d" % a_idx
+        b = end_node[b_idx] if b_kind == "g" else "in%d" % b_idx
+        out, mid = "g%d" % gidx, "g%d_m" % gidx
+        geo = []
+        for _ in range(4):
+            w = 0.5e-6 * float(rng.uniform(0.8, 1.2))
+            l = 0.13e-6 * float(rng.uniform(0.8, 1.2))
+            geo.append(dict(W=w, L=l, **_diffusion(w)))
+        if rng.random() < 0.5:   # NAND2: parallel PMOS, series NMOS
+            nl.add_mos("MPA%d" % gidx, out, a, "vdd", "vdd", "PB", **geo[0])
+            nl.add_mos("MPB%d" % gidx, out, b, "vdd", "vdd", "PB", **geo[1])
+            nl.add_mos("MNA%d" % gidx, out, a, mid, "0", "NB", **geo[2])
+            nl.add_mos("MNB%d" % gidx, mid, b, "0", "0", "NB", **geo[3])
+        else:                    # NOR2: series PMOS, parallel NMOS
+            nl.add_mos("MPA%d" % gidx, mid, a, "vdd", "vdd", "PB", **geo[0])
+            nl.add_mos("MPB%d" % gidx, out, b, mid, "vdd", "PB", **geo[1])
+            nl.add_mos("MNA%d" % gidx, out, a, "0", "0", "NB", **geo[2])
+            nl.add_mos("MNB%d" % gidx, out, b, "0", "0", "NB", **geo[3])
+    return nl
+
+
 def random_node_voltages(nl, seed=0, lo=-0.2, hi=2.0):
     """Solution vector (length n_nodes + 1, ground slot last = 0) with
     node voltages drawn from U[lo, hi] so that every region is exercised."""

@@ -35,6 +35,7 @@ struct Handle {
   // reduction programs
   ReduceDev red_vec[4]{}, red_mat[2]{};
   double *slots = nullptr;
+  double *partial = nullptr;  // chunk partial sums of the reduction
   int64_t *count_buf = nullptr;
   int32_t *limited_all = nullptr;  // m1 then b3 eval slots
   int32_t n_limited_slots = 0;
@@ -278,11 +279,16 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
       ok = false;
   }
   auto up_prog = [&](const xg::ReduceProgram &rp, xg::ReduceDev &rd) {
-    rd.n_work = (int32_t)rp.tgt.size();
-    chk(rd.tgt = h.upload(rp.tgt));
-    chk(rd.ptr = h.upload(rp.ptr));
+    rd.n_chunk = (int32_t)rp.cdst.size();
+    rd.n_multi = (int32_t)rp.mtgt.size();
+    rd.pbase = rp.pbase;
     chk(rd.src = h.upload(rp.src));
+    chk(rd.cptr = h.upload(rp.cptr));
+    chk(rd.cdst = h.upload(rp.cdst));
+    chk(rd.mtgt = h.upload(rp.mtgt));
+    chk(rd.mchunk = h.upload(rp.mchunk));
   };
+  chk(h.partial = h.alloc<double>((size_t)P.n_partials));
   for (int k = 0; k < 4; ++k) up_prog(P.red_vec[k], h.red_vec[k]);
   for (int k = 0; k < 2; ++k) up_prog(P.red_mat[k], h.red_mat[k]);
   if (!ok) {
@@ -401,7 +407,7 @@ int xg_load_vectors(xg_handle *H, double *F, double *Q, double *dFdxdVp,
     return fail(XG_E_ARG, "xg_load_vectors: null pointer");
   xg::Handle &h = H->h;
   double *outs[4] = {F, Q, dFdxdVp, dQdxdVp};
-  hipError_t e = xg::launch_reduce(4, h.red_vec, outs, h.slots, h.stream);
+  hipError_t e = xg::launch_reduce(4, h.red_vec, outs, h.slots, h.partial, h.stream);
   if (e != hipSuccess) return hip_fail(e, "vector reduction");
   return XG_OK;
 }
@@ -413,7 +419,7 @@ int xg_load_matrices(xg_handle *H, double *dFdx, double *dQdx) {
     return fail(XG_E_ARG, "xg_load_matrices: null pointer");
   xg::Handle &h = H->h;
   double *outs[2] = {dFdx, dQdx};
-  hipError_t e = xg::launch_reduce(2, h.red_mat, outs, h.slots, h.stream);
+  hipError_t e = xg::launch_reduce(2, h.red_mat, outs, h.slots, h.partial, h.stream);
   if (e != hipSuccess) return hip_fail(e, "matrix reduction");
   return XG_OK;
 }
@@ -424,9 +430,16 @@ int xg_load_all(xg_handle *H, const double *x_next, double *sta_curr,
                 double *dFdx, double *dQdx) {
   int rc = xg_update_state(H, x_next, sta_curr, sto_curr, sta_next, sto_next);
   if (rc) return rc;
-  rc = xg_load_vectors(H, F, Q, dFdxdVp, dQdxdVp);
-  if (rc) return rc;
-  return xg_load_matrices(H, dFdx, dQdx);
+  if (!F || !Q || !dFdxdVp || !dQdxdVp || !dFdx || !dQdx)
+    return fail(XG_E_ARG, "xg_load_all: null pointer");
+  // vector and matrix folds in one reduction pass
+  xg::Handle &h = H->h;
+  xg::ReduceDev progs[6] = {h.red_vec[0], h.red_vec[1], h.red_vec[2],
+                            h.red_vec[3], h.red_mat[0], h.red_mat[1]};
+  double *outs[6] = {F, Q, dFdxdVp, dQdxdVp, dFdx, dQdx};
+  hipError_t e = xg::launch_reduce(6, progs, outs, h.slots, h.partial, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "reduction");
+  return XG_OK;
 }
 
 int xg_get_stats(xg_handle *H, xg_stats *st) {

@@ -11,16 +11,19 @@ namespace xg {
 struct Plan;
 
 struct Bsim3Dev {
-  int32_t n_slots = 0;         // padded evaluation slots (multiple of 64)
-  const int32_t *inst_of = nullptr;
+  int32_t n_slots = 0;              // padded evaluation slots (multiple of 64)
+  int32_t n_sdp = 0;                // size-dependent records
+  const int32_t *inst_of = nullptr; // eval slot -> instance, -1 = padding
   const int32_t *wave_card = nullptr;
   const double *cards = nullptr;    // n_cards x B3C_N
-  const double *fields = nullptr;   // B3F_N x n_slots
+  const double *sdp = nullptr;      // B3R_N x n_sdp
+  const double *geo = nullptr;      // B3I_N x n_slots
   const int32_t *lid4 = nullptr;    // 4 x n_slots
   const int32_t *lid_int = nullptr; // 2 x n_slots
   const int32_t *sta_base = nullptr;
   const int32_t *sto_base = nullptr;
   const int32_t *flags = nullptr;
+  const int32_t *sdp_idx = nullptr; // n_slots
   double *slots = nullptr;          // B3S_N x n_slots (inside the shared buffer)
   int32_t *limited = nullptr;
   int64_t slot_base = 0;

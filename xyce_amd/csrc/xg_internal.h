@@ -54,22 +54,33 @@ enum Mos1Slot : int {
 };
 
 // ---------------------------------------------------------------------------
-// Ordered reduction program for one output array: for work item w,
-//   out[tgt[w]] = ((out[tgt[w]] + s[src[p0]]) + s[src[p0+1]]) + ...
-// with p in [ptr[w], ptr[w+1]).  src == -1 means "+ 0.0" (a structurally
-// zero reference statement whose only effect is to turn -0.0 into +0.0).
+// Ordered reduction program for one output array.  The sources of each
+// output entry (slot indices in the reference's accumulation order; -1
+// means "+ 0.0", a structurally zero reference statement whose only effect
+// is to turn -0.0 into +0.0) are stored contiguously in src and cut into
+// chunks of at most kChunk.  Chunk c covers src[cptr[c] .. cptr[c+1]); if it
+// is the whole segment of entry e then cdst[c] = e, else -1 and its sum
+// goes to partial[pbase + c].  Multi-chunk entry i (mtgt[i]) owns chunks
+// [mchunk[2i], mchunk[2i+1]).
+constexpr int32_t kChunk = 32;
+
 struct ReduceProgram {
-  std::vector<int32_t> tgt;
-  std::vector<int32_t> ptr;
   std::vector<int32_t> src;
+  std::vector<int32_t> cptr, cdst;
+  std::vector<int32_t> mtgt, mchunk;
+  int32_t pbase = 0;
 };
 
 // Device-side view of a reduction program.
 struct ReduceDev {
-  int32_t n_work = 0;
-  int32_t *tgt = nullptr;
-  int32_t *ptr = nullptr;
+  int32_t n_chunk = 0;
+  int32_t n_multi = 0;
+  int32_t pbase = 0;
   int32_t *src = nullptr;
+  int32_t *cptr = nullptr;
+  int32_t *cdst = nullptr;
+  int32_t *mtgt = nullptr;
+  int32_t *mchunk = nullptr;
 };
 
 struct SolverStateDev {
@@ -105,7 +116,7 @@ struct LoadPtrs {
 hipError_t launch_mos1_eval(const Mos1Args &a, const SolverStateDev &ss,
                             const LoadPtrs &p, hipStream_t s);
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
-                         const double *slots, hipStream_t s);
+                         const double *slots, double *partial, hipStream_t s);
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s);
 

@@ -342,23 +342,39 @@ static inline int32_t mos1_node(const Mos1Inst &in, int local) {
 // ---------------------------------------------------------------------------
 // Reduction-program assembly.
 
+// Groups the contributions by output entry, keeping the reference's order
+// within each entry.  Entries flagged in `skip` (the ground row and column:
+// the reference writes them into the ground slot that N_LAS discards before
+// the solve, N_LAS_Builder.C:424-434) get no work item.
 static void finish_program(const std::vector<Contrib> &cs, int32_t n_out,
+                           const std::vector<char> &skip,
                            ReduceProgram &prog) {
-  // Stable bucket by output index: keeps the reference's insertion order.
   std::vector<int32_t> cnt(n_out + 1, 0);
   for (auto &c : cs) cnt[c.out + 1]++;
   for (int32_t i = 0; i < n_out; ++i) cnt[i + 1] += cnt[i];
   std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
   std::vector<int32_t> srcs(cs.size());
   for (auto &c : cs) srcs[pos[c.out]++] = c.src;
-  prog.tgt.clear();
-  prog.ptr.assign(1, 0);
   prog.src.clear();
+  prog.cptr.assign(1, 0);
+  prog.cdst.clear();
+  prog.mtgt.clear();
+  prog.mchunk.clear();
   for (int32_t k = 0; k < n_out; ++k) {
-    if (cnt[k + 1] == cnt[k]) continue;
-    prog.tgt.push_back(k);
-    for (int32_t j = cnt[k]; j < cnt[k + 1]; ++j) prog.src.push_back(srcs[j]);
-    prog.ptr.push_back((int32_t)prog.src.size());
+    const int32_t len = cnt[k + 1] - cnt[k];
+    if (len == 0 || skip[k]) continue;
+    const int32_t first_chunk = (int32_t)prog.cdst.size();
+    for (int32_t j = cnt[k]; j < cnt[k + 1]; j += kChunk) {
+      const int32_t e = std::min(cnt[k + 1], j + kChunk);
+      for (int32_t q = j; q < e; ++q) prog.src.push_back(srcs[q]);
+      prog.cptr.push_back((int32_t)prog.src.size());
+      prog.cdst.push_back(len <= kChunk ? k : -1);
+    }
+    if (len > kChunk) {
+      prog.mtgt.push_back(k);
+      prog.mchunk.push_back(first_chunk);
+      prog.mchunk.push_back((int32_t)prog.cdst.size());
+    }
   }
 }
 
@@ -573,9 +589,22 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     err = "slot buffer exceeds 2^31 values";
     return XG_E_ARG;
   }
-  for (int k = 0; k < 4; ++k) finish_program(vec[k], n + 1, plan.red_vec[k]);
+  std::vector<char> vskip(n + 1, 0);
+  vskip[G] = 1;
+  for (int k = 0; k < 4; ++k) finish_program(vec[k], n + 1, vskip, plan.red_vec[k]);
   const int32_t nnz = (int32_t)plan.colidx.size();
-  for (int k = 0; k < 2; ++k) finish_program(mat[k], nnz, plan.red_mat[k]);
+  std::vector<char> mskip(nnz, 0);
+  for (int32_t r = 0; r <= n; ++r)
+    for (int32_t j = plan.rowptr[r]; j < plan.rowptr[r + 1]; ++j)
+      mskip[j] = (r == G || plan.colidx[j] == G) ? 1 : 0;
+  for (int k = 0; k < 2; ++k) finish_program(mat[k], nnz, mskip, plan.red_mat[k]);
+  int32_t pbase = 0;
+  for (auto *prog : {&plan.red_vec[0], &plan.red_vec[1], &plan.red_vec[2],
+                     &plan.red_vec[3], &plan.red_mat[0], &plan.red_mat[1]}) {
+    prog->pbase = pbase;
+    pbase += (int32_t)prog->cdst.size();
+  }
+  plan.n_partials = pbase;
   return XG_OK;
 }
 

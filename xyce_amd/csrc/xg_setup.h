@@ -95,6 +95,7 @@ struct Plan {
   int64_t n_slot_values = 0;
   ReduceProgram red_vec[4];  // F, Q, dFdxdVp, dQdxdVp
   ReduceProgram red_mat[2];  // dFdx, dQdx
+  int32_t n_partials = 0;    // chunk partial sums over all six programs
   ~Plan();
 };
 

@@ -108,7 +108,7 @@ class DeviceLoad:
         return off[: self.n_mos1]
 
     def bsim3_offsets(self):
-        off = np.zeros((max(self.n_bsim3, 1), 24), np.int32)
+        off = np.zeros((max(self.n_bsim3, 1), 22), np.int32)
         check(lib().xg_get_bsim3_offsets(self.h, off.ctypes.data_as(_lib.c_int32_p)), "offsets")
         return off[: self.n_bsim3]
 
