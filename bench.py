@@ -34,6 +34,11 @@ B3_SDP_FIELDS = 84
 B3_GEO_FIELDS = 15
 B3_BYTES_PER_EVAL = 32 + 40 + 80 + 400 + 152 + 8 * (B3_SDP_FIELDS + B3_GEO_FIELDS)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per bsim3_eval_kernel launch on this workload, from the FETCH_SIZE
+# and WRITE_SIZE counter passes scaled by a plain-copy calibration
+# (profiles/bsim3_device_load_random_logic_5000g_hbm_traffic.md); valid for
+# the default --gates only.
+B3_EVAL_TRAFFIC_BYTES = 38.81e6
 
 
 def parse():
@@ -45,7 +50,38 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches only (for counter collection)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 512 MiB device copy, the reference kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def copy_calibration(torch):
+    """A plain device-to-device copy of known size (512 MiB read, 512 MiB
+    written), three times; its counter readings give the KiB the profiler
+    reports per real HBM byte."""
+    src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def graph_timing(torch, dev, stream, step, timed, args):
+    """The same step captured once in a hipGraph and replayed: every kernel
+    of the step still runs on every replay; only the per-launch host cost
+    goes."""
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        dev.set_stream(torch.cuda.current_stream().cuda_stream)
+        step()
+    dev.set_stream(stream.cuda_stream)
+    for _ in range(args.warmup):
+        graph.replay()
+    return timed(graph.replay, args.steps)
 
 
 def cpu_baseline(nl, x, sta, sto, budget_s):
@@ -98,14 +134,17 @@ def main():
     cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
     X, SC, TC = cu(x), cu(sta), cu(sto)
     SN, TN = cu(np.zeros(nl.n_state)), cu(sto)
-    vec = torch.zeros(4, n1, dtype=torch.float64, device="cuda")
-    mat = torch.zeros(2, dev.nnz, dtype=torch.float64, device="cuda")
-    F, Q, FL, QL = vec[0], vec[1], vec[2], vec[3]
-    JF, JQ = mat[0], mat[1]
+    # F, Q, dFdxdVp, dQdxdVp and the dF/dx, dQ/dx CSR values in one buffer,
+    # zeroed by one fill (the assembler's job before every load).
+    out = torch.zeros(4 * n1 + 2 * dev.nnz, dtype=torch.float64, device="cuda")
+    F, Q, FL, QL = (out[i * n1:(i + 1) * n1] for i in range(4))
+    JF = out[4 * n1:4 * n1 + dev.nnz]
+    JQ = out[4 * n1 + dev.nnz:]
+    if args.copy_calib:
+        copy_calibration(torch)
 
     def step():
-        vec.zero_()
-        mat.zero_()
+        out.zero_()
         dev.load_all(X, SC, TC, SN, TN, F, Q, FL, QL, JF, JQ)
 
     for _ in range(args.warmup):
@@ -123,50 +162,40 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps, time.perf_counter() - t0
 
-    ms_eager, _ = timed(step, args.steps)
+    ms_eager, wall_eager = timed(step, args.steps)
+    ms_graph, wall_graph = float("inf"), None
+    if not args.no_graph:
+        ms_graph, wall_graph = graph_timing(torch, dev, stream, step, timed, args)
+    # report the faster of the two launch modes; both run the whole step
+    if ms_graph < ms_eager:
+        ms_step, wall, launch = ms_graph, wall_graph, "hipGraph replay of the whole step"
+    else:
+        ms_step, wall, launch = ms_eager, wall_eager, "eager stream launches"
 
-    # The same step captured once in a hipGraph and replayed: every kernel of
-    # the step still runs on every replay; only the per-launch host cost goes.
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph):
-        dev.set_stream(torch.cuda.current_stream().cuda_stream)
-        step()
-    dev.set_stream(stream.cuda_stream)
-    for _ in range(args.warmup):
-        graph.replay()
-    ms_step, wall = timed(graph.replay, args.steps)
-
-    # per-kernel timing of the dominant kernel (the fused BSIM3 evaluation)
-    k_ms = []
-    for _ in range(min(args.steps, 50)):
-        vec.zero_()
-        mat.zero_()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
+    # Kernel durations: back-to-back launches of one kernel between two
+    # events on the library's stream, so the figure is the kernel's own
+    # average duration (the inputs are unchanged between launches, so every
+    # launch does the same work).
+    n_k = min(args.steps, 50)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(n_k):
         dev.update_state(X, SC, TC, SN, TN)
-        b.record(stream)
+    b.record(stream)
+    torch.cuda.synchronize()
+    eval_ms = a.elapsed_time(b) / n_k
+    a.record(stream)
+    for _ in range(n_k):
         dev.load_vectors(F, Q, FL, QL)
         dev.load_matrices(JF, JQ)
-        k_ms.append((a, b))
+    b.record(stream)
     torch.cuda.synchronize()
-    r_ms = []
-    for _ in range(min(args.steps, 50)):
-        vec.zero_()
-        mat.zero_()
-        dev.update_state(X, SC, TC, SN, TN)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        dev.load_vectors(F, Q, FL, QL)
-        dev.load_matrices(JF, JQ)
-        b.record(stream)
-        r_ms.append((a, b))
-    torch.cuda.synchronize()
-    eval_ms = float(np.mean([a.elapsed_time(b) for a, b in k_ms]))
-    reduce_ms = float(np.mean([a.elapsed_time(b) for a, b in r_ms]))
+    reduce_ms = a.elapsed_time(b) / n_k
     achieved = B3_BYTES_PER_EVAL * n_b3 / (eval_ms * 1e-3) / 1e9
 
     value = n_b3 / (ms_step * 1e-3)
-    out = {
+    res = {
         "metric": "BSIM3 instance-evals/s",
         "value": value,
         "unit": "instance-evals/s",
@@ -184,18 +213,20 @@ def main():
                                "at a transient Newton iteration" % (args.gates, n_b3),
                    "n_nodes": nl.n_nodes, "nnz": dev.nnz, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": B3_EVAL_TRAFFIC_BYTES if args.gates == 5000 else None,
                      "kernel": "bsim3_eval_kernel", "kernel_ms": eval_ms,
                      "bytes_per_eval": B3_BYTES_PER_EVAL},
         "wall_s_timed_region": wall,
-        "launch": "hipGraph replay of the whole step (zeroing, eval, ordered reduction)",
+        "launch": launch,
         "ms_per_step_eager": ms_eager,
-        "reduction_ms_eager": reduce_ms,
+        "ms_per_step_graph": ms_graph,
+        "reduction_ms": reduce_ms,
     }
     if not args.no_cpu:
         sta_h, sto_h = sta.copy(), sto.copy()
-        out["cpu_baseline"] = cpu_baseline(nl, x, sta_h, sto_h, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+        res["cpu_baseline"] = cpu_baseline(nl, x, sta_h, sto_h, args.cpu_seconds)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

@@ -182,7 +182,8 @@ ref_mos1_model *ref_mos1_model_new(int dtype, int n, const char **keys,
     else if (!strcmp(k, "FC")) m->fwdCapDepCoeff = v;
     else if (!strcmp(k, "NSUB")) { m->substrateDoping = v; m->g_NSUB = 1; }
     else if (!strcmp(k, "NSS")) { m->surfaceStateDensity = v; m->g_NSS = 1; }
-    else if (!strcmp(k, "TNOM")) { m->tnom = v; m->g_TNOM = 1; }
+    /* TNOM and TEMP are given in Celsius: N_DEV_DeviceEntity.C:863-866 */
+    else if (!strcmp(k, "TNOM")) { m->tnom = v + 273.15; m->g_TNOM = 1; }
     else if (!strcmp(k, "TPG")) { m->gateType = (int)v; m->g_TPG = 1; }
     else { free(m); return NULL; }
   }
@@ -333,6 +334,7 @@ int ref_mos1_add_instance(ref_mos1_master *M, ref_mos1_model *m, int n,
     else if (!strcmp(k, "IC1")) { in->icVDS = v; in->IC_GIVEN = 1; }
     else if (!strcmp(k, "IC2")) { in->icVGS = v; in->IC_GIVEN = 1; }
     else if (!strcmp(k, "IC3")) { in->icVBS = v; in->IC_GIVEN = 1; }
+    else if (!strcmp(k, "TEMP")) in->temp = v + 273.15;  /* Celsius, as TNOM */
     else return -1;
   }
   if (!gL) in->l = m->model_l;

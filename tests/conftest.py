@@ -57,6 +57,16 @@ def compare_loads(got, ref, rowptr, colidx, n_nodes):
         assert_close(np.asarray(a)[keep], np.asarray(b)[keep], what=nm)
 
 
+@pytest.fixture(scope="session", autouse=True)
+def built_library():
+    """The load library is built in-tree; build it once if a fresh checkout
+    runs the tests before any build step did."""
+    from xyce_amd import build
+    if not os.path.exists(build.LIB):
+        build.build()
+    return build.LIB
+
+
 @pytest.fixture(scope="session")
 def oracle_lib():
     from oracle import oracle

@@ -1,0 +1,242 @@
+"""Pins of the oracle (the CPU restatement of the reference load) that do not
+depend on the oracle itself: closed forms, conservation laws, symmetry and
+the reference's own numerical-Jacobian check.
+
+* Shichman-Hodges drain current, gm and gds in cutoff, linear and saturation
+  (N_DEV_MOSFET1.C:2972-3011) and the junction diode of SPICE3 level 1
+  (N_DEV_MOSFET1.C:2848-2871): I = Is(e^{V/vt}-1) + gmin V forward, and the
+  linearised I = (Is/vt) V + gmin V for V <= 0.
+* KCL: a device's F (and Q) entries over its terminals sum to zero, and the
+  columns of its dF/dx (dQ/dx) block sum to zero (current leaving one node
+  enters another); rows sum to zero (only voltage differences matter).
+* NMOS/PMOS mirror through the device type: with every sign-carrying card
+  value negated, F(-x) = -F(x) and dF/dx(-x) = dF/dx(x) (state and store
+  vectors are kept in the type-normalised frame, so they carry over).
+* Analytic dF/dx and dQ/dx against central differences of F and Q, in the
+  error form of the reference's Jacobian test (N_DEV_NumericalJacobian.C:
+  407-477: |a - n| <= relTol |n| + absTol), with tighter tolerances than its
+  defaults (relTol 0.01, absTol 1e-8; N_DEV_DeviceOptions.C:95-97).
+"""
+import numpy as np
+import pytest
+
+from xyce_amd import circuits
+from xyce_amd.circuits import Netlist
+
+
+def dense(rowptr, colidx, vals, n):
+    J = np.zeros((n, n))
+    for r in range(n):
+        for k in range(rowptr[r], rowptr[r + 1]):
+            J[r, colidx[k]] += vals[k]
+    return J
+
+
+def single(kind, dtype, card, inst, nodes=("d", "g", "s", "b")):
+    nl = Netlist("one %s" % kind)
+    nl.add_model("M", kind, dtype, **card)
+    nl.add_mos("X1", *nodes, "M", **inst)
+    nl.layout()
+    return nl
+
+
+def load(nl, x, ss, gmin=1e-12, limiter=True, sto=None, sta=None):
+    from oracle.oracle import RefLoad
+    ref = RefLoad(nl, gmin=gmin, limiter=limiter)
+    ref.set_solver_state(**ss)
+    sta = np.zeros(nl.n_state) if sta is None else sta.copy()
+    sto = np.zeros(nl.n_store) if sto is None else sto.copy()
+    out = ref.full_load(x, sta, sto, np.zeros(nl.n_state), sto.copy())
+    n1 = nl.n_nodes + 1
+    JF = dense(ref.rowptr, ref.colidx, out[4], n1)
+    JQ = dense(ref.rowptr, ref.colidx, out[5], n1)
+    ref.close()
+    return out[0], out[1], out[2], out[3], JF, JQ
+
+
+def xvec(nl, volts):
+    """Solution vector with the named terminal voltages (ground slot 0)."""
+    x = np.zeros(nl.n_nodes + 1)
+    for name, v in volts.items():
+        x[nl.node(name)] = v
+    return x
+
+
+DC = dict(dcop=1, newtonIter=1)
+TRAN = dict(transient=1, newtonIter=2)
+
+
+# ---------------------------------------------------------------- level 1
+SH_CARD = dict(VTO=0.5, KP=100e-6, LD=1e-8, IS=0.0)   # GAMMA = LAMBDA = 0, no junctions
+SH_INST = dict(W=2e-6, L=0.5e-6)
+
+
+def _sh(vgs, vds, vto=0.5, kp=100e-6, w=2e-6, l=0.5e-6, ld=1e-8):
+    beta = kp * w / (l - 2 * ld)
+    vgst = vgs - vto
+    if vgst <= 0:
+        return 0.0, 0.0, 0.0
+    if vds < vgst:
+        return beta * vds * (vgst - 0.5 * vds), beta * vds, beta * (vgst - vds)
+    return 0.5 * beta * vgst ** 2, beta * vgst, 0.0
+
+
+@pytest.mark.parametrize("vgs,vds", [(1.2, 1.5), (1.5, 0.3), (0.9, 0.05), (0.3, 1.0),
+                                     (1.8, 1.8), (0.7, 0.1)])
+def test_mos1_shichman_hodges(oracle_lib, vgs, vds):
+    nl = single("mos1", 1, SH_CARD, SH_INST)
+    x = xvec(nl, dict(d=vds, g=vgs, s=0.0, b=0.0))
+    F, Q, _, _, JF, _ = load(nl, x, DC, gmin=0.0, limiter=False)
+    ids, gm, gds = _sh(vgs, vds)
+    d, g, s = nl.node("d"), nl.node("g"), nl.node("s")
+    assert abs(F[d] - ids) <= 1e-13 * max(abs(ids), 1e-6)
+    assert abs(F[s] + ids) <= 1e-13 * max(abs(ids), 1e-6)
+    assert abs(JF[d, g] - gm) <= 1e-13 * max(gm, 1e-9)
+    assert abs(JF[d, d] - gds) <= 1e-13 * max(gds, 1e-9)
+    assert abs(JF[d, s] + gm + gds) <= 1e-13 * max(gm + gds, 1e-9)
+    assert F[g] == 0.0 and F[nl.node("b")] == 0.0
+
+
+@pytest.mark.parametrize("vbs", [-1.0, -0.2, 0.1, 0.45, 0.6])
+def test_mos1_junction_diode(oracle_lib, vbs):
+    """Bulk-source diode with the channel off: F[b] = Is(e^{vbs/vt}-1) + gmin vbs
+    (plus the reverse-biased bulk-drain diode)."""
+    card = dict(VTO=5.0, KP=1e-6, IS=1e-14)
+    nl = single("mos1", 1, card, dict(W=1e-6, L=1e-6))
+    gmin = 1e-12
+    x = xvec(nl, dict(d=0.0, g=0.0, s=0.0, b=vbs))
+    F, _, _, _, JF, _ = load(nl, x, DC, gmin=gmin, limiter=False)
+    vt = 300.15 * 1.3806226e-23 / 1.6021918e-19
+    if vbs > 0:
+        i1 = 1e-14 * np.expm1(vbs / vt) + gmin * vbs
+        g1 = 1e-14 / vt * np.exp(vbs / vt) + gmin
+    else:
+        i1 = (1e-14 / vt + gmin) * vbs
+        g1 = 1e-14 / vt + gmin
+    # bulk-source and bulk-drain diodes see the same voltage here
+    b = nl.node("b")
+    assert abs(F[b] - 2 * i1) <= 1e-12 * abs(2 * i1)
+    assert abs(JF[b, b] - 2 * g1) <= 1e-12 * 2 * g1
+
+
+# ---------------------------------------------------------------- invariants
+def _cases():
+    mos1 = dict(VTO=0.45, KP=110e-6, GAMMA=0.4, LAMBDA=0.05, PHI=0.65, TOX=1.5e-8,
+                CGSO=2e-10, CGDO=2e-10, CGBO=1e-10, CJ=2e-4, CJSW=1e-9, JS=1e-4,
+                RD=10.0, RS=8.0)
+    mos1_i = dict(W=2e-6, L=0.3e-6, AD=1e-12, AS=1e-12, PD=4e-6, PS=4e-6)
+    b3 = dict(circuits.BSIM3_NMOS_130, RSH=5.0)
+    w = 1e-6
+    b3_i = dict(W=w, L=0.13e-6, **circuits._diffusion(w), NRD=1.0, NRS=1.5)
+    return [("mos1", mos1, mos1_i), ("bsim3", b3, b3_i),
+            ("bsim3", dict(circuits.BSIM3_NMOS_130, XPART=0.0, MOBMOD=3),
+             dict(W=w, L=0.2e-6, **circuits._diffusion(w)))]
+
+
+CASES = _cases()
+
+
+def _rand_x(nl, rng, lo=-0.3, hi=1.5):
+    x = rng.uniform(lo, hi, nl.n_nodes + 1)
+    x[-1] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+@pytest.mark.parametrize("ss", [DC, TRAN], ids=["dc", "tran"])
+def test_kcl_and_translation(oracle_lib, ci, ss):
+    kind, card, inst = CASES[ci]
+    nl = single(kind, 1, card, inst)
+    rng = np.random.default_rng(ci)
+    for _ in range(8):
+        x = _rand_x(nl, rng)
+        sto = rng.uniform(-0.3, 1.5, nl.n_store)
+        F, Q, FL, QL, JF, JQ = load(nl, x, ss, sto=sto)
+        vecs = [F, Q, FL, QL]
+        if kind == "mos1":
+            # The reference's MOSFET1 dQdxdVp bulk row adds gcgb*(vbd -
+            # vbd_orig) where the drain' row subtracts gcbd*(vbd - vbd_orig)
+            # (N_DEV_MOSFET1.C:1987-1995), so that vector is not conservative
+            # in the reference either; the restatement keeps the statement.
+            vecs = vecs[:3]
+        for v in vecs:
+            assert abs(v.sum()) <= 1e-12 * max(np.abs(v).max(), 1e-300)
+        for J in (JF, JQ):
+            sc = max(np.abs(J).max(), 1e-300)
+            assert np.all(np.abs(J.sum(axis=0)) <= 1e-11 * sc)
+            assert np.all(np.abs(J.sum(axis=1)) <= 1e-11 * sc)
+
+
+def _mirror(card):
+    out = dict(card)
+    for k in ("VTO", "VTH0", "VFB"):
+        if k in out:
+            out[k] = -out[k]
+    return out
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_nmos_pmos_mirror(oracle_lib, ci):
+    kind, card, inst = CASES[ci]
+    nn = single(kind, 1, card, inst)
+    pp = single(kind, -1, _mirror(card), inst)
+    rng = np.random.default_rng(40 + ci)
+    for ss in (DC, TRAN):
+        x = _rand_x(nn, rng)
+        sto = rng.uniform(-0.3, 1.5, nn.n_store)
+        sta = rng.uniform(-1e-15, 1e-15, nn.n_state)
+        a = load(nn, x, ss, sto=sto, sta=sta)
+        # state and store hold voltages and charges in the device-type
+        # normalised frame (multiplied by the type), so they carry over as is
+        b = load(pp, -x, ss, sto=sto, sta=sta)
+        for k in range(4):
+            np.testing.assert_allclose(b[k], -a[k], rtol=1e-13, atol=1e-13 * np.abs(a[k]).max())
+        for k in (4, 5):
+            np.testing.assert_allclose(b[k], a[k], rtol=1e-13, atol=1e-13 * np.abs(a[k]).max())
+
+
+def _fd_check(nl, x, ss, what, rel, absf, sto=None, rel_q=None):
+    F0, Q0, _, _, JF, JQ = load(nl, x, ss, limiter=False, sto=sto)
+    n = nl.n_nodes
+    for c in range(n):
+        h = 1e-6 * (1.0 + abs(x[c]))
+        xp, xm = x.copy(), x.copy()
+        xp[c] += h
+        xm[c] -= h
+        Fp, Qp, *_ = load(nl, xp, ss, limiter=False, sto=sto)
+        Fm, Qm, *_ = load(nl, xm, ss, limiter=False, sto=sto)
+        for nm, vp, vm, J in (("F", Fp, Fm, JF), ("Q", Qp, Qm, JQ)):
+            if nm not in what:
+                continue
+            num = (vp[:n] - vm[:n]) / (2 * h)
+            ana = J[:n, c]
+            scale = max(np.abs(J[:n, :n]).max(), 1e-300)
+            r = rel_q if (nm == "Q" and rel_q is not None) else rel
+            tol = r * np.abs(num) + absf * scale
+            bad = np.abs(ana - num) > tol
+            assert not bad.any(), (nm, c, np.flatnonzero(bad), ana[bad], num[bad])
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_jacobian_matches_finite_differences(oracle_lib, ci):
+    kind, card, inst = CASES[ci]
+    nl = single(kind, 1, card, inst)
+    rng = np.random.default_rng(70 + ci)
+    for _ in range(4):
+        x = _rand_x(nl, rng)
+        if kind == "mos1":
+            # Level 1 with forward-biased bulk replaces sqrt(phi - vbs) by a
+            # linear extension whose slope gmbs does not follow
+            # (N_DEV_MOSFET1.C:2955-2970, SPICE3 behaviour); the analytic
+            # Jacobian is exact only with the bulk at the lowest potential.
+            x[nl.node("b")] = x[:-1].min() - 0.1
+        if kind == "mos1":
+            _fd_check(nl, x, DC, "F", rel=1e-6, absf=1e-9)
+        else:
+            # BSIM3's analytic derivatives follow its currents and (capMod 3)
+            # charges to a few 1e-5 relative, not to rounding: the model
+            # differentiates some terms with respect to the smoothed Vbseff
+            # or holds factors fixed (mobMod 3 uc*T4, Abulk).  1e-3 is still
+            # ten times tighter than the reference's relTol of 1e-2.
+            _fd_check(nl, x, DC, "F", rel=1e-3, absf=1e-9)
+            _fd_check(nl, x, TRAN, "FQ", rel=1e-3, absf=1e-9)

@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxg_load.so")
 
 XG_OK = 0
+XG_E_ARG, XG_E_STATE, XG_E_HIP, XG_E_PATTERN, XG_E_NOMEM, XG_E_PARAM = -1, -2, -3, -4, -5, -6
 ERRORS = {
     -1: "XG_E_ARG", -2: "XG_E_STATE", -3: "XG_E_HIP", -4: "XG_E_PATTERN",
     -5: "XG_E_NOMEM", -6: "XG_E_PARAM",

@@ -204,8 +204,9 @@ def mos1_variety(seed=0, n_nodes=24, n_dev=96):
     """Parity workload covering the MOSFET1 options a netlist can set:
     series RD/RS and RSH (internal d'/s' nodes), TOX (Meyer capacitances),
     CBD/CBS and CJ/CJSW junctions with MJ != MJSW, JS with areas, GAMMA,
-    LAMBDA, NSUB-derived parameters, LD, OFF, IC=, M=, both polarities and
-    devices tied to ground on any terminal."""
+    LAMBDA, NSUB-derived parameters, LD, OFF, IC=, M=, model TNOM and
+    instance TEMP (both in Celsius), both polarities and devices tied to
+    ground on any terminal."""
     rng = np.random.default_rng(seed)
     nl = Netlist("MOSFET1 variety")
     base = dict(TOX=1.5e-8, LAMBDA=0.05, GAMMA=0.4, PHI=0.65, CGSO=2e-10,
@@ -218,6 +219,8 @@ def mos1_variety(seed=0, n_nodes=24, n_dev=96):
     nl.add_model("P_SH", "mos1", -1, VTO=-0.4, KP=40e-6, RSH=20.0, TOX=2e-8,
                  NSUB=1e16, MJ=0.5, MJSW=0.5, CJ=1e-4, CJSW=5e-10)
     nl.add_model("N_PLAIN", "mos1", 1, VTO=0.45, KP=120e-6)
+    nl.add_model("N_HOT", "mos1", 1, VTO=0.45, KP=100e-6, TNOM=40.0, NSUB=2e16,
+                 CJ=1.5e-4, CJSW=8e-10, JS=2e-4, TOX=1.8e-8)
     names = list(nl.models)
     nodes = ["n%d" % k for k in range(n_nodes)] + ["0"]
     for i in range(n_dev):
@@ -233,6 +236,8 @@ def mos1_variety(seed=0, n_nodes=24, n_dev=96):
             p["M"] = float(rng.integers(2, 4))
         if rng.random() < 0.08:
             p["OFF"] = 1.0
+        if rng.random() < 0.15:
+            p["TEMP"] = float(rng.choice([-20.0, 60.0, 110.0]))
         if rng.random() < 0.08:
             p.update(IC1=float(rng.uniform(0, 1.8)), IC2=float(rng.uniform(0, 1.8)),
                      IC3=float(rng.uniform(-0.5, 0)))

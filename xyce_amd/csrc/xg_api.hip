@@ -136,6 +136,21 @@ int xg_add_mos1_model(xg_circuit *c, int32_t dtype, int32_t n,
   return XG_OK;
 }
 
+// Solution ids must address the circuit's nodes (or be negative = ground),
+// and the instance's state / store slots must lie inside the vectors: the
+// kernels index x, state and store with these values unchecked.
+static int check_ids(const xg::Circuit &c, const int32_t lids[6],
+                     int32_t sta_base, int32_t n_sta, int32_t sto_base,
+                     int32_t n_sto, const char *who) {
+  for (int k = 0; k < 6; ++k)
+    if (lids[k] >= c.n_nodes)
+      return fail(XG_E_ARG, std::string(who) + ": node id out of range");
+  if (sta_base < 0 || (int64_t)sta_base + n_sta > c.n_state ||
+      sto_base < 0 || (int64_t)sto_base + n_sto > c.n_store)
+    return fail(XG_E_ARG, std::string(who) + ": state/store slots out of range");
+  return XG_OK;
+}
+
 int xg_add_mos1_instance(xg_circuit *c, int32_t model_id, int32_t n,
                          const char *const *keys, const double *vals,
                          const int32_t lids[6], int32_t sta_base,
@@ -143,6 +158,9 @@ int xg_add_mos1_instance(xg_circuit *c, int32_t model_id, int32_t n,
   if (!c || !lids || !inst_id || model_id < 0 ||
       model_id >= (int32_t)c->c.mos1_models.size())
     return fail(XG_E_ARG, "xg_add_mos1_instance: bad argument");
+  if (int rc = check_ids(c->c, lids, sta_base, 8, sto_base, 5,
+                         "xg_add_mos1_instance"))
+    return rc;
   xg::Mos1Inst in;
   in.model = model_id;
   if (xg::parse_params(n, keys, vals, in.p))
@@ -178,6 +196,9 @@ int xg_add_bsim3_instance(xg_circuit *c, int32_t model_id, int32_t n,
                           int32_t sto_base, int32_t *inst_id) {
   if (!c || !lids || !inst_id)
     return fail(XG_E_ARG, "xg_add_bsim3_instance: bad argument");
+  if (int rc = check_ids(c->c, lids, sta_base, 12, sto_base, 5,
+                         "xg_add_bsim3_instance"))
+    return rc;
   xg::ParamSet p;
   if (xg::parse_params(n, keys, vals, p))
     return fail(XG_E_ARG, "xg_add_bsim3_instance: bad parameter list");

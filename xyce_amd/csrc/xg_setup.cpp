@@ -88,7 +88,9 @@ int mos1_model_process(Mos1Model &m, const Options &opt, std::string &err) {
   m.fc = p.get("FC", 0.5);
   m.nsub = p.get("NSUB", 0.0);
   m.nss = p.get("NSS", 0.0);
-  m.tnom = p.given("TNOM") ? p.get("TNOM", 0.0) : opt.tnom;
+  // TNOM / TEMP are given in Celsius (N_DEV_DeviceEntity.C:863-866); the
+  // circuit options carry Kelvin.
+  m.tnom = p.given("TNOM") ? p.get("TNOM", 0.0) + 273.15 : opt.tnom;
   m.tpg = (int)p.get("TPG", 0.0);
   m.capBDGiven = p.given("CBD") || m.capBD != 0;
   m.capBSGiven = p.given("CBS") || m.capBS != 0;
@@ -232,7 +234,7 @@ int mos1_instance_setup(const Mos1Model &m, Mos1Inst &in, const Options &opt,
   const double nrd = p.get("NRD", 1.0), nrs = p.get("NRS", 1.0);
   const double pd = p.get("PD", 0.0), ps = p.get("PS", 0.0);
   const double np = p.get("M", 1.0);
-  in.temp = p.given("TEMP") ? p.get("TEMP", 0) : opt.temp;
+  in.temp = p.given("TEMP") ? p.get("TEMP", 0) + 273.15 : opt.temp;
   // series resistances (N_DEV_MOSFET1.C:1184-1225)
   double gd = 0, gs = 0;
   if (m.rd != 0) gd = 1 / m.rd;
