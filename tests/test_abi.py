@@ -88,11 +88,11 @@ def test_unknown_parameters_are_rejected(L):
 def test_bsim3_unsupported_options_fail_loudly(L):
     """capMod 0/1/2, NQS and IC= are outside the library's BSIM3 path: they
     are refused at setup rather than evaluated differently."""
-    from xyce_amd._lib import XG_OK
+    from xyce_amd._lib import XG_OK, XG_E_PARAM
     c = _circuit(L)
     mid = ctypes.c_int32()
     n, k, v = _kv(dict(VTH0=0.4, CAPMOD=2.0))
-    assert L.xg_add_bsim3_model(c, 1, n, k, v, ctypes.byref(mid)) != XG_OK
+    assert L.xg_add_bsim3_model(c, 1, n, k, v, ctypes.byref(mid)) == XG_E_PARAM
     assert b"CAPMOD" in L.xg_last_error().upper()
     n, k, v = _kv(dict(VTH0=0.4))
     assert L.xg_add_bsim3_model(c, 1, n, k, v, ctypes.byref(mid)) == XG_OK
@@ -101,7 +101,7 @@ def test_bsim3_unsupported_options_fail_loudly(L):
     for bad in (dict(NQSMOD=1.0), dict(IC1=0.5)):
         n, k, v = _kv(dict(W=1e-6, L=1e-6, **bad))
         assert L.xg_add_bsim3_instance(c, mid, n, k, v, lids, 0, 0,
-                                       ctypes.byref(iid)) != XG_OK
+                                       ctypes.byref(iid)) == XG_E_PARAM
     L.xg_circuit_free(c)
 
 

@@ -134,3 +134,37 @@ def test_bsim3_bitwise_deterministic():
         dev.sync()
         outs.append(np.concatenate([t.cpu().numpy() for t in (F, Q, FL, QL, JF, JQ, SN, TN)]))
     assert outs[0].tobytes() == outs[1].tobytes() == outs[2].tobytes()
+
+
+def test_nonfinite_inputs_are_counted():
+    """A NaN node voltage reaches every instance that reads the node; the
+    stats report exactly those instances as non-finite (the load itself
+    completes: no evaluation step loops on data)."""
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd.loader import DeviceLoad
+    nl = circuits.random_logic_bsim3(n_gates=200, seed=6)
+    nl.layout()
+    n1 = nl.n_nodes + 1
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-0.2, 1.5, n1)
+    x[-1] = 0.0
+    bad = nl.node(nl.mos[7].g)
+    x[bad] = np.nan
+    expect = sum(1 for r in nl.bsim3 if bad in r["lids"])
+    assert expect > 0
+    cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    dev = DeviceLoad(nl)
+    dev.set_solver_state(transient=1, newtonIter=1)
+    F, Q, FL, QL = (cu(np.zeros(n1)) for _ in range(4))
+    JF, JQ = cu(np.zeros(dev.nnz)), cu(np.zeros(dev.nnz))
+    dev.load_all(cu(x), cu(np.zeros(nl.n_state)), cu(np.zeros(nl.n_store)),
+                 cu(np.zeros(nl.n_state)), cu(np.zeros(nl.n_store)), F, Q, FL, QL, JF, JQ)
+    dev.sync()
+    _, n_nonfinite = dev.stats()
+    assert n_nonfinite == expect
+    x[bad] = 0.5
+    dev.load_all(cu(x), cu(np.zeros(nl.n_state)), cu(np.zeros(nl.n_store)),
+                 cu(np.zeros(nl.n_state)), cu(np.zeros(nl.n_store)), F, Q, FL, QL, JF, JQ)
+    dev.sync()
+    assert dev.stats()[1] == 0

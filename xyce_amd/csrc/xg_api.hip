@@ -288,7 +288,7 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
   chk(h.slots = h.alloc<double>((size_t)P.n_slot_values));
   h.n_limited_slots = P.m1_slots + xg::bsim3_eval_slots(P);
   chk(h.limited_all = h.alloc<int32_t>((size_t)h.n_limited_slots));
-  chk(h.count_buf = h.alloc<int64_t>(64));
+  chk(h.count_buf = h.alloc<int64_t>(128));
   if (ok) {
     h.m1.slots = h.slots;
     h.m1.limited = h.limited_all;
@@ -474,12 +474,15 @@ int xg_get_stats(xg_handle *H, xg_stats *st) {
   hipError_t e = xg::launch_count_flags(h.limited_all, h.n_limited_slots,
                                         h.count_buf, h.stream);
   if (e != hipSuccess) return hip_fail(e, "count");
-  int64_t part[64];
+  int64_t part[128];
   e = hipMemcpyAsync(part, h.count_buf, sizeof(part), hipMemcpyDeviceToHost,
                      h.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h.stream);
   if (e != hipSuccess) return hip_fail(e, "count copy");
-  for (int k = 0; k < 64; ++k) st->n_limited += part[k];
+  for (int k = 0; k < 64; ++k) {
+    st->n_limited += part[k];
+    st->n_nonfinite += part[64 + k];
+  }
   return XG_OK;
 }
 

@@ -1208,8 +1208,6 @@ __global__ void __launch_bounds__(kBlock)
     p.sta_curr[sta + 3] = qbs;
     p.sta_curr[sta + 4] = qbd;
   }
-  a.limited[slot] = limitedFlag ? 1 : 0;
-
   // ---- Master::loadDAEVectors values
   double *out = a.slots + slot;
   auto W = [&](int k, double v) { out[(size_t)k * S] = v; };
@@ -1371,6 +1369,13 @@ __global__ void __launch_bounds__(kBlock)
     W(JQ + 14, -((-CAPcsdb) * np));
     W(JQ + 15, (+CAPcssb) * np);
   }
+  // Bit 0: junction voltages limited (the reference's limitedFlag).  Bit 1:
+  // a non-finite current, conductance, charge or capacitance (x * 0.0 is
+  // NaN exactly when x is NaN or infinite).
+  const double probe = (cdreq + ceqbs + ceqbd + Gm_ + Gmbs_ + gds + gbd + gbs +
+                        qg + qb + qd + CAPcggb + CAPcddb + CAPcssb + Idrain +
+                        Isource) * 0.0;
+  a.limited[slot] = (limitedFlag ? 1 : 0) | (probe != probe ? 2 : 0);
 }
 
 }  // namespace

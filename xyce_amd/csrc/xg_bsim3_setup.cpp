@@ -209,10 +209,11 @@ int bsim3_add_instance(Circuit &c, int model, const ParamSet &p,
     for (const char *k : keys) ok = ok || kv.first == k;
     if (!ok) {
       if (kv.first == "IC1" || kv.first == "IC2" || kv.first == "IC3" ||
-          kv.first == "NQSMOD")
+          kv.first == "NQSMOD") {
         err = "BSIM3 " + kv.first + " (extra unknowns) is not supported";
-      else
-        err = "unknown BSIM3 instance parameter " + kv.first;
+        return XG_E_PARAM;
+      }
+      err = "unknown BSIM3 instance parameter " + kv.first;
       return XG_E_ARG;
     }
   }

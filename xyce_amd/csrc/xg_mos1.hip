@@ -362,8 +362,6 @@ __global__ void __launch_bounds__(kBlock)
   p.sta_next[sta + 2] = qgb;
   p.sta_next[sta + 6] = qbd;
   p.sta_next[sta + 7] = qbs;
-  a.limited[slot] = limitedFlag ? 1 : 0;
-
   // ---- Master::loadDAEVectors values
   double *out = a.slots + slot;
   auto W = [&](int k, double v) { out[(size_t)k * S] = v; };
@@ -461,6 +459,13 @@ __global__ void __launch_bounds__(kBlock)
   W(JQ + 11, -(gcgs * np));
   W(JQ + 12, -(+gcbs * np));
   W(JQ + 13, (+gcbs + gcgs) * np);
+  // Bit 0: junction voltages limited (the reference's limitedFlag).  Bit 1:
+  // a non-finite current, conductance or charge (x * 0.0 is NaN exactly
+  // when x is NaN or infinite).
+  const double probe = (cdreq + cbs + cbd + Gm + Gmbs + gds + gbd + gbs + qgs +
+                        qgd + qgb + qbs + qbd + Capgs + capbs + capbd + Idrain +
+                        Isource) * 0.0;
+  a.limited[slot] = (limitedFlag ? 1 : 0) | (probe != probe ? 2 : 0);
 }
 
 }  // namespace

@@ -111,19 +111,31 @@ __global__ void __launch_bounds__(kBlock)
   if (lane == 0) out[t] = r;
 }
 
-__global__ void count_kernel(const int32_t *flags, int32_t n, int64_t *out) {
-  __shared__ int64_t cnt[256];
-  int64_t c = 0;
+// Per-instance flag words: bit 0 limited, bit 1 non-finite.  Block b
+// writes its two counts to out[b] and out[64 + b].
+__global__ void __launch_bounds__(256)
+    count_kernel(const int32_t *flags, int32_t n, int64_t *out) {
+  __shared__ int64_t cnt[2][256];
+  int64_t c0 = 0, c1 = 0;
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x)
-    c += flags[i] != 0;
-  cnt[threadIdx.x] = c;
+       i += gridDim.x * blockDim.x) {
+    c0 += flags[i] & 1;
+    c1 += (flags[i] >> 1) & 1;
+  }
+  cnt[0][threadIdx.x] = c0;
+  cnt[1][threadIdx.x] = c1;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) cnt[threadIdx.x] += cnt[threadIdx.x + s];
+    if ((int)threadIdx.x < s) {
+      cnt[0][threadIdx.x] += cnt[0][threadIdx.x + s];
+      cnt[1][threadIdx.x] += cnt[1][threadIdx.x + s];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = cnt[0];
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = cnt[0][0];
+    out[64 + blockIdx.x] = cnt[1][0];
+  }
 }
 
 }  // namespace
@@ -156,7 +168,8 @@ hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
   return hipGetLastError();
 }
 
-// Writes one partial count per block (64 blocks); the caller sums them.
+// Writes two partial counts per block (64 blocks: limited in out[0..63],
+// non-finite in out[64..127]); the caller sums them.
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s) {
   hipLaunchKernelGGL(count_kernel, dim3(64), dim3(256), 0, s, flags, n, out);
