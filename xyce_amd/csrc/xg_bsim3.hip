@@ -1210,7 +1210,11 @@ __global__ void __launch_bounds__(kBlock)
   }
   // ---- Master::loadDAEVectors values
   double *out = a.slots + slot;
-  auto W = [&](int k, double v) { out[(size_t)k * S] = v; };
+  double chk = 0.0;  // running sum of everything written, for the finiteness flag
+  auto W = [&](int k, double v) {
+    out[(size_t)k * S] = v;
+    chk = chk + v;
+  };
   double Gm_, Gmbs_, FwdSum, RevSum, cdreq, ceqbd, ceqbs, gbbdp, gbbsp, gbdpg,
       gbdpdp, gbdpb, gbdpsp, gbspg, gbspdp, gbspb, gbspsp;
   const double cdv = cdrain;
@@ -1370,12 +1374,10 @@ __global__ void __launch_bounds__(kBlock)
     W(JQ + 15, (+CAPcssb) * np);
   }
   // Bit 0: junction voltages limited (the reference's limitedFlag).  Bit 1:
-  // a non-finite current, conductance, charge or capacitance (x * 0.0 is
-  // NaN exactly when x is NaN or infinite).
-  const double probe = (cdreq + ceqbs + ceqbd + Gm_ + Gmbs_ + gds + gbd + gbs +
-                        qg + qb + qd + CAPcggb + CAPcddb + CAPcssb + Idrain +
-                        Isource) * 0.0;
-  a.limited[slot] = (limitedFlag ? 1 : 0) | (probe != probe ? 2 : 0);
+  // a non-finite statement value or next-state/store value (their sum is
+  // finite exactly when every term is: the magnitudes are far from overflow).
+  chk = chk + qb + qg + qd + qbs + qbd + vbs + vgs + vds + vbd + von;
+  a.limited[slot] = (limitedFlag ? 1 : 0) | ((chk - chk) != 0.0 ? 2 : 0);
 }
 
 }  // namespace

@@ -364,7 +364,11 @@ __global__ void __launch_bounds__(kBlock)
   p.sta_next[sta + 7] = qbs;
   // ---- Master::loadDAEVectors values
   double *out = a.slots + slot;
-  auto W = [&](int k, double v) { out[(size_t)k * S] = v; };
+  double chk = 0.0;  // running sum of everything written, for the finiteness flag
+  auto W = [&](int k, double v) {
+    out[(size_t)k * S] = v;
+    chk = chk + v;
+  };
   const double np = F(M1F_M);
   const double ceqbs = dtype * (cbs), ceqbd = dtype * (cbd);
   const double ceqgb = 0.0, ceqgs = 0.0, ceqgd = 0.0;
@@ -460,12 +464,10 @@ __global__ void __launch_bounds__(kBlock)
   W(JQ + 12, -(+gcbs * np));
   W(JQ + 13, (+gcbs + gcgs) * np);
   // Bit 0: junction voltages limited (the reference's limitedFlag).  Bit 1:
-  // a non-finite current, conductance or charge (x * 0.0 is NaN exactly
-  // when x is NaN or infinite).
-  const double probe = (cdreq + cbs + cbd + Gm + Gmbs + gds + gbd + gbs + qgs +
-                        qgd + qgb + qbs + qbd + Capgs + capbs + capbd + Idrain +
-                        Isource) * 0.0;
-  a.limited[slot] = (limitedFlag ? 1 : 0) | (probe != probe ? 2 : 0);
+  // a non-finite statement value or next-state value (their sum is finite
+  // exactly when every term is: the magnitudes are far from overflow).
+  chk = chk + qgs + qgd + qgb + qbd + qbs;
+  a.limited[slot] = (limitedFlag ? 1 : 0) | ((chk - chk) != 0.0 ? 2 : 0);
 }
 
 }  // namespace
