@@ -199,6 +199,16 @@ int xg_sync(xg_handle *h);
 /* Stream the handle enqueues on (hipStream_t; 0 = default stream). */
 int xg_set_stream(xg_handle *h, void *stream);
 
+/* BSIM3 evaluation kernel selection (no counterpart in the reference: a
+   launch-configuration knob for tuning and tests).  XG_EVAL_AUTO picks by
+   circuit size; the others force one form.  All forms compute identical
+   results. */
+#define XG_EVAL_AUTO 0
+#define XG_EVAL_HBM 1          /* 4 waves per block, records read from HBM  */
+#define XG_EVAL_STAGED 2       /* 1 wave per block, records staged in LDS   */
+#define XG_EVAL_STAGED_SPLIT 3 /* as STAGED, DC and charge parts on two waves */
+int xg_set_eval_variant(xg_handle *h, int32_t variant);
+
 /* Sizes the library was finalized with. */
 int xg_sizes(const xg_handle *h, int32_t *n_nodes, int32_t *n_state,
              int32_t *n_store, int32_t *n_mos1, int32_t *n_bsim3);

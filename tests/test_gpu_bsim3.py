@@ -25,7 +25,7 @@ SOLVER_STATES = [
 ]
 
 
-def run_pair(nl, ss, seed, torch, vhi=1.5):
+def run_pair(nl, ss, seed, torch, vhi=1.5, variant="auto"):
     from oracle.oracle import RefLoad
     from xyce_amd.loader import DeviceLoad
     rng = np.random.default_rng(seed)
@@ -45,6 +45,7 @@ def run_pair(nl, ss, seed, torch, vhi=1.5):
     lim_r, _ = ref.b3.flags()
 
     dev = DeviceLoad(nl)
+    dev.set_eval_variant(variant)
     cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
     X, SC, TC = cu(x), cu(sta_c), cu(sto_c)
     SN, TN = cu(np.zeros(nl.n_state)), cu(sto_n0)
@@ -168,3 +169,16 @@ def test_nonfinite_inputs_are_counted():
                  cu(np.zeros(nl.n_state)), cu(np.zeros(nl.n_store)), F, Q, FL, QL, JF, JQ)
     dev.sync()
     assert dev.stats()[1] == 0
+
+
+@pytest.mark.parametrize("variant", ["hbm", "staged", "staged_split"])
+@pytest.mark.parametrize("si", [0, 2, 3, 5])
+def test_bsim3_every_kernel_form(variant, si):
+    """The three forms of the evaluation kernel (records from HBM; staged in
+    LDS; staged and split into DC and charge waves) each match the oracle,
+    whatever the circuit size would select."""
+    import torch
+    from xyce_amd import circuits
+    nl = circuits.bsim3_variety(seed=7)
+    print("case", variant, SOLVER_STATES[si], flush=True)
+    check(*run_pair(nl, SOLVER_STATES[si], 50 + si, torch, variant=variant))

@@ -398,6 +398,13 @@ int xg_set_solver_state(xg_handle *H, const xg_solver_state *s) {
   return XG_OK;
 }
 
+int xg_set_eval_variant(xg_handle *H, int32_t variant) {
+  if (!H || variant < XG_EVAL_AUTO || variant > XG_EVAL_STAGED_SPLIT)
+    return fail(XG_E_ARG, "xg_set_eval_variant: bad argument");
+  H->h.b3.variant = variant;
+  return XG_OK;
+}
+
 int xg_set_stream(xg_handle *H, void *stream) {
   if (!H) return fail(XG_E_ARG, "xg_set_stream");
   H->h.stream = (hipStream_t)stream;

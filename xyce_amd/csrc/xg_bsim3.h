@@ -24,6 +24,10 @@ struct Bsim3Dev {
   const int32_t *sto_base = nullptr;
   const int32_t *flags = nullptr;
   const int32_t *sdp_idx = nullptr; // n_slots
+  // Per-wave blocks of the slots' size-dependent + junction fields
+  // ((n_slots / 64) x (B3R_N + B3I_N) x 64), for the LDS-staged kernel.
+  const double *rec = nullptr;
+  int32_t variant = 0;  // XG_EVAL_* of include/xg_load.h
   double *slots = nullptr;          // B3S_N x n_slots (inside the shared buffer)
   int32_t *limited = nullptr;
   int64_t slot_base = 0;

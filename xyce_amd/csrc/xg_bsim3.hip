@@ -93,46 +93,84 @@ __device__ __forceinline__ void b3_overlap(double v, double cgxo, double cgxl,
   q = (cgxo + T3) * v - T3 * (T2 + 0.5 * ckappa * (T4 - 1.0));
 }
 
-__global__ void __launch_bounds__(kBlock)
-    bsim3_eval_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p) {
-  __shared__ double card_lds[kWavesPerBlock][B3C_N];
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  const int slot = blockIdx.x * kBlock + tid;
+// What one instance's evaluation reads from HBM before any arithmetic: its
+// flags, slot bases, the six terminal voltages and the five previous-store
+// values.  Loaded in the kernels' prologues so that every load is in flight
+// before the first wait.
+struct B3Inputs {
+  int32_t flags, sta, sto;
+  double Vd, Vg, Vs, Vb, Vdp, Vsp;
+  double old[5];  // vbd, vbs, vgs, vds, von of the store the iteration reads
+};
+
+__device__ __forceinline__ void b3_inputs(const Bsim3Dev &a,
+                                          const SolverStateDev &ss,
+                                          const LoadPtrs &p, int slot,
+                                          B3Inputs &in) {
   const int S = a.n_slots;
-  {
-    const int nwaves = S >> 6;
-    for (int k = tid; k < kWavesPerBlock * B3C_N; k += kBlock) {
-      const int w = k / B3C_N, f = k % B3C_N;
-      const int gw = blockIdx.x * kWavesPerBlock + w;
-      card_lds[w][f] = gw < nwaves ? a.cards[a.wave_card[gw] * B3C_N + f] : 0.0;
-    }
-  }
-  __syncthreads();
-  if (slot >= S) return;
-  if (a.inst_of[slot] < 0) return;
-  const double *cd = card_lds[wave];
+  in.flags = a.flags[slot];
+  in.sta = a.sta_base[slot];
+  in.sto = a.sto_base[slot];
+  // every index is valid for padding slots too (ground node, slot 0), so
+  // nothing here waits on another load before issuing
+  const int ld = a.lid4[slot], lg = a.lid4[S + slot], ls = a.lid4[2 * S + slot],
+            lb = a.lid4[3 * S + slot];
+  const int li_d = a.lid_int[slot], li_s = a.lid_int[S + slot];
+  const int ldp = (in.flags & XF_HAS_DP) ? li_d : ld;
+  const int lsp = (in.flags & XF_HAS_SP) ? li_s : ls;
+  in.Vd = p.x[ld];
+  in.Vg = p.x[lg];
+  in.Vs = p.x[ls];
+  in.Vb = p.x[lb];
+  in.Vdp = p.x[ldp];
+  in.Vsp = p.x[lsp];
+  // Master::updateState reads the next store from the second Newton
+  // iteration on, the current store at the first (N_DEV_MOSFET_B3.C:4799-4840)
+  const double *st = ss.newtonIter == 0 ? p.sto_curr : p.sto_next;
+  const bool from_store = !(ss.newtonIter == 0 && ss.dcop);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) in.old[k] = from_store ? st[in.sto + k] : 0.0;
+}
+
+// Which part of an instance's evaluation a wave performs.  kFull: all of
+// it.  kDC / kCV: two waves share the instances of one eval wave; both
+// redo the limiting and the threshold voltage (cheap), then the kDC wave
+// computes the channel and junction currents, their conductances, the
+// store values and the F, dFdxdVp and dF/dx statements, and the kCV wave
+// the capMod-3, junction and overlap charges and capacitances, the state
+// values and the Q, dQdxdVp and dQ/dx statements.  Every value is computed
+// by the same expressions as in kFull, so results are identical.
+enum B3Role : int { kFull = 0, kDC = 1, kCV = 2 };
+
+// The evaluation of one instance.  Card fields come from LDS (cd); the
+// size-dependent record and junction fields through R / GEO:
+//   kStaged = false: R -> the record's column of the field-major record
+//                    array, GEO -> the slot's column of the geometry array;
+//   kStaged = true:  both from this lane's column of the wave's LDS copy.
+template <bool kStaged, int kRole>
+__device__ __forceinline__ void b3_eval(const Bsim3Dev &a,
+                                        const SolverStateDev &ss,
+                                        const LoadPtrs &p, int slot,
+                                        const double *cd, const double *R,
+                                        const double *Gp, const B3Inputs &in) {
+  const int S = a.n_slots;
   const double dtype = cd[B3C_DTYPE];
   const double gmin = cd[B3C_GMIN];
   const bool limiter = cd[B3C_LIMITER] != 0.0;
-  const int NS = a.n_sdp;
-  const int sidx = a.sdp_idx[slot];
-  const double *R = a.sdp + sidx;
-  auto P = [&](int f) { return R[(size_t)f * NS]; };
-  auto GEO = [&](int f) { return a.geo[(size_t)f * S + slot]; };
+  const size_t rs = kStaged ? 64 : (size_t)a.n_sdp;
+  const size_t gs = kStaged ? 64 : (size_t)S;
+  auto P = [&](int f) { return R[(size_t)f * rs]; };
+  auto GEO = [&](int f) { return Gp[(size_t)f * gs]; };
 
-  const int32_t flags = a.flags[slot];
+  const int32_t flags = in.flags;
   const bool OFF = flags & XF_OFF;
-  const int ld = a.lid4[slot], lg = a.lid4[S + slot], ls = a.lid4[2 * S + slot],
-            lb = a.lid4[3 * S + slot];
-  const int ldp = (flags & XF_HAS_DP) ? a.lid_int[slot] : ld;
-  const int lsp = (flags & XF_HAS_SP) ? a.lid_int[S + slot] : ls;
-  const int sta = a.sta_base[slot], sto = a.sto_base[slot];
+  const int sta = in.sta, sto = in.sto;
   const bool charge = ss.tranop || ss.acop || ss.transient;
+  constexpr bool doDC = kRole != kCV, doCV = kRole != kDC;
 
   // ---- voltages and limiting (N_DEV_MOSFET_B3.C:4691-5018)
-  const double Vd = p.x[ld], Vg = p.x[lg], Vs = p.x[ls], Vb = p.x[lb];
-  const double Vsp = p.x[lsp], Vdp = p.x[ldp];
+  const double Vd = in.Vd, Vg = in.Vg, Vs = in.Vs, Vb = in.Vb;
+  const double Vsp = in.Vsp, Vdp = in.Vdp;
   const double Vddp = Vd - Vdp, Vssp = Vs - Vsp;
   double vbs = dtype * (Vb - Vsp), vgs = dtype * (Vg - Vsp),
          vds = dtype * (Vdp - Vsp);
@@ -151,26 +189,18 @@ __global__ void __launch_bounds__(kBlock)
     vbs = vgs = vds = 0;
   }
   double vbs_old, vbd_old, vgs_old, vds_old, von_local;
-  if (ss.newtonIter == 0) {
-    if (!ss.dcop) {
-      vbs_old = p.sto_curr[sto + 1];
-      vbd_old = p.sto_curr[sto + 0];
-      vgs_old = p.sto_curr[sto + 2];
-      vds_old = p.sto_curr[sto + 3];
-      von_local = p.sto_curr[sto + 4];
-    } else {
-      vbs_old = vbs;
-      vbd_old = vbd;
-      vgs_old = vgs;
-      vds_old = vds;
-      von_local = 0.0;
-    }
+  if (ss.newtonIter == 0 && ss.dcop) {
+    vbs_old = vbs;
+    vbd_old = vbd;
+    vgs_old = vgs;
+    vds_old = vds;
+    von_local = 0.0;
   } else {
-    vbs_old = p.sto_next[sto + 1];
-    vbd_old = p.sto_next[sto + 0];
-    vgs_old = p.sto_next[sto + 2];
-    vds_old = p.sto_next[sto + 3];
-    von_local = p.sto_next[sto + 4];
+    vbd_old = in.old[0];
+    vbs_old = in.old[1];
+    vgs_old = in.old[2];
+    vds_old = in.old[3];
+    von_local = in.old[4];
   }
   if (limiter && !(ss.initFix && OFF)) {
     const double vgdo = vgs_old - vds_old;
@@ -206,8 +236,8 @@ __global__ void __launch_bounds__(kBlock)
   // ---- junction diodes (N_DEV_MOSFET_B3.C:5037-5125)
   const double Nvtm = cd[B3C_NVTM];
   const double ijth = cd[B3C_IJTH];
-  double gbs, cbs, gbd, cbd;
-  {
+  double gbs = 0.0, cbs = 0.0, gbd = 0.0, cbd = 0.0;
+  if constexpr (doDC) {
     const double ss_ = GEO(B3I_SSATCUR);
     if (ss_ <= 0.0) {
       gbs = gmin;
@@ -406,8 +436,9 @@ __global__ void __launch_bounds__(kBlock)
   }
   const double Vgst = Vgs_eff - Vth;
   const double voff = P(B3R_VOFF), cdep0 = P(B3R_CDEP0);
-  double Vgsteff, dVgsteff_dVg, dVgsteff_dVd, dVgsteff_dVb;
-  {
+  double Vgsteff = 0.0, dVgsteff_dVg = 0.0, dVgsteff_dVd = 0.0,
+         dVgsteff_dVb = 0.0;
+  if constexpr (doDC) {
     T10 = 2.0 * n * Vtm;
     const double VgstNVt = Vgst / T10;
     const double ExpArg = (2.0 * voff - Vgst) / T10;
@@ -519,6 +550,9 @@ __global__ void __launch_bounds__(kBlock)
     Abulk *= T0;
     Abulk0 *= T0;
   }
+  double cdrain = 0.0, gds = 0.0, gm = 0.0, gmbs = 0.0;
+  double gbbs = 0.0, gbgs = 0.0, gbds = 0.0, csub = 0.0;
+  if constexpr (doDC) {
   // ---- mobility (N_DEV_MOSFET_B3.C:5511-5563)
   double ueff, dueff_dVg, dueff_dVd, dueff_dVb;
   {
@@ -875,15 +909,22 @@ __global__ void __launch_bounds__(kBlock)
       Gbb *= dVbseff_dVb;
     }
   }
-  const double cdrain = Ids, gds = Gds, gm = Gm, gmbs = Gmb;
-  const double gbbs = Gbb, gbgs = Gbg, gbds = Gbd, csub = Isub;
+  cdrain = Ids;
+  gds = Gds;
+  gm = Gm;
+  gmbs = Gmb;
+  gbbs = Gbb;
+  gbgs = Gbg;
+  gbds = Gbd;
+  csub = Isub;
+  }  // doDC
 
   // ---- charges (capMod 3) and capacitances (N_DEV_MOSFET_B3.C:5994-7252)
   double qg = 0.0, qd = 0.0, qb = 0.0, qbs = 0.0, qbd = 0.0;
   double CAPcggb = 0.0, CAPcgdb = 0.0, CAPcgsb = 0.0, CAPcdgb = 0.0,
          CAPcddb = 0.0, CAPcdsb = 0.0, CAPcsgb = 0.0, CAPcsdb = 0.0,
          CAPcssb = 0.0, CAPcbgb = 0.0, CAPcbdb = 0.0, CAPcbsb = 0.0;
-  if (charge) {
+  if (doCV && charge) {
     const double xpart = cd[B3C_XPART];
     double qgate = 0.0, qdrn = 0.0, qsrc = 0.0, qbulk = 0.0;
     double cggb = 0.0, cgsb = 0.0, cgdb = 0.0, cdgb = 0.0, cdsb = 0.0,
@@ -1190,23 +1231,27 @@ __global__ void __launch_bounds__(kBlock)
   const double Isource = gsr * Vssp;
 
   // ---- Master::updateState
-  p.sto_next[sto + 1] = vbs;
-  p.sto_next[sto + 2] = vgs;
-  p.sto_next[sto + 3] = vds;
-  p.sto_next[sto + 0] = vbd;
-  p.sto_next[sto + 4] = von;
-  p.sta_next[sta + 0] = qb;
-  p.sta_next[sta + 1] = qg;
-  p.sta_next[sta + 2] = qd;
-  p.sta_next[sta + 3] = qbs;
-  p.sta_next[sta + 4] = qbd;
   const bool first_tran = !ss.dcop && ss.initTran && ss.newtonIter == 0;
-  if (first_tran) {
-    p.sta_curr[sta + 0] = qb;
-    p.sta_curr[sta + 1] = qg;
-    p.sta_curr[sta + 2] = qd;
-    p.sta_curr[sta + 3] = qbs;
-    p.sta_curr[sta + 4] = qbd;
+  if constexpr (doDC) {
+    p.sto_next[sto + 1] = vbs;
+    p.sto_next[sto + 2] = vgs;
+    p.sto_next[sto + 3] = vds;
+    p.sto_next[sto + 0] = vbd;
+    p.sto_next[sto + 4] = von;
+  }
+  if constexpr (doCV) {
+    p.sta_next[sta + 0] = qb;
+    p.sta_next[sta + 1] = qg;
+    p.sta_next[sta + 2] = qd;
+    p.sta_next[sta + 3] = qbs;
+    p.sta_next[sta + 4] = qbd;
+    if (first_tran) {
+      p.sta_curr[sta + 0] = qb;
+      p.sta_curr[sta + 1] = qg;
+      p.sta_curr[sta + 2] = qd;
+      p.sta_curr[sta + 3] = qbs;
+      p.sta_curr[sta + 4] = qbd;
+    }
   }
   // ---- Master::loadDAEVectors values
   double *out = a.slots + slot;
@@ -1215,179 +1260,280 @@ __global__ void __launch_bounds__(kBlock)
     out[(size_t)k * S] = v;
     chk = chk + v;
   };
-  double Gm_, Gmbs_, FwdSum, RevSum, cdreq, ceqbd, ceqbs, gbbdp, gbbsp, gbdpg,
-      gbdpdp, gbdpb, gbdpsp, gbspg, gbspdp, gbspb, gbspsp;
-  const double cdv = cdrain;
-  if (mode >= 0) {
-    Gm_ = gm;
-    Gmbs_ = gmbs;
-    FwdSum = Gm_ + Gmbs_;
-    RevSum = 0.0;
-    cdreq = dtype * (cdv);
-    ceqbd = -dtype * (csub);
-    ceqbs = 0.0;
-    gbbdp = -gbds;
-    gbbsp = (gbds + gbgs + gbbs);
-    gbdpg = gbgs;
-    gbdpdp = gbds;
-    gbdpb = gbbs;
-    gbdpsp = -(gbdpg + gbdpdp + gbdpb);
-    gbspg = 0.0;
-    gbspdp = 0.0;
-    gbspb = 0.0;
-    gbspsp = 0.0;
-  } else {
-    Gm_ = -gm;
-    Gmbs_ = -gmbs;
-    FwdSum = 0.0;
-    RevSum = -(Gm_ + Gmbs_);
-    cdreq = -dtype * (cdv);
-    ceqbs = -dtype * (csub);
-    ceqbd = 0.0;
-    gbbsp = -gbds;
-    gbbdp = (gbds + gbgs + gbbs);
-    gbdpg = 0.0;
-    gbdpsp = 0.0;
-    gbdpb = 0.0;
-    gbdpdp = 0.0;
-    gbspg = gbgs;
-    gbspsp = gbds;
-    gbspb = gbbs;
-    gbspdp = -(gbspg + gbspsp + gbspb);
-  }
-  if (dtype > 0) {
-    ceqbs += (cbs);
-    ceqbd += (cbd);
-  } else {
-    ceqbs -= (cbs);
-    ceqbd -= (cbd);
-  }
-  if (flags & XF_HAS_DP) W(B3S_F_D, Idrain * np);
-  if (flags & XF_HAS_SP) W(B3S_F_S, Isource * np);
-  W(B3S_F_B, (ceqbs + ceqbd) * np);
-  W(B3S_F_DP, (-(ceqbd - cdreq) - Idrain) * np);
-  W(B3S_F_SP, (-(cdreq + ceqbs) - Isource) * np);
-  if (limiter && !origFlag) {
-    double cdreq_J = 0.0, ceqbd_J = 0.0, ceqbs_J = 0.0, tmp;
+  const int JF = B3S_JF0, JQ = B3S_JQ0;
+  if constexpr (doDC) {
+    double Gm_, Gmbs_, FwdSum, RevSum, cdreq, ceqbd, ceqbs, gbbdp, gbbsp, gbdpg,
+        gbdpdp, gbdpb, gbdpsp, gbspg, gbspdp, gbspb, gbspsp;
+    const double cdv = cdrain;
     if (mode >= 0) {
-      tmp = dtype * (-gds * (vds - vds_orig) - Gm_ * (vgs - vgs_orig) - Gmbs_ * (vbs - vbs_orig));
-      cdreq_J += tmp;
-      tmp = -dtype * (-gbds * (vds - vds_orig) - gbgs * (vgs - vgs_orig) - gbbs * (vbs - vbs_orig));
-      ceqbd_J += tmp;
+      Gm_ = gm;
+      Gmbs_ = gmbs;
+      FwdSum = Gm_ + Gmbs_;
+      RevSum = 0.0;
+      cdreq = dtype * (cdv);
+      ceqbd = -dtype * (csub);
+      ceqbs = 0.0;
+      gbbdp = -gbds;
+      gbbsp = (gbds + gbgs + gbbs);
+      gbdpg = gbgs;
+      gbdpdp = gbds;
+      gbdpb = gbbs;
+      gbdpsp = -(gbdpg + gbdpdp + gbdpb);
+      gbspg = 0.0;
+      gbspdp = 0.0;
+      gbspb = 0.0;
+      gbspsp = 0.0;
     } else {
-      tmp = -dtype * (gds * (vds - vds_orig) + Gm_ * (vgd - vgd_orig) + Gmbs_ * (vbd - vbd_orig));
-      cdreq_J += tmp;
-      tmp = -dtype * (gbds * (vds - vds_orig) - gbgs * (vgd - vgd_orig) - gbbs * (vbd - vbd_orig));
-      ceqbd_J += tmp;
+      Gm_ = -gm;
+      Gmbs_ = -gmbs;
+      FwdSum = 0.0;
+      RevSum = -(Gm_ + Gmbs_);
+      cdreq = -dtype * (cdv);
+      ceqbs = -dtype * (csub);
+      ceqbd = 0.0;
+      gbbsp = -gbds;
+      gbbdp = (gbds + gbgs + gbbs);
+      gbdpg = 0.0;
+      gbdpsp = 0.0;
+      gbdpb = 0.0;
+      gbdpdp = 0.0;
+      gbspg = gbgs;
+      gbspsp = gbds;
+      gbspb = gbbs;
+      gbspdp = -(gbspg + gbspsp + gbspb);
     }
     if (dtype > 0) {
-      ceqbs_J += (-gbs * (vbs - vbs_orig));
-      ceqbd_J += (-gbd * (vbd - vbd_orig));
+      ceqbs += (cbs);
+      ceqbd += (cbd);
     } else {
-      ceqbs_J -= (-gbs * (vbs - vbs_orig));
-      ceqbd_J -= (-gbd * (vbd - vbd_orig));
+      ceqbs -= (cbs);
+      ceqbd -= (cbd);
     }
-    W(B3S_FL_B, -(ceqbs_J + ceqbd_J) * np);
-    W(B3S_FL_DP, (ceqbd_J - cdreq_J) * np);
-    W(B3S_FL_SP, (cdreq_J + ceqbs_J) * np);
-  } else {
-    W(B3S_FL_B, -0.0);
-    W(B3S_FL_DP, -0.0);
-    W(B3S_FL_SP, -0.0);
+    if (flags & XF_HAS_DP) W(B3S_F_D, Idrain * np);
+    if (flags & XF_HAS_SP) W(B3S_F_S, Isource * np);
+    W(B3S_F_B, (ceqbs + ceqbd) * np);
+    W(B3S_F_DP, (-(ceqbd - cdreq) - Idrain) * np);
+    W(B3S_F_SP, (-(cdreq + ceqbs) - Isource) * np);
+    if (limiter && !origFlag) {
+      double cdreq_J = 0.0, ceqbd_J = 0.0, ceqbs_J = 0.0, tmp;
+      if (mode >= 0) {
+        tmp = dtype * (-gds * (vds - vds_orig) - Gm_ * (vgs - vgs_orig) - Gmbs_ * (vbs - vbs_orig));
+        cdreq_J += tmp;
+        tmp = -dtype * (-gbds * (vds - vds_orig) - gbgs * (vgs - vgs_orig) - gbbs * (vbs - vbs_orig));
+        ceqbd_J += tmp;
+      } else {
+        tmp = -dtype * (gds * (vds - vds_orig) + Gm_ * (vgd - vgd_orig) + Gmbs_ * (vbd - vbd_orig));
+        cdreq_J += tmp;
+        tmp = -dtype * (gbds * (vds - vds_orig) - gbgs * (vgd - vgd_orig) - gbbs * (vbd - vbd_orig));
+        ceqbd_J += tmp;
+      }
+      if (dtype > 0) {
+        ceqbs_J += (-gbs * (vbs - vbs_orig));
+        ceqbd_J += (-gbd * (vbd - vbd_orig));
+      } else {
+        ceqbs_J -= (-gbs * (vbs - vbs_orig));
+        ceqbd_J -= (-gbd * (vbd - vbd_orig));
+      }
+      W(B3S_FL_B, -(ceqbs_J + ceqbd_J) * np);
+      W(B3S_FL_DP, (ceqbd_J - cdreq_J) * np);
+      W(B3S_FL_SP, (cdreq_J + ceqbs_J) * np);
+    } else {
+      W(B3S_FL_B, -0.0);
+      W(B3S_FL_DP, -0.0);
+      W(B3S_FL_SP, -0.0);
+    }
+    // ---- Master::loadDAEMatrices dF/dx values ("x -= v" stored as "+ (-v)")
+    if (flags & XF_HAS_DP) {
+      W(JF + 0, gdr * np);
+      W(JF + 1, -(gdr * np));
+      W(JF + 8, -(gdr * np));
+    }
+    if (flags & XF_HAS_SP) {
+      W(JF + 2, gsr * np);
+      W(JF + 3, -(gsr * np));
+      W(JF + 15, -(gsr * np));
+    }
+    W(JF + 4, (-gbgs) * np);
+    W(JF + 5, (gbd + gbs - gbbs) * np);
+    W(JF + 6, (-gbd + gbbdp) * np);
+    W(JF + 7, (-gbs + gbbsp) * np);
+    W(JF + 9, -((gbd - Gmbs_ - gbdpb) * np));
+    W(JF + 10, (Gm_ + gbdpg) * np);
+    W(JF + 11, (gdr + gds + gbd + RevSum + gbdpdp) * np);
+    W(JF + 12, -((gds + FwdSum - gbdpsp) * np));
+    W(JF + 13, (-Gm_ + gbspg) * np);
+    W(JF + 14, -((gbs + Gmbs_ - gbspb) * np));
+    W(JF + 16, -((gds + RevSum - gbspdp) * np));
+    W(JF + 17, (gsr + gds + gbs + FwdSum + gbspsp) * np);
+    chk = chk + vbs + vgs + vds + vbd + von;
   }
-  const double Qeqqg = dtype > 0 ? qg : -qg;
-  const double Qeqqb = dtype > 0 ? qb : -qb;
-  const double Qeqqd = dtype > 0 ? qd : -qd;
-  W(B3S_Q_G, Qeqqg * np);
-  W(B3S_Q_B, (Qeqqb)*np);
-  W(B3S_Q_DP, (-(-Qeqqd)) * np);
-  W(B3S_Q_SP, (-(+Qeqqg + Qeqqb + Qeqqd)) * np);
-  if (limiter && !origFlag) {
-    // auxChargeCalculations; with charges off a fresh instance holds zeros
-    double Jg = 0.0, Jb = 0.0, Jd = 0.0;
-    if (charge) {
-      const double vgb_orig = vgs_orig - vbs_orig;
-      const double dvgb = vgb - vgb_orig, dvbd = vbd - vbd_orig,
-                   dvbs = vbs - vbs_orig;
-      Jg = -CAPcggb * dvgb + CAPcgdb * dvbd + CAPcgsb * dvbs;
-      Jb = -CAPcbgb * dvgb + CAPcbdb * dvbd + CAPcbsb * dvbs;
-      Jd = -CAPcdgb * dvgb + CAPcddb * dvbd + CAPcdsb * dvbs;
+  if constexpr (doCV) {
+    const double Qeqqg = dtype > 0 ? qg : -qg;
+    const double Qeqqb = dtype > 0 ? qb : -qb;
+    const double Qeqqd = dtype > 0 ? qd : -qd;
+    W(B3S_Q_G, Qeqqg * np);
+    W(B3S_Q_B, (Qeqqb)*np);
+    W(B3S_Q_DP, (-(-Qeqqd)) * np);
+    W(B3S_Q_SP, (-(+Qeqqg + Qeqqb + Qeqqd)) * np);
+    if (limiter && !origFlag) {
+      // auxChargeCalculations; with charges off a fresh instance holds zeros
+      double Jg = 0.0, Jb = 0.0, Jd = 0.0;
+      if (charge) {
+        const double vgb_orig = vgs_orig - vbs_orig;
+        const double dvgb = vgb - vgb_orig, dvbd = vbd - vbd_orig,
+                     dvbs = vbs - vbs_orig;
+        Jg = -CAPcggb * dvgb + CAPcgdb * dvbd + CAPcgsb * dvbs;
+        Jb = -CAPcbgb * dvgb + CAPcbdb * dvbd + CAPcbsb * dvbs;
+        Jd = -CAPcdgb * dvgb + CAPcddb * dvbd + CAPcdsb * dvbs;
+      }
+      if (dtype < 0) {
+        Jg = -Jg;
+        Jb = -Jb;
+        Jd = -Jd;
+      }
+      W(B3S_QL_G, -Jg * np);
+      W(B3S_QL_B, -(+Jb) * np);
+      W(B3S_QL_DP, (-Jd) * np);
+      W(B3S_QL_SP, (+Jg + Jb + Jd) * np);
+    } else {
+      W(B3S_QL_G, -0.0);
+      W(B3S_QL_B, -0.0);
+      W(B3S_QL_DP, -0.0);
+      W(B3S_QL_SP, -0.0);
     }
-    if (dtype < 0) {
-      Jg = -Jg;
-      Jb = -Jb;
-      Jd = -Jd;
+    // ---- Master::loadDAEMatrices dQ/dx values
+    if (first_tran) {
+      for (int e = 0; e < 16; ++e) W(JQ + e, -0.0);
+    } else {
+      W(JQ + 0, (CAPcggb)*np);
+      W(JQ + 1, -((CAPcggb + CAPcgdb + CAPcgsb) * np));
+      W(JQ + 2, (CAPcgdb)*np);
+      W(JQ + 3, (CAPcgsb)*np);
+      W(JQ + 4, (CAPcbgb)*np);
+      W(JQ + 5, (-CAPcbgb - CAPcbdb - CAPcbsb) * np);
+      W(JQ + 6, (CAPcbdb)*np);
+      W(JQ + 7, (CAPcbsb)*np);
+      W(JQ + 8, -((+CAPcdgb + CAPcddb + CAPcdsb) * np));
+      W(JQ + 9, (CAPcdgb)*np);
+      W(JQ + 10, (+CAPcddb) * np);
+      W(JQ + 11, -((-CAPcdsb) * np));
+      W(JQ + 12, (CAPcsgb)*np);
+      W(JQ + 13, -((+CAPcsgb + CAPcsdb + CAPcssb) * np));
+      W(JQ + 14, -((-CAPcsdb) * np));
+      W(JQ + 15, (+CAPcssb) * np);
     }
-    W(B3S_QL_G, -Jg * np);
-    W(B3S_QL_B, -(+Jb) * np);
-    W(B3S_QL_DP, (-Jd) * np);
-    W(B3S_QL_SP, (+Jg + Jb + Jd) * np);
-  } else {
-    W(B3S_QL_G, -0.0);
-    W(B3S_QL_B, -0.0);
-    W(B3S_QL_DP, -0.0);
-    W(B3S_QL_SP, -0.0);
+    chk = chk + qb + qg + qd + qbs + qbd;
   }
+  // Bit 0: junction voltages limited (the reference's limitedFlag).  Bit 1
+  // (kFull, kDC) or bit 9 (kCV, byte 1 of the word, so that the two waves of
+  // a split evaluation store to different bytes): a non-finite statement,
+  // state or store value (their sum is finite exactly when every term is:
+  // the magnitudes are far from overflow).
+  const bool bad = (chk - chk) != 0.0;
+  if constexpr (kRole == kFull) {
+    a.limited[slot] = (limitedFlag ? 1 : 0) | (bad ? 2 : 0);
+  } else if constexpr (kRole == kDC) {
+    reinterpret_cast<uint8_t *>(a.limited + slot)[0] =
+        (uint8_t)((limitedFlag ? 1 : 0) | (bad ? 2 : 0));
+  } else {
+    reinterpret_cast<uint8_t *>(a.limited + slot)[1] = (uint8_t)(bad ? 2 : 0);
+  }
+}
 
-  // ---- Master::loadDAEMatrices values ("x -= v" stored as "+ (-v)")
-  const int JF = B3S_JF0, JQ = B3S_JQ0;
-  if (flags & XF_HAS_DP) {
-    W(JF + 0, gdr * np);
-    W(JF + 1, -(gdr * np));
-    W(JF + 8, -(gdr * np));
+// Many waves per CU: four waves per block, records read from HBM where the
+// evaluation uses them (other waves hide the latency).
+__global__ void __launch_bounds__(kBlock)
+    bsim3_eval_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p) {
+  __shared__ double card_lds[kWavesPerBlock][B3C_N];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int slot = blockIdx.x * kBlock + tid;
+  const int S = a.n_slots;
+  {
+    const int nwaves = S >> 6;
+    for (int k = tid; k < kWavesPerBlock * B3C_N; k += kBlock) {
+      const int w = k / B3C_N, f = k % B3C_N;
+      const int gw = blockIdx.x * kWavesPerBlock + w;
+      card_lds[w][f] = gw < nwaves ? a.cards[a.wave_card[gw] * B3C_N + f] : 0.0;
+    }
   }
-  if (flags & XF_HAS_SP) {
-    W(JF + 2, gsr * np);
-    W(JF + 3, -(gsr * np));
-    W(JF + 15, -(gsr * np));
-  }
-  W(JF + 4, (-gbgs) * np);
-  W(JF + 5, (gbd + gbs - gbbs) * np);
-  W(JF + 6, (-gbd + gbbdp) * np);
-  W(JF + 7, (-gbs + gbbsp) * np);
-  W(JF + 9, -((gbd - Gmbs_ - gbdpb) * np));
-  W(JF + 10, (Gm_ + gbdpg) * np);
-  W(JF + 11, (gdr + gds + gbd + RevSum + gbdpdp) * np);
-  W(JF + 12, -((gds + FwdSum - gbdpsp) * np));
-  W(JF + 13, (-Gm_ + gbspg) * np);
-  W(JF + 14, -((gbs + Gmbs_ - gbspb) * np));
-  W(JF + 16, -((gds + RevSum - gbspdp) * np));
-  W(JF + 17, (gsr + gds + gbs + FwdSum + gbspsp) * np);
-  if (first_tran) {
-    for (int e = 0; e < 16; ++e) W(JQ + e, -0.0);
+  __syncthreads();
+  if (slot >= S) return;
+  if (a.inst_of[slot] < 0) return;
+  B3Inputs in;
+  b3_inputs(a, ss, p, slot, in);
+  b3_eval<false, kFull>(a, ss, p, slot, card_lds[wave], a.sdp + a.sdp_idx[slot],
+                        a.geo + slot, in);
+}
+
+// Few waves per CU (small circuits, where one wave's latency is the kernel's
+// time): one wave per block, and the wave's 64 instance records (size-
+// dependent and junction fields, kRecN x 64 doubles, 50.7 KB) are copied to
+// LDS with every load in flight at once, so the evaluation never waits on
+// HBM for a parameter.
+// kSplit: two blocks per eval wave, the even one the kDC part and the odd
+// one the kCV part, so that twice as many SIMDs each run a shorter wave.
+constexpr int kRecN = B3R_N + B3I_N;
+
+template <bool kSplit>
+__global__ void __launch_bounds__(64)
+    bsim3_eval_staged_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p) {
+  __shared__ double rec[kRecN * 64];
+  __shared__ double card[B3C_N];
+  const int lane = threadIdx.x;
+  const int w = kSplit ? (blockIdx.x >> 1) : blockIdx.x;
+  const int slot = w * 64 + lane;
+  const double *src = a.rec + (size_t)w * kRecN * 64 + lane;
+  double v[kRecN];
+#pragma unroll
+  for (int f = 0; f < kRecN; ++f) v[f] = src[f * 64];
+  B3Inputs in;
+  b3_inputs(a, ss, p, slot, in);
+  const bool live = a.inst_of[slot] >= 0;
+#pragma unroll
+  for (int f = 0; f < kRecN; ++f) rec[f * 64 + lane] = v[f];
+  if (lane < B3C_N) card[lane] = a.cards[a.wave_card[w] * B3C_N + lane];
+  __syncthreads();
+  if (!live) return;
+  const double *R = rec + lane, *Gp = rec + B3R_N * 64 + lane;
+  if constexpr (kSplit) {
+    if (blockIdx.x & 1)
+      b3_eval<true, kCV>(a, ss, p, slot, card, R, Gp, in);
+    else
+      b3_eval<true, kDC>(a, ss, p, slot, card, R, Gp, in);
   } else {
-    W(JQ + 0, (CAPcggb)*np);
-    W(JQ + 1, -((CAPcggb + CAPcgdb + CAPcgsb) * np));
-    W(JQ + 2, (CAPcgdb)*np);
-    W(JQ + 3, (CAPcgsb)*np);
-    W(JQ + 4, (CAPcbgb)*np);
-    W(JQ + 5, (-CAPcbgb - CAPcbdb - CAPcbsb) * np);
-    W(JQ + 6, (CAPcbdb)*np);
-    W(JQ + 7, (CAPcbsb)*np);
-    W(JQ + 8, -((+CAPcdgb + CAPcddb + CAPcdsb) * np));
-    W(JQ + 9, (CAPcdgb)*np);
-    W(JQ + 10, (+CAPcddb) * np);
-    W(JQ + 11, -((-CAPcdsb) * np));
-    W(JQ + 12, (CAPcsgb)*np);
-    W(JQ + 13, -((+CAPcsgb + CAPcsdb + CAPcssb) * np));
-    W(JQ + 14, -((-CAPcsdb) * np));
-    W(JQ + 15, (+CAPcssb) * np);
+    b3_eval<true, kFull>(a, ss, p, slot, card, R, Gp, in);
   }
-  // Bit 0: junction voltages limited (the reference's limitedFlag).  Bit 1:
-  // a non-finite statement value or next-state/store value (their sum is
-  // finite exactly when every term is: the magnitudes are far from overflow).
-  chk = chk + qb + qg + qd + qbs + qbd + vbs + vgs + vds + vbd + von;
-  a.limited[slot] = (limitedFlag ? 1 : 0) | ((chk - chk) != 0.0 ? 2 : 0);
 }
 
 }  // namespace
 
+// Launched waves up to which the LDS-staged kernel is used: its 50.7 KB of
+// LDS per wave allows three waves per CU on the 256 CUs.
+#ifndef XG_B3_STAGED_MAX_WAVES
+#define XG_B3_STAGED_MAX_WAVES (3 * 256)
+#endif
+
 hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
                              const LoadPtrs &p, hipStream_t s) {
   if (d.n_slots == 0) return hipSuccess;
-  const int grid = (d.n_slots + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(bsim3_eval_kernel, dim3(grid), dim3(kBlock), 0, s, d, ss,
-                     p);
+  const int waves = d.n_slots / 64;
+  int v = d.variant;
+  if (v == XG_EVAL_AUTO)
+    v = 2 * waves <= XG_B3_STAGED_MAX_WAVES ? XG_EVAL_STAGED_SPLIT
+        : waves <= XG_B3_STAGED_MAX_WAVES   ? XG_EVAL_STAGED
+                                            : XG_EVAL_HBM;
+  if (v == XG_EVAL_STAGED_SPLIT) {
+    hipLaunchKernelGGL(bsim3_eval_staged_kernel<true>, dim3(2 * waves),
+                       dim3(64), 0, s, d, ss, p);
+  } else if (v == XG_EVAL_STAGED) {
+    hipLaunchKernelGGL(bsim3_eval_staged_kernel<false>, dim3(waves), dim3(64),
+                       0, s, d, ss, p);
+  } else {
+    const int grid = (d.n_slots + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(bsim3_eval_kernel, dim3(grid), dim3(kBlock), 0, s, d,
+                       ss, p);
+  }
   return hipGetLastError();
 }
 

@@ -86,7 +86,7 @@ struct Bsim3Plan {
   std::vector<std::vector<int>> sdp_of_model;  // model -> sdp indices
   std::vector<int32_t> inst_of, wave_card, lid4, lid_int, sta, sto, flags,
       sdp_idx;
-  std::vector<double> cards, sdp_soa, geo_soa;
+  std::vector<double> cards, sdp_soa, geo_soa, rec_blk;
   std::vector<int32_t> offsets;
   std::vector<double> fields_ref;  // per instance: B3R_N sdp + B3I_N geo
   int64_t slot_base = 0;
@@ -675,6 +675,18 @@ int bsim3_build(Circuit &c, Plan &plan, int64_t slot_base,
     std::memcpy(fr, bp->sdps[in.sdp].r, sizeof(double) * B3R_N);
     std::memcpy(fr + B3R_N, in.geo, sizeof(double) * B3I_N);
   }
+  // wave-blocked copy of the same fields: block w holds field f of lane l
+  // at ((w * kRec) + f) * 64 + l
+  {
+    constexpr int kRec = B3R_N + B3I_N;
+    bp->rec_blk.assign((size_t)S * kRec, 0.0);
+    for (int32_t i = 0; i < n; ++i) {
+      const int32_t sl = slot_of[i];
+      const double *fr = &bp->fields_ref[(size_t)i * kRec];
+      const size_t base = (size_t)(sl / 64) * kRec * 64 + (size_t)(sl % 64);
+      for (int f = 0; f < kRec; ++f) bp->rec_blk[base + (size_t)f * 64] = fr[f];
+    }
+  }
 
   // offsets and reduction contributions in reference order
   auto csr_pos = [&](int32_t r, int32_t col, int32_t &pos) -> bool {
@@ -766,6 +778,7 @@ int bsim3_upload(const Plan &P, Bsim3Dev &d, UpD upd, UpI upi) {
   chk(d.sto_base = upi(b.sto));
   chk(d.flags = upi(b.flags));
   chk(d.sdp_idx = upi(b.sdp_idx));
+  chk(d.rec = upd(b.rec_blk));
   return ok ? 0 : -1;
 }
 

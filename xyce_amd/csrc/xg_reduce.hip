@@ -111,8 +111,8 @@ __global__ void __launch_bounds__(kBlock)
   if (lane == 0) out[t] = r;
 }
 
-// Per-instance flag words: bit 0 limited, bit 1 non-finite.  Block b
-// writes its two counts to out[b] and out[64 + b].
+// Per-instance flag words: bit 0 limited, bit 1 or bit 9 non-finite.
+// Block b writes its two counts to out[b] and out[64 + b].
 __global__ void __launch_bounds__(256)
     count_kernel(const int32_t *flags, int32_t n, int64_t *out) {
   __shared__ int64_t cnt[2][256];
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256)
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
     c0 += flags[i] & 1;
-    c1 += (flags[i] >> 1) & 1;
+    c1 += (flags[i] & 0x202) != 0;
   }
   cnt[0][threadIdx.x] = c0;
   cnt[1][threadIdx.x] = c1;

@@ -135,6 +135,13 @@ class DeviceLoad:
                               acop, transient)
         check(lib().xg_set_solver_state(self.h, ctypes.byref(ss)), "set solver state")
 
+    EVAL_VARIANTS = {"auto": 0, "hbm": 1, "staged": 2, "staged_split": 3}
+
+    def set_eval_variant(self, name):
+        """Force one BSIM3 evaluation kernel form (all give identical
+        results); "auto" chooses by circuit size."""
+        check(lib().xg_set_eval_variant(self.h, self.EVAL_VARIANTS[name]), "eval variant")
+
     def set_stream(self, stream_handle):
         check(lib().xg_set_stream(self.h, ctypes.c_void_p(stream_handle)), "set stream")
 
