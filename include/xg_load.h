@@ -199,6 +199,20 @@ int xg_sync(xg_handle *h);
 /* Stream the handle enqueues on (hipStream_t; 0 = default stream). */
 int xg_set_stream(xg_handle *h, void *stream);
 
+/* Output mode of xg_load_vectors / xg_load_matrices / xg_load_all.
+   XG_OUT_ACCUMULATE (default): every contribution is added to what the
+   output holds, as the reference's loads add into vectors and matrices the
+   assembler zeroed (N_TIA_Assembler.C:156-162; N_DEV_DeviceMgr.C:2314-2323,
+   2503-2515).  XG_OUT_OVERWRITE: every entry of the given outputs is written:
+   contributions are summed from +0.0, in the same order, and entries no
+   MOSFET reaches (and the ground slot, row and column) are set to +0.0 --
+   the same bits as zeroing first and accumulating, without the zeroing
+   pass.  No counterpart in the reference: it folds the assembler's zeroing
+   into the load for callers whose other devices add afterwards. */
+#define XG_OUT_ACCUMULATE 0
+#define XG_OUT_OVERWRITE 1
+int xg_set_output_mode(xg_handle *h, int32_t mode);
+
 /* BSIM3 evaluation kernel selection (no counterpart in the reference: a
    launch-configuration knob for tuning and tests).  XG_EVAL_AUTO picks by
    circuit size; the others force one form.  All forms compute identical

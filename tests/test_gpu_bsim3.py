@@ -182,3 +182,56 @@ def test_bsim3_every_kernel_form(variant, si):
     nl = circuits.bsim3_variety(seed=7)
     print("case", variant, SOLVER_STATES[si], flush=True)
     check(*run_pair(nl, SOLVER_STATES[si], 50 + si, torch, variant=variant))
+
+
+def test_overwrite_mode_equals_zero_then_accumulate():
+    """XG_OUT_OVERWRITE writes every output entry: the bits equal zeroing
+    the outputs and accumulating, except the ground slot/row/column, which
+    it sets to +0.0 (the accumulate mode leaves them to the caller)."""
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd.loader import DeviceLoad
+    nl = circuits.mos1_variety(seed=5, n_dev=40)
+    v = circuits.bsim3_variety(seed=5, n_dev=60)
+    for name, m in v.models.items():
+        nl.add_model("B" + name, m.kind, m.dtype, **m.params)
+    for m in v.mos:
+        nl.add_mos("B" + m.name, m.d, m.g, m.s, m.b, "B" + m.model, **m.params)
+    nl.add_res("R1", "n0", "n1", 100.0)
+    nl.add_cap("C1", "x_only_cap", "0", 1e-15)   # an entry no MOSFET reaches
+    nl.layout()
+    n1 = nl.n_nodes + 1
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-0.2, 1.5, n1)
+    x[-1] = 0.0
+    cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    dev = DeviceLoad(nl)
+    dev.set_solver_state(transient=1, newtonIter=1)
+    sto = rng.uniform(-0.2, 1.5, nl.n_store)
+
+    def run(fill, overwrite):
+        # fresh state/store each time: the load reads the next store it wrote
+        state = [cu(np.zeros(nl.n_state)), cu(sto), cu(np.zeros(nl.n_state)), cu(sto)]
+        dev.set_output_mode(overwrite)
+        outs = [cu(np.full(n1, fill)) for _ in range(4)] + \
+               [cu(np.full(dev.nnz, fill)) for _ in range(2)]
+        dev.load_all(cu(x), *state, *outs)
+        dev.sync()
+        return [t.cpu().numpy() for t in outs]
+
+    acc = run(0.0, False)
+    ovw = run(7.5, True)
+    ovw2 = run(-3.0, True)
+    rowptr, colidx = dev.pattern()
+    rows = np.repeat(np.arange(n1), np.diff(rowptr))
+    gmask = (rows == nl.n_nodes) | (colidx == nl.n_nodes)
+    for k in range(6):
+        a, o, o2 = acc[k].copy(), ovw[k], ovw2[k]
+        g = np.zeros(len(a), bool)
+        if k < 4:
+            g[nl.n_nodes] = True
+        else:
+            g = gmask
+        assert np.all(o[g] == 0.0) and not np.signbit(o[g]).any()
+        a[g] = 0.0
+        assert a.tobytes() == o.tobytes() == o2.tobytes(), k

@@ -62,6 +62,7 @@ SIGNATURES = [
     ("xg_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_set_eval_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("xg_set_output_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),
 ]
 

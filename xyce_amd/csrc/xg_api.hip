@@ -42,6 +42,7 @@ struct Handle {
   // host staging for the *_host variants
   double *pinned = nullptr;
   bool plan_only = false;
+  bool overwrite = false;  // XG_OUT_OVERWRITE
 
   template <class T>
   T *alloc(size_t n) {
@@ -300,9 +301,14 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
       ok = false;
   }
   auto up_prog = [&](const xg::ReduceProgram &rp, xg::ReduceDev &rd) {
+    rd.n_short = (int32_t)rp.sdst.size();
     rd.n_chunk = (int32_t)rp.cdst.size();
     rd.n_multi = (int32_t)rp.mtgt.size();
+    rd.n_zero = (int32_t)rp.zidx.size();
     rd.pbase = rp.pbase;
+    chk(rd.ssrc = h.upload(rp.ssrc));
+    chk(rd.sdst = h.upload(rp.sdst));
+    chk(rd.zidx = h.upload(rp.zidx));
     chk(rd.src = h.upload(rp.src));
     chk(rd.cptr = h.upload(rp.cptr));
     chk(rd.cdst = h.upload(rp.cdst));
@@ -398,6 +404,13 @@ int xg_set_solver_state(xg_handle *H, const xg_solver_state *s) {
   return XG_OK;
 }
 
+int xg_set_output_mode(xg_handle *H, int32_t mode) {
+  if (!H || (mode != XG_OUT_ACCUMULATE && mode != XG_OUT_OVERWRITE))
+    return fail(XG_E_ARG, "xg_set_output_mode: bad argument");
+  H->h.overwrite = mode == XG_OUT_OVERWRITE;
+  return XG_OK;
+}
+
 int xg_set_eval_variant(xg_handle *H, int32_t variant) {
   if (!H || variant < XG_EVAL_AUTO || variant > XG_EVAL_STAGED_SPLIT)
     return fail(XG_E_ARG, "xg_set_eval_variant: bad argument");
@@ -435,7 +448,7 @@ int xg_load_vectors(xg_handle *H, double *F, double *Q, double *dFdxdVp,
     return fail(XG_E_ARG, "xg_load_vectors: null pointer");
   xg::Handle &h = H->h;
   double *outs[4] = {F, Q, dFdxdVp, dQdxdVp};
-  hipError_t e = xg::launch_reduce(4, h.red_vec, outs, h.slots, h.partial, h.stream);
+  hipError_t e = xg::launch_reduce(4, h.red_vec, outs, h.slots, h.partial, h.overwrite, h.stream);
   if (e != hipSuccess) return hip_fail(e, "vector reduction");
   return XG_OK;
 }
@@ -447,7 +460,7 @@ int xg_load_matrices(xg_handle *H, double *dFdx, double *dQdx) {
     return fail(XG_E_ARG, "xg_load_matrices: null pointer");
   xg::Handle &h = H->h;
   double *outs[2] = {dFdx, dQdx};
-  hipError_t e = xg::launch_reduce(2, h.red_mat, outs, h.slots, h.partial, h.stream);
+  hipError_t e = xg::launch_reduce(2, h.red_mat, outs, h.slots, h.partial, h.overwrite, h.stream);
   if (e != hipSuccess) return hip_fail(e, "matrix reduction");
   return XG_OK;
 }
@@ -465,7 +478,7 @@ int xg_load_all(xg_handle *H, const double *x_next, double *sta_curr,
   xg::ReduceDev progs[6] = {h.red_vec[0], h.red_vec[1], h.red_vec[2],
                             h.red_vec[3], h.red_mat[0], h.red_mat[1]};
   double *outs[6] = {F, Q, dFdxdVp, dQdxdVp, dFdx, dQdx};
-  hipError_t e = xg::launch_reduce(6, progs, outs, h.slots, h.partial, h.stream);
+  hipError_t e = xg::launch_reduce(6, progs, outs, h.slots, h.partial, h.overwrite, h.stream);
   if (e != hipSuccess) return hip_fail(e, "reduction");
   return XG_OK;
 }

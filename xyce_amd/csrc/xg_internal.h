@@ -55,32 +55,48 @@ enum Mos1Slot : int {
 
 // ---------------------------------------------------------------------------
 // Ordered reduction program for one output array.  The sources of each
-// output entry (slot indices in the reference's accumulation order; -1
-// means "+ 0.0", a structurally zero reference statement whose only effect
-// is to turn -0.0 into +0.0) are stored contiguously in src and cut into
-// chunks of at most kChunk.  Chunk c covers src[cptr[c] .. cptr[c+1]); if it
-// is the whole segment of entry e then cdst[c] = e, else -1 and its sum
-// goes to partial[pbase + c].  Multi-chunk entry i (mtgt[i]) owns chunks
-// [mchunk[2i], mchunk[2i+1]).
+// output entry are slot indices in the reference's accumulation order; -1
+// means "+ 0.0" (a structurally zero reference statement whose only effect
+// is to turn -0.0 into +0.0) and kPad means "+ -0.0", the exact identity,
+// used to pad fixed-width records.
+//
+//  * short entries (at most kShort sources): record i is the fixed-width
+//    list ssrc[i * kShort .. +kShort) for entry sdst[i];
+//  * the other entries' sources are stored contiguously in src and cut into
+//    chunks of at most kChunk.  Chunk c covers src[cptr[c] .. cptr[c+1]); if
+//    it is the whole segment of entry e then cdst[c] = e, else -1 and its
+//    sum goes to partial[pbase + c].  Multi-chunk entry i (mtgt[i]) owns
+//    chunks [mchunk[2i], mchunk[2i+1]);
+//  * zidx lists the entries no source reaches (and the ground slot, row and
+//    column), which the overwrite output mode sets to +0.0.
+constexpr int32_t kShort = 8;
 constexpr int32_t kChunk = 32;
+constexpr int32_t kPad = -2;
 
 struct ReduceProgram {
+  std::vector<int32_t> ssrc, sdst;
   std::vector<int32_t> src;
   std::vector<int32_t> cptr, cdst;
   std::vector<int32_t> mtgt, mchunk;
+  std::vector<int32_t> zidx;
   int32_t pbase = 0;
 };
 
 // Device-side view of a reduction program.
 struct ReduceDev {
+  int32_t n_short = 0;
   int32_t n_chunk = 0;
   int32_t n_multi = 0;
+  int32_t n_zero = 0;
   int32_t pbase = 0;
+  int32_t *ssrc = nullptr;
+  int32_t *sdst = nullptr;
   int32_t *src = nullptr;
   int32_t *cptr = nullptr;
   int32_t *cdst = nullptr;
   int32_t *mtgt = nullptr;
   int32_t *mchunk = nullptr;
+  int32_t *zidx = nullptr;
 };
 
 struct SolverStateDev {
@@ -115,8 +131,12 @@ struct LoadPtrs {
 // Kernel launchers (defined in the .hip files).
 hipError_t launch_mos1_eval(const Mos1Args &a, const SolverStateDev &ss,
                             const LoadPtrs &p, hipStream_t s);
+// overwrite = false: outputs accumulate ("+=", the reference's loads into
+// zeroed vectors); true: every entry is written, contributions summed from
+// +0.0 and entries without contributions set to +0.0.
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
-                         const double *slots, double *partial, hipStream_t s);
+                         const double *slots, double *partial, bool overwrite,
+                         hipStream_t s);
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s);
 

@@ -4,22 +4,24 @@
 //
 // Every output entry k owns a segment of sources (slot indices) listed in
 // the reference's accumulation order (device type, instance, statement).
-// Segments are cut into chunks of at most kChunk sources.
 //
-//  * A segment of one chunk is folded by one thread starting from out[k]:
+//  * A segment of at most kShort sources is a fixed-width record; one thread
+//    loads the record (two 16-byte loads), gathers its values and folds
 //      acc = out[k]; acc = acc + s0; acc = acc + s1; ...; out[k] = acc
 //    which is exactly the sequence of "*ptr += v" / "vec[li] += v"
 //    statements the reference executes (N_DEV_MOSFET1.C:4441-4722,
-//    N_DEV_MOSFET_B3.C:9654-9928), so these entries are bit-identical to
-//    the reference's order.
+//    N_DEV_MOSFET_B3.C:9654-9928); padding adds -0.0, the exact identity.
+//  * A segment of at most kChunk sources is folded the same way by one
+//    thread from the contiguous source list.
 //  * A longer segment (a supply rail or a heavily shared node) is folded in
 //    two passes: each chunk is summed in order into a partial, then one
 //    wavefront adds the partials to out[k] in chunk order (lane runs in
 //    lane order).  The order is fixed; the result differs from the strictly
 //    sequential sum by rounding only.
 //
-// Sources are gathered eight at a time so that a thread keeps eight loads
-// in flight; the additions themselves stay in order.
+// In the overwrite mode the fold starts from +0.0 instead of out[k] (the
+// same bits as zeroing first) and entries no source reaches are set to
+// +0.0, so the caller needs no zeroing pass.
 #include "xg_internal.h"
 
 namespace xg {
@@ -32,11 +34,16 @@ constexpr int kWaves = kBlock / 64;
 
 struct ReduceBatch {
   int n;
-  int32_t cstart[kMaxProgs + 1];  // chunk work items, prefix over programs
-  int32_t mstart[kMaxProgs + 1];  // multi-chunk segments, prefix
+  int overwrite;
+  // work-item prefixes over the programs: short records, chunks, zero
+  // entries, multi-chunk segments
+  int32_t sstart[kMaxProgs + 1];
+  int32_t cstart[kMaxProgs + 1];
+  int32_t zstart[kMaxProgs + 1];
+  int32_t mstart[kMaxProgs + 1];
   ReduceDev prog[kMaxProgs];
   double *out[kMaxProgs];
-  double *partial;                // scratch for chunk partial sums
+  double *partial;  // scratch for chunk partial sums
 };
 
 __device__ __forceinline__ int find_prog(const int32_t *start, int n, int w) {
@@ -45,6 +52,12 @@ __device__ __forceinline__ int find_prog(const int32_t *start, int n, int w) {
   for (int j = 1; j < kMaxProgs; ++j)
     if (j < n && w >= start[j]) k = j;
   return k;
+}
+
+// Value of one source index: a slot, "+ 0.0" (-1) or the padding identity.
+__device__ __forceinline__ double source_value(int32_t s,
+                                               const double *__restrict__ slots) {
+  return s >= 0 ? slots[s] : (s == -1 ? 0.0 : -0.0);
 }
 
 __device__ __forceinline__ double fold(double acc, const int32_t *__restrict__ src,
@@ -57,38 +70,70 @@ __device__ __forceinline__ double fold(double acc, const int32_t *__restrict__ s
 #pragma unroll
     for (int u = 0; u < 8; ++u) s[u] = src[j + u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = s[u] >= 0 ? slots[s[u]] : 0.0;
+    for (int u = 0; u < 8; ++u) v[u] = source_value(s[u], slots);
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc = acc + v[u];
   }
-  for (; j < end; ++j) {
-    const int32_t s = src[j];
-    acc = acc + (s >= 0 ? slots[s] : 0.0);
-  }
+  for (; j < end; ++j) acc = acc + source_value(src[j], slots);
   return acc;
 }
 
-// Pass 1: one thread per chunk.
+// One launch for every short record, every chunk and (overwrite mode) every
+// zero entry of all programs, in that order of thread ids, so each wave
+// does one kind of work.
 __global__ void __launch_bounds__(kBlock)
-    chunk_kernel(ReduceBatch b, const double *__restrict__ slots) {
-  const int w = blockIdx.x * kBlock + threadIdx.x;
-  if (w >= b.cstart[b.n]) return;
-  const int k = find_prog(b.cstart, b.n, w);
-  const ReduceDev &p = b.prog[k];
-  const int i = w - b.cstart[k];
-  const int32_t beg = p.cptr[i], end = p.cptr[i + 1];
-  const int32_t dst = p.cdst[i];
-  if (dst >= 0) {
-    // whole segment in this chunk: exact sequential fold into the output
+    fold_kernel(ReduceBatch b, const double *__restrict__ slots) {
+  int w = blockIdx.x * kBlock + threadIdx.x;
+  const int n_short = b.sstart[b.n], n_chunk = b.cstart[b.n];
+  if (w < n_short) {
+    const int k = find_prog(b.sstart, b.n, w);
+    const ReduceDev &p = b.prog[k];
+    const int i = w - b.sstart[k];
+    const int4 *rec = reinterpret_cast<const int4 *>(p.ssrc) + 2 * (size_t)i;
+    const int4 r0 = rec[0], r1 = rec[1];
+    const int32_t dst = p.sdst[i];
     double *out = b.out[k];
-    out[dst] = fold(out[dst], p.src, beg, end, slots);
-  } else {
-    // -0.0 is the additive identity, so the chunk sum starts exactly
-    b.partial[p.pbase + i] = fold(-0.0, p.src, beg, end, slots);
+    const double v0 = source_value(r0.x, slots), v1 = source_value(r0.y, slots),
+                 v2 = source_value(r0.z, slots), v3 = source_value(r0.w, slots),
+                 v4 = source_value(r1.x, slots), v5 = source_value(r1.y, slots),
+                 v6 = source_value(r1.z, slots), v7 = source_value(r1.w, slots);
+    double acc = b.overwrite ? 0.0 : out[dst];
+    acc = acc + v0;
+    acc = acc + v1;
+    acc = acc + v2;
+    acc = acc + v3;
+    acc = acc + v4;
+    acc = acc + v5;
+    acc = acc + v6;
+    acc = acc + v7;
+    out[dst] = acc;
+    return;
+  }
+  w -= n_short;
+  if (w < n_chunk) {
+    const int k = find_prog(b.cstart, b.n, w);
+    const ReduceDev &p = b.prog[k];
+    const int i = w - b.cstart[k];
+    const int32_t beg = p.cptr[i], end = p.cptr[i + 1];
+    const int32_t dst = p.cdst[i];
+    if (dst >= 0) {
+      // whole segment in this chunk: sequential fold into the output
+      double *out = b.out[k];
+      out[dst] = fold(b.overwrite ? 0.0 : out[dst], p.src, beg, end, slots);
+    } else {
+      // -0.0 is the additive identity, so the chunk sum starts exactly
+      b.partial[p.pbase + i] = fold(-0.0, p.src, beg, end, slots);
+    }
+    return;
+  }
+  w -= n_chunk;
+  if (b.overwrite && w < b.zstart[b.n]) {
+    const int k = find_prog(b.zstart, b.n, w);
+    b.out[k][b.prog[k].zidx[w - b.zstart[k]]] = 0.0;
   }
 }
 
-// Pass 2: one wavefront per multi-chunk segment.
+// Second pass: one wavefront per multi-chunk segment.
 __global__ void __launch_bounds__(kBlock)
     combine_kernel(ReduceBatch b) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -106,7 +151,7 @@ __global__ void __launch_bounds__(kBlock)
   double acc = -0.0;
   for (int32_t c = rb; c < re; ++c) acc = acc + b.partial[p.pbase + c];
   double *out = b.out[k];
-  double r = out[t];
+  double r = b.overwrite ? 0.0 : out[t];
   for (int l = 0; l < 64; ++l) r = r + __shfl(acc, l, 64);
   if (lane == 0) out[t] = r;
 }
@@ -141,25 +186,35 @@ __global__ void __launch_bounds__(256)
 }  // namespace
 
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
-                         const double *slots, double *partial, hipStream_t s) {
+                         const double *slots, double *partial, bool overwrite,
+                         hipStream_t s) {
   if (n_out <= 0 || n_out > kMaxProgs) return hipErrorInvalidValue;
   ReduceBatch b{};
   b.n = n_out;
+  b.overwrite = overwrite ? 1 : 0;
   b.partial = partial;
   for (int k = 0; k < n_out; ++k) {
     b.prog[k] = progs[k];
     b.out[k] = outs[k];
-    if (!outs[k] && progs[k].n_chunk) return hipErrorInvalidValue;
+    const int32_t items = progs[k].n_short + progs[k].n_chunk +
+                          (overwrite ? progs[k].n_zero : 0);
+    if (!outs[k] && items) return hipErrorInvalidValue;
+    b.sstart[k + 1] = b.sstart[k] + progs[k].n_short;
     b.cstart[k + 1] = b.cstart[k] + progs[k].n_chunk;
+    b.zstart[k + 1] = b.zstart[k] + progs[k].n_zero;
     b.mstart[k + 1] = b.mstart[k] + progs[k].n_multi;
   }
   for (int k = n_out; k < kMaxProgs; ++k) {
+    b.sstart[k + 1] = b.sstart[n_out];
     b.cstart[k + 1] = b.cstart[n_out];
+    b.zstart[k + 1] = b.zstart[n_out];
     b.mstart[k + 1] = b.mstart[n_out];
   }
-  const int n_chunks = b.cstart[n_out], n_multi = b.mstart[n_out];
-  if (n_chunks == 0) return hipSuccess;
-  hipLaunchKernelGGL(chunk_kernel, dim3((n_chunks + kBlock - 1) / kBlock),
+  const int64_t items = (int64_t)b.sstart[n_out] + b.cstart[n_out] +
+                        (overwrite ? b.zstart[n_out] : 0);
+  const int n_multi = b.mstart[n_out];
+  if (items == 0) return hipSuccess;
+  hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((items + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, s, b, slots);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || n_multi == 0) return e;

@@ -347,7 +347,8 @@ static inline int32_t mos1_node(const Mos1Inst &in, int local) {
 // Groups the contributions by output entry, keeping the reference's order
 // within each entry.  Entries flagged in `skip` (the ground row and column:
 // the reference writes them into the ground slot that N_LAS discards before
-// the solve, N_LAS_Builder.C:424-434) get no work item.
+// the solve, N_LAS_Builder.C:424-434) get no summing work item; they and the
+// entries without contributions are listed for the overwrite mode.
 static void finish_program(const std::vector<Contrib> &cs, int32_t n_out,
                            const std::vector<char> &skip,
                            ReduceProgram &prog) {
@@ -357,14 +358,26 @@ static void finish_program(const std::vector<Contrib> &cs, int32_t n_out,
   std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
   std::vector<int32_t> srcs(cs.size());
   for (auto &c : cs) srcs[pos[c.out]++] = c.src;
+  prog.ssrc.clear();
+  prog.sdst.clear();
   prog.src.clear();
   prog.cptr.assign(1, 0);
   prog.cdst.clear();
   prog.mtgt.clear();
   prog.mchunk.clear();
+  prog.zidx.clear();
   for (int32_t k = 0; k < n_out; ++k) {
     const int32_t len = cnt[k + 1] - cnt[k];
-    if (len == 0 || skip[k]) continue;
+    if (len == 0 || skip[k]) {
+      prog.zidx.push_back(k);
+      continue;
+    }
+    if (len <= kShort) {
+      for (int32_t q = 0; q < kShort; ++q)
+        prog.ssrc.push_back(q < len ? srcs[cnt[k] + q] : kPad);
+      prog.sdst.push_back(k);
+      continue;
+    }
     const int32_t first_chunk = (int32_t)prog.cdst.size();
     for (int32_t j = cnt[k]; j < cnt[k + 1]; j += kChunk) {
       const int32_t e = std::min(cnt[k + 1], j + kChunk);

@@ -137,6 +137,12 @@ class DeviceLoad:
 
     EVAL_VARIANTS = {"auto": 0, "hbm": 1, "staged": 2, "staged_split": 3}
 
+    def set_output_mode(self, overwrite):
+        """overwrite=False: loads add into the outputs (the reference's
+        "+=" into zeroed vectors); True: loads write every entry, so the
+        outputs need no zeroing first."""
+        check(lib().xg_set_output_mode(self.h, 1 if overwrite else 0), "output mode")
+
     def set_eval_variant(self, name):
         """Force one BSIM3 evaluation kernel form (all give identical
         results); "auto" chooses by circuit size."""
