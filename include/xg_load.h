@@ -126,6 +126,23 @@ int xg_add_bsim3_instance(xg_circuit *c, int32_t model_id, int32_t n,
                           const int32_t lids[6], int32_t sta_base,
                           int32_t sto_base, int32_t *inst_id);
 
+/* Linear devices, loaded after the MOSFETs through the same ordered
+   reduction (device types in the order resistor, capacitor, source; each in
+   the order added).  Node ids: -1 = ground.
+   Resistor: N_DEV_Resistor.C:1073-1174 (G = 1 / r).
+   Capacitor: N_DEV_Capacitor.C:1821-2060 (q = C (v+ - v-), written to state
+   slot sta; no IC=).
+   Voltage source: N_DEV_Vsrc.C:941-1036, branch current unknown at
+   `branch`; pulse = {V1, V2, TD, TR, TF, PW, PER} (defaults resolved by the
+   caller, N_DEV_SourceData.C:902-916) or NULL for a DC source; the pulse
+   applies at tranop / transient iterations, dc otherwise
+   (N_DEV_SourceData.C:928-1000). */
+int xg_add_resistor(xg_circuit *c, int32_t pos, int32_t neg, double r);
+int xg_add_capacitor(xg_circuit *c, int32_t pos, int32_t neg, double cap,
+                     int32_t sta);
+int xg_add_vsource(xg_circuit *c, int32_t pos, int32_t neg, int32_t branch,
+                   double dc, const double *pulse);
+
 /* Stamp of a device the library does not evaluate (resistor, capacitor,
    source ...), so that the CSR pattern is the whole circuit's.
    gids[n_nodes_dev] are the device's solution ids (-1 = ground); entry e is
@@ -222,6 +239,11 @@ int xg_set_output_mode(xg_handle *h, int32_t mode);
 #define XG_EVAL_STAGED 2       /* 1 wave per block, records staged in LDS   */
 #define XG_EVAL_STAGED_SPLIT 3 /* as STAGED, DC and charge parts on two waves */
 int xg_set_eval_variant(xg_handle *h, int32_t variant);
+
+/* Time (seconds) at which the next xg_update_state evaluates PULSE sources,
+   and the breakpoint tolerance of the PULSE corners (SolverState::bpTol,
+   N_DEV_DeviceMgr.C:3488). */
+int xg_set_time(xg_handle *h, double time, double bp_tol);
 
 /* Sizes the library was finalized with. */
 int xg_sizes(const xg_handle *h, int32_t *n_nodes, int32_t *n_state,

@@ -160,6 +160,8 @@ class RefLoad:
         if nl.bsim3:
             from . import bsim3_bind
             self.b3 = bsim3_bind.RefBsim3(self, nl, gmin, limiter, temp)
+        from .linear_ref import RefLinear
+        self.lin = RefLinear(nl, self.rowptr, self.colidx)
 
     def set_solver_state(self, dcop=0, initJct=0, initFix=0, initTran=0, newtonIter=0,
                          tranop=0, acop=0, transient=0):
@@ -169,16 +171,22 @@ class RefLoad:
         lib().ref_mos1_set_solver_state(self.m1, _i(f))
         if self.b3:
             self.b3.set_solver_state(f)
+        self.lin.set_solver_state(tranop=tranop, transient=transient)
+
+    def set_time(self, t, bp_tol=0.0):
+        self.lin.set_time(t, bp_tol)
 
     def update_state(self, x, sta_c, sto_c, sta_n, sto_n):
         lib().ref_mos1_update_state(self.m1, _d(x), _d(sta_c), _d(sto_c), _d(sta_n), _d(sto_n))
         if self.b3:
             self.b3.update_state(x, sta_c, sto_c, sta_n, sto_n)
+        self.lin.update_state(x, sta_n)
 
     def load_vectors(self, F, Q, dFdxdVp, dQdxdVp):
         lib().ref_mos1_load_vectors(self.m1, _d(F), _d(Q), _d(dFdxdVp), _d(dQdxdVp))
         if self.b3:
             self.b3.load_vectors(F, Q, dFdxdVp, dQdxdVp)
+        self.lin.load_vectors(F, Q)
 
     def load_matrices(self):
         """Adds into self.dFdx / self.dQdx (zero them first, as the
@@ -186,6 +194,7 @@ class RefLoad:
         lib().ref_mos1_load_matrices(self.m1)
         if self.b3:
             self.b3.load_matrices()
+        self.lin.load_matrices(self.dFdx, self.dQdx)
 
     def mos1_flags(self):
         n = len(self.nl.mos1)

@@ -240,3 +240,51 @@ def test_jacobian_matches_finite_differences(oracle_lib, ci):
             # ten times tighter than the reference's relTol of 1e-2.
             _fd_check(nl, x, DC, "F", rel=1e-3, absf=1e-9)
             _fd_check(nl, x, TRAN, "FQ", rel=1e-3, absf=1e-9)
+
+
+# ---------------------------------------------------------------- linear devices
+def _pulse_brute(p, t):
+    """PULSE as the piecewise-linear waveform through its corner points
+    (periodic after TD), without tolerance handling."""
+    V1, V2, TD, TR, TF, PW, PER = p
+    tt = t - TD
+    if tt <= 0:
+        return V1
+    if PER:
+        tt = tt % PER if tt > PER else tt
+    return float(np.interp(tt, [0.0, TR, TR + PW, TR + PW + TF], [V1, V2, V2, V1],
+                           right=V1))
+
+
+def test_pulse_waveform_matches_piecewise_linear():
+    from oracle.linear_ref import pulse_value
+    p = [0.0, 1.8, 0.1e-9, 0.05e-9, 0.08e-9, 0.4e-9, 1.0e-9]
+    for t in np.linspace(-0.2e-9, 3.3e-9, 701):
+        assert abs(pulse_value(p, t, 0.0) - _pulse_brute(p, t)) < 1e-12
+
+
+def test_linear_devices_closed_form():
+    """R, C and V statements give F, Q, dF/dx, dQ/dx of the ideal elements:
+    KCL at every node, Ohm's law, q = C v, and the source branch equation."""
+    nl = Netlist("rc + source")
+    nl.add_vsrc("V1", "a", "0", 1.5)
+    nl.add_res("R1", "a", "b", 250.0)
+    nl.add_cap("C1", "b", "0", 2e-12)
+    nl.add_res("R2", "b", "c", 1e3)
+    nl.add_cap("C2", "b", "c", 5e-13)
+    nl.layout()
+    x = np.zeros(nl.n_nodes + 1)
+    a, b, c = nl.node("a"), nl.node("b"), nl.node("c")
+    br = nl.vsrc_branch[0]
+    x[a], x[b], x[c], x[br] = 1.2, 0.7, 0.2, -3e-3
+    F, Q, FL, QL, JF, JQ = load(nl, x, DC)
+    i1, i2 = (1.2 - 0.7) / 250.0, (0.7 - 0.2) / 1e3
+    assert abs(F[a] - (i1 + x[br])) < 1e-15
+    assert abs(F[b] - (-i1 + i2)) < 1e-15
+    assert abs(F[c] - (-i2)) < 1e-15
+    assert abs(F[br] - (1.2 - 1.5)) < 1e-15
+    assert abs(Q[b] - (2e-12 * 0.7 + 5e-13 * 0.5)) < 1e-27
+    assert abs(Q[c] + 5e-13 * 0.5) < 1e-27
+    assert JF[a, a] == 1 / 250.0 and JF[a, br] == 1.0 and JF[br, a] == 1.0
+    assert JQ[b, b] == 2e-12 + 5e-13 and JQ[b, c] == -5e-13
+    assert not FL.any() and not QL.any()

@@ -61,6 +61,14 @@ SIGNATURES = [
     ("xg_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("xg_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_add_resistor", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_double]),
+    ("xg_add_capacitor", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_double, ctypes.c_int32]),
+    ("xg_add_vsource", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_double)]),
+    ("xg_set_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     ("xg_set_eval_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_set_output_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),

@@ -22,6 +22,7 @@ SOURCES = [
     "xg_mos1.hip",
     "xg_bsim3.hip",
     "xg_reduce.hip",
+    "xg_linear.hip",
     "xg_setup.cpp",
     "xg_bsim3_setup.cpp",
 ]

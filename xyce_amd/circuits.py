@@ -40,13 +40,14 @@ class Mosfet:
 
 @dataclass
 class Pulse:
+    """PULSE(V1 V2 TD TR TF PW PER); None = the reference's default."""
     v1: float
     v2: float
-    td: float
-    tr: float
-    tf: float
-    pw: float
-    per: float
+    td: float = 0.0
+    tr: float = None
+    tf: float = None
+    pw: float = None
+    per: float = None
 
 
 @dataclass
@@ -154,9 +155,39 @@ class Netlist:
         for v in self.vsrc:
             self.vsrc_branch.append(n)
             n += 1
+        # one charge state slot per capacitor (N_DEV_Capacitor.C li_QState)
+        self.cap_sta = []
+        for _ in self.caps:
+            self.cap_sta.append(sta)
+            sta += 1
         self.n_nodes, self.n_state, self.n_store = n, sta, sto
         self._laid_out = True
         return self
+
+    def pulse_params(self, v, tstep=1e-12, tstop=1e-9):
+        """The seven PULSE values with the reference's defaults for omitted
+        ones (N_DEV_SourceData.C:902-916: TR, TF = starting step; PW, PER =
+        final time), or None for a DC source."""
+        if v.pulse is None:
+            return None
+        p = v.pulse
+        tr = p.tr if p.tr is not None else tstep
+        tf = p.tf if p.tf is not None else tstep
+        pw = p.pw if p.pw is not None else tstop
+        per = p.per if p.per is not None else tstop
+        return [p.v1, p.v2, p.td or 0.0, tr, tf, pw, per]
+
+    def linear_devices(self):
+        """(kind, pos, neg, branch, value, pulse, sta) in load order:
+        resistors, capacitors, sources, each in netlist order."""
+        out = []
+        for name, a, b, r in self.res:
+            out.append(("R", self.node(a), self.node(b), -1, r, None, 0))
+        for (name, a, b, c), st in zip(self.caps, self.cap_sta):
+            out.append(("C", self.node(a), self.node(b), -1, c, None, st))
+        for v, br in zip(self.vsrc, self.vsrc_branch):
+            out.append(("V", self.node(v.p), self.node(v.n), br, v.dc, self.pulse_params(v), 0))
+        return out
 
     def foreign_stamps(self):
         """(gids, rows, cols) of every device the load library does not

@@ -32,6 +32,9 @@ struct Handle {
   Mos1Args m1{};
   // BSIM3 device data
   Bsim3Dev b3{};
+  // linear devices and the time their sources are evaluated at
+  LinArgs lin{};
+  double time = 0.0, bp_tol = 0.0;
   // reduction programs
   ReduceDev red_vec[4]{}, red_mat[2]{};
   double *slots = nullptr;
@@ -209,6 +212,54 @@ int xg_add_bsim3_instance(xg_circuit *c, int32_t model_id, int32_t n,
   return rc ? fail(rc, err) : XG_OK;
 }
 
+static int add_lin(xg_circuit *c, int32_t type, int32_t pos, int32_t neg,
+                   int32_t br, double val, const double *pulse, int32_t sta,
+                   const char *who) {
+  if (!c) return fail(XG_E_ARG, std::string(who) + ": bad argument");
+  const int32_t n = c->c.n_nodes;
+  if (pos >= n || neg >= n || br >= n || (type == xg::LIN_V && br < 0))
+    return fail(XG_E_ARG, std::string(who) + ": node id out of range");
+  if (type == xg::LIN_C && (sta < 0 || sta >= c->c.n_state))
+    return fail(XG_E_ARG, std::string(who) + ": state slot out of range");
+  if (type == xg::LIN_R && val == 0.0)
+    return fail(XG_E_PARAM, std::string(who) + ": zero resistance");
+  xg::LinDev d{};
+  d.type = type;
+  d.lids[0] = pos < 0 ? -1 : pos;
+  d.lids[1] = neg < 0 ? -1 : neg;
+  d.lids[2] = br < 0 ? -1 : br;
+  d.val = val;
+  d.sta = sta;
+  if (pulse) {
+    d.pulse[0] = 1.0;
+    for (int k = 0; k < 7; ++k) d.pulse[k + 1] = pulse[k];
+  }
+  c->c.lin.push_back(d);
+  return XG_OK;
+}
+
+int xg_add_resistor(xg_circuit *c, int32_t pos, int32_t neg, double r) {
+  return add_lin(c, xg::LIN_R, pos, neg, -1, r, nullptr, 0, "xg_add_resistor");
+}
+
+int xg_add_capacitor(xg_circuit *c, int32_t pos, int32_t neg, double cap,
+                     int32_t sta) {
+  return add_lin(c, xg::LIN_C, pos, neg, -1, cap, nullptr, sta,
+                 "xg_add_capacitor");
+}
+
+int xg_add_vsource(xg_circuit *c, int32_t pos, int32_t neg, int32_t branch,
+                   double dc, const double *pulse) {
+  return add_lin(c, xg::LIN_V, pos, neg, branch, dc, pulse, 0, "xg_add_vsource");
+}
+
+int xg_set_time(xg_handle *H, double time, double bp_tol) {
+  if (!H) return fail(XG_E_ARG, "xg_set_time: bad argument");
+  H->h.time = time;
+  H->h.bp_tol = bp_tol;
+  return XG_OK;
+}
+
 int xg_add_foreign_stamp(xg_circuit *c, int32_t n_dev_nodes,
                          const int32_t *gids, int32_t n_entries,
                          const int32_t *rows, const int32_t *cols) {
@@ -290,7 +341,15 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
   h.n_limited_slots = P.m1_slots + xg::bsim3_eval_slots(P);
   chk(h.limited_all = h.alloc<int32_t>((size_t)h.n_limited_slots));
   chk(h.count_buf = h.alloc<int64_t>(128));
+  // linear devices
+  h.lin.n_slots = P.lin_slots;
+  chk(h.lin.type = h.upload(P.lin_type));
+  chk(h.lin.lid = h.upload(P.lin_lid));
+  chk(h.lin.val = h.upload(P.lin_val));
+  chk(h.lin.pulse = h.upload(P.lin_pulse));
+  chk(h.lin.sta = h.upload(P.lin_sta));
   if (ok) {
+    h.lin.slots = h.slots + P.lin_slot_base;
     h.m1.slots = h.slots;
     h.m1.limited = h.limited_all;
     xg::bsim3_bind(h.b3, h.slots, h.limited_all + P.m1_slots);
@@ -437,6 +496,8 @@ int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
   if (e != hipSuccess) return hip_fail(e, "mos1 eval");
   e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream);
   if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
+  e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "linear eval");
   return XG_OK;
 }
 

@@ -99,6 +99,21 @@ struct ReduceDev {
   int32_t *zidx = nullptr;
 };
 
+// Linear devices (resistor, capacitor, independent voltage source), loaded
+// through the same statement / ordered-reduction path as the MOSFETs.
+enum LinType : int32_t { LIN_R = 0, LIN_C = 1, LIN_V = 2 };
+constexpr int kLinStatements = 8;
+
+struct LinArgs {
+  int32_t n_slots = 0;            // padded (multiple of 64), grouped by type
+  const int32_t *type = nullptr;  // n_slots; -1 = padding
+  const int32_t *lid = nullptr;   // 3 x n_slots: pos, neg, branch (ground = n_nodes)
+  const double *val = nullptr;    // n_slots: G (R), C (C), DC value (V)
+  const double *pulse = nullptr;  // 8 x n_slots: has_pulse, V1, V2, TD, TR, TF, PW, PER
+  const int32_t *sta = nullptr;   // n_slots: capacitor charge state slot
+  double *slots = nullptr;        // kLinStatements x n_slots (in the shared buffer)
+};
+
 struct SolverStateDev {
   int32_t dcop, initJct, initFix, initTran, newtonIter, tranop, acop,
       transient;
@@ -137,6 +152,9 @@ hipError_t launch_mos1_eval(const Mos1Args &a, const SolverStateDev &ss,
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
                          const double *slots, double *partial, bool overwrite,
                          hipStream_t s);
+hipError_t launch_linear_eval(const LinArgs &a, const SolverStateDev &ss,
+                              const LoadPtrs &p, double time, double bp_tol,
+                              hipStream_t s);
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s);
 

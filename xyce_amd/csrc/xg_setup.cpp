@@ -442,6 +442,22 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
   for (auto &fs : c.foreign)
     for (size_t e = 0; e < fs.rows.size(); ++e)
       entries.push_back({L(fs.gids[fs.rows[e]]), L(fs.gids[fs.cols[e]])});
+  for (auto &ld : c.lin) {
+    const int32_t p = L(ld.lids[0]), q = L(ld.lids[1]), b = L(ld.lids[2]);
+    if (ld.type == LIN_V) {
+      // N_DEV_Vsrc.C jacStamp: (pos, br), (neg, br), (br, pos), (br, neg)
+      entries.push_back({p, b});
+      entries.push_back({q, b});
+      entries.push_back({b, p});
+      entries.push_back({b, q});
+    } else {
+      // N_DEV_Resistor.C / N_DEV_Capacitor.C: the full 2 x 2 block
+      entries.push_back({p, p});
+      entries.push_back({p, q});
+      entries.push_back({q, p});
+      entries.push_back({q, q});
+    }
+  }
 
   if (c.have_pattern) {
     plan.rowptr = c.rowptr;
@@ -599,6 +615,89 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     if (rc) return rc;
   }
   slot_total += bsim3_slot_count(plan);
+
+  // ---- linear devices, after the MOSFETs; resistors, then capacitors,
+  //      then sources (one Master loop per device type), each in the order
+  //      they were added, each type padded to whole waves
+  {
+    std::vector<int32_t> order;
+    for (int t = LIN_R; t <= LIN_V; ++t) {
+      for (int32_t i = 0; i < (int32_t)c.lin.size(); ++i)
+        if (c.lin[i].type == t) order.push_back(i);
+      while (order.size() % 64) order.push_back(-1);
+    }
+    // drop the padding of empty groups
+    std::vector<int32_t> ord2;
+    for (size_t w = 0; w < order.size(); w += 64) {
+      bool any = false;
+      for (size_t j = w; j < w + 64; ++j) any = any || order[j] >= 0;
+      if (any) ord2.insert(ord2.end(), order.begin() + w, order.begin() + w + 64);
+    }
+    const int32_t SL = (int32_t)ord2.size();
+    plan.lin_n = (int32_t)c.lin.size();
+    plan.lin_slots = SL;
+    plan.lin_slot_base = slot_total;
+    plan.lin_type.assign(SL, -1);
+    plan.lin_lid.assign((size_t)3 * SL, G);
+    plan.lin_sta.assign(SL, 0);
+    plan.lin_val.assign(SL, 0.0);
+    plan.lin_pulse.assign((size_t)8 * SL, 0.0);
+    auto LS = [&](int e, int32_t s) -> int32_t {
+      return (int32_t)(slot_total + (int64_t)e * SL + s);
+    };
+    // vectors first for every device (loadDAEVectors), matrices after
+    // (loadDAEMatrices), each in device-type then instance order
+    for (int32_t s = 0; s < SL; ++s) {
+      const int32_t i = ord2[s];
+      if (i < 0) continue;
+      const LinDev &ld = c.lin[i];
+      const int32_t p = L(ld.lids[0]), q = L(ld.lids[1]), b = L(ld.lids[2]);
+      plan.lin_type[s] = ld.type;
+      plan.lin_lid[s] = p;
+      plan.lin_lid[(size_t)SL + s] = q;
+      plan.lin_lid[(size_t)2 * SL + s] = b;
+      plan.lin_sta[s] = ld.sta;
+      plan.lin_val[s] = ld.type == LIN_R ? 1.0 / ld.val : ld.val;
+      for (int k = 0; k < 8; ++k) plan.lin_pulse[(size_t)k * SL + s] = ld.pulse[k];
+      if (ld.type == LIN_R) {          // N_DEV_Resistor.C:1118-1131
+        vec[0].push_back({p, LS(0, s)});
+        vec[0].push_back({q, LS(1, s)});
+      } else if (ld.type == LIN_C) {   // N_DEV_Capacitor.C:1881-1960
+        vec[1].push_back({p, LS(0, s)});
+        vec[1].push_back({q, LS(1, s)});
+      } else {                         // N_DEV_Vsrc.C:1002-1013
+        vec[0].push_back({p, LS(0, s)});
+        vec[0].push_back({q, LS(1, s)});
+        vec[0].push_back({b, LS(2, s)});
+        vec[0].push_back({b, LS(3, s)});
+      }
+    }
+    for (int32_t s = 0; s < SL; ++s) {
+      const int32_t i = ord2[s];
+      if (i < 0) continue;
+      const LinDev &ld = c.lin[i];
+      const int32_t p = L(ld.lids[0]), q = L(ld.lids[1]), b = L(ld.lids[2]);
+      int32_t rc_[4][2];
+      if (ld.type == LIN_V) {          // N_DEV_Vsrc.C:1023-1036
+        const int32_t t[4][2] = {{p, b}, {q, b}, {b, p}, {b, q}};
+        std::memcpy(rc_, t, sizeof(t));
+      } else {                         // Resistor.C:1159-1174, Capacitor.C:2044-2050
+        const int32_t t[4][2] = {{p, p}, {p, q}, {q, p}, {q, q}};
+        std::memcpy(rc_, t, sizeof(t));
+      }
+      const int e0 = ld.type == LIN_V ? 4 : 2;
+      auto &prog = ld.type == LIN_C ? mat[1] : mat[0];
+      for (int k = 0; k < 4; ++k) {
+        int32_t pos;
+        if (!csr_pos(rc_[k][0], rc_[k][1], pos)) {
+          err = "supplied CSR pattern lacks a linear-device stamp entry";
+          return XG_E_PATTERN;
+        }
+        prog.push_back({pos, LS(e0 + k, s)});
+      }
+    }
+    slot_total += (int64_t)kLinStatements * SL;
+  }
   plan.n_slot_values = slot_total;
   if (slot_total > INT32_MAX) {
     err = "slot buffer exceeds 2^31 values";

@@ -54,6 +54,15 @@ struct Mos1Inst {
   double gd, gs;  // drain / source conductance
 };
 
+// A linear device as added through the C ABI.
+struct LinDev {
+  int32_t type;        // LIN_R / LIN_C / LIN_V
+  int32_t lids[3];     // pos, neg, branch (-1 = ground)
+  double val;          // R (ohm), C (farad) or DC value (volt)
+  double pulse[8];     // has_pulse, V1, V2, TD, TR, TF, PW, PER
+  int32_t sta;         // capacitor charge state slot
+};
+
 struct ForeignStamp {
   std::vector<int32_t> gids;
   std::vector<int32_t> rows, cols;
@@ -72,6 +81,7 @@ struct Circuit {
   std::vector<Bsim3Model *> b3_models;
   std::vector<Bsim3Inst *> b3;
   std::vector<ForeignStamp> foreign;
+  std::vector<LinDev> lin;
   bool have_pattern = false;
   std::vector<int32_t> rowptr, colidx;
   ~Circuit();
@@ -91,6 +101,11 @@ struct Plan {
   int64_t m1_slot_base = 0;
   // BSIM3
   Bsim3Plan *b3 = nullptr;
+  // linear devices (eval order: resistors, capacitors, sources)
+  int32_t lin_n = 0, lin_slots = 0;
+  std::vector<int32_t> lin_type, lin_lid, lin_sta;
+  std::vector<double> lin_val, lin_pulse;
+  int64_t lin_slot_base = 0;
   // slot buffer and reduction programs
   int64_t n_slot_values = 0;
   ReduceProgram red_vec[4];  // F, Q, dFdxdVp, dQdxdVp

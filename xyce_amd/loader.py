@@ -58,12 +58,14 @@ class DeviceLoad:
                 check(L.xg_add_bsim3_instance(c, mid[rec["model"]], n, k, v, lids,
                                               rec["sta"], rec["sto"], ctypes.byref(out)),
                       "add BSIM3 " + rec["name"])
-            for gids, rows, cols in nl.foreign_stamps():
-                g = (ctypes.c_int32 * len(gids))(*gids)
-                r = (ctypes.c_int32 * len(rows))(*rows)
-                cc = (ctypes.c_int32 * len(cols))(*cols)
-                check(L.xg_add_foreign_stamp(c, len(gids), g, len(rows), r, cc),
-                      "add stamp")
+            for kind, p, q, br, val, pulse, st in nl.linear_devices():
+                if kind == "R":
+                    check(L.xg_add_resistor(c, p, q, val), "add resistor")
+                elif kind == "C":
+                    check(L.xg_add_capacitor(c, p, q, val, st), "add capacitor")
+                else:
+                    pp = (ctypes.c_double * 7)(*pulse) if pulse else None
+                    check(L.xg_add_vsource(c, p, q, br, val, pp), "add source")
             if pattern is not None:
                 rp, ci = (np.ascontiguousarray(a, dtype=np.int32) for a in pattern)
                 check(L.xg_circuit_set_pattern(
@@ -147,6 +149,10 @@ class DeviceLoad:
         """Force one BSIM3 evaluation kernel form (all give identical
         results); "auto" chooses by circuit size."""
         check(lib().xg_set_eval_variant(self.h, self.EVAL_VARIANTS[name]), "eval variant")
+
+    def set_time(self, t, bp_tol=0.0):
+        """Time at which the next update_state evaluates PULSE sources."""
+        check(lib().xg_set_time(self.h, float(t), float(bp_tol)), "set time")
 
     def set_stream(self, stream_handle):
         check(lib().xg_set_stream(self.h, ctypes.c_void_p(stream_handle)), "set stream")
