@@ -240,6 +240,42 @@ int xg_set_output_mode(xg_handle *h, int32_t mode);
 #define XG_EVAL_STAGED_SPLIT 3 /* as STAGED, DC and charge parts on two waves */
 int xg_set_eval_variant(xg_handle *h, int32_t variant);
 
+/* ---- Newton iteration linear algebra (the rows after the device load) */
+
+/* Independent sub-circuits: connected components of the circuit graph with
+   ground removed.  comp_of_node (n_nodes entries) may be NULL. */
+int xg_components(const xg_handle *h, int32_t *n_comp, int32_t *comp_of_node);
+
+/* Residual and Jacobian of one Newton iteration (device pointers; vectors
+   n_nodes + 1, CSR values of the circuit pattern).  transient != 0:
+   N_TIA_OneStep::obtainResidual / obtainJacobian (N_TIA_OneStep.C:215-330):
+     order 1: rhs = -((Q - qn0)/dt + F) + dQdxdVp/dt + dFdxdVp,
+              J = dQdx/dt + dFdx;
+     order 2: rhs = -((Q - qn0)/dt + F/2 + fprev/2) + dQdxdVp/dt + dFdxdVp/2,
+              J = dQdx/dt + dFdx/2;
+   (limiter terms only when the circuit's limiter option is on).
+   transient == 0 (DC operating point): rhs = -F + dFdxdVp, J = dFdx.
+   The ground entry of rhs is set to 0. */
+int xg_tia_assemble(xg_handle *h, int32_t transient, int32_t order, double dt,
+                    const double *qn0, const double *fprev, const double *F,
+                    const double *Q, const double *dFdxdVp,
+                    const double *dQdxdVp, const double *dFdx,
+                    const double *dQdx, double *rhs, double *J);
+
+/* Solves J dx = rhs per component (dense LU with partial pivoting; replaces
+   the KLU solve of N_LAS_AmesosSolver.C:231-470), applies x += dx, and
+   writes per component stats[4c..4c+3] = {max |rhs|, weighted RMS norm of dx
+   with weights reltol * max(|wa|, |wb|) + abstol (N_NLS_DampedNewton.C:
+   311-335), 1 if singular else 0, size}.  active (n_comp flags, or NULL for
+   all) skips converged components. */
+int xg_dense_solve(xg_handle *h, const double *J, const double *rhs, double *x,
+                   const double *wa, const double *wb, double reltol,
+                   double abstol, const int32_t *active, double *stats);
+
+/* stats[4c] = max |rhs| over component c's rows (the Newton convergence
+   test's residual norm, N_NLS_DampedNewton.C:1787-1810). */
+int xg_rhs_norms(xg_handle *h, const double *rhs, double *stats);
+
 /* Time (seconds) at which the next xg_update_state evaluates PULSE sources,
    and the breakpoint tolerance of the PULSE corners (SolverState::bpTol,
    N_DEV_DeviceMgr.C:3488). */

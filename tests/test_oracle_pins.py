@@ -288,3 +288,27 @@ def test_linear_devices_closed_form():
     assert JF[a, a] == 1 / 250.0 and JF[a, br] == 1.0 and JF[br, a] == 1.0
     assert JQ[b, b] == 2e-12 + 5e-13 and JQ[b, c] == -5e-13
     assert not FL.any() and not QL.any()
+
+
+def test_oracle_transient_rc_closed_form():
+    """The oracle transient (two backward-Euler steps, then trapezoidal) on
+    an RC step equals the discrete recurrence of those methods."""
+    from oracle.transient_ref import RefTransient
+    from xyce_amd.circuits import Pulse
+    nl = Netlist("rc")
+    nl.add_vsrc("V1", "a", "0", 0.0, Pulse(0.0, 1.0, 0.0, 1e-12, 1e-12, 1.0, 2.0))
+    nl.add_res("R1", "a", "b", 1e3)
+    nl.add_cap("C1", "b", "0", 1e-12)
+    h, tau = 10e-12, 1e-9
+    t, w = RefTransient(nl).run(1e-9, h, ["a", "b"])
+    vb = [0.0]
+    for k in range(1, len(t)):
+        va1, va0 = w[0, k], w[0, k - 1]
+        if k <= 2:
+            vb.append((vb[-1] + (h / tau) * va1) / (1 + h / tau))
+        else:
+            vb.append((vb[-1] * (1 - h / (2 * tau)) + (h / (2 * tau)) * (va1 + va0))
+                      / (1 + h / (2 * tau)))
+    assert np.max(np.abs(np.array(vb) - w[1])) < 1e-13
+    # the continuous solution is approached: 1 - e^{-1} at t = tau
+    assert abs(w[1][-1] - (1 - np.exp(-1.0))) < 1e-4

@@ -69,6 +69,13 @@ SIGNATURES = [
                                        ctypes.c_int32, ctypes.c_double,
                                        ctypes.POINTER(ctypes.c_double)]),
     ("xg_set_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    ("xg_rhs_norms", ctypes.c_int, [ctypes.c_void_p] * 3),
+    ("xg_components", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.c_void_p]),
+    ("xg_tia_assemble", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_double] + [ctypes.c_void_p] * 10),
+    ("xg_dense_solve", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 +
+     [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_set_eval_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_set_output_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),

@@ -23,6 +23,7 @@ SOURCES = [
     "xg_bsim3.hip",
     "xg_reduce.hip",
     "xg_linear.hip",
+    "xg_tia.hip",
     "xg_setup.cpp",
     "xg_bsim3_setup.cpp",
 ]

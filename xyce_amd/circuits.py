@@ -209,17 +209,28 @@ MOS1_NMOS = dict(VTO=0.45, KP=120e-6)
 MOS1_PMOS = dict(VTO=-0.45, KP=40e-6)
 
 
-def ring_oscillator_mos1(stages=101, seed=1, vdd=1.8):
+def supply(vdd, ramp):
+    """Vdd source: DC, or (ramp > 0) a PULSE from 0 to vdd in `ramp` seconds
+    that stays on (the start kick of a transient: the DC operating point is
+    then all-zero and the unequal stage loads break the ring's symmetry)."""
+    if not ramp:
+        return dict(dc=vdd)
+    return dict(dc=0.0, pulse=Pulse(0.0, vdd, 0.0, ramp, ramp, 1.0, 2.0))
+
+
+def ring_oscillator_mos1(stages=101, seed=1, vdd=1.8, ramp=0.0):
     """101-stage CMOS ring oscillator of MOSFET level-1 devices.
 
     NMOS W=1u L=0.18u VTO=0.45 KP=120u; PMOS W=2u L=0.18u VTO=-0.45 KP=40u;
-    load capacitor per stage output drawn from U[5, 15] fF (seeded); Vdd 1.8 V.
+    load capacitor per stage output drawn from U[5, 15] fF (seeded); Vdd 1.8 V
+    (ramped from 0 over `ramp` seconds when given).
     """
     rng = np.random.default_rng(seed)
     nl = Netlist("ring oscillator, %d stages, MOSFET level 1" % stages)
     nl.add_model("NMOS1", "mos1", 1, **MOS1_NMOS)
     nl.add_model("PMOS1", "mos1", -1, **MOS1_PMOS)
-    nl.add_vsrc("VDD", "vdd", "0", vdd)
+    sv = supply(vdd, ramp)
+    nl.add_vsrc("VDD", "vdd", "0", sv["dc"], sv.get("pulse"))
     for i in range(stages):
         inp = "out%d" % ((i - 1) % stages)
         out = "out%d" % i
@@ -349,7 +360,7 @@ def bsim3_variety(seed=0, n_nodes=30, n_dev=128):
     return nl
 
 
-def ring_oscillator_bsim3(stages=11, seed=3, vdd=1.2, dvth0=0.0, dtox=0.0,
+def ring_oscillator_bsim3(stages=11, seed=3, vdd=1.2, dvth0=0.0, dtox=0.0, ramp=0.0,
                           rng=None, prefix=""):
     """Ring oscillator of BSIM3 inverters.  With dvth0 / dtox nonzero every
     device gets its own model card with VTH0 += N(0, dvth0) and
@@ -360,7 +371,8 @@ def ring_oscillator_bsim3(stages=11, seed=3, vdd=1.2, dvth0=0.0, dtox=0.0,
     if not per_device:
         nl.add_model("NB", "bsim3", 1, **BSIM3_NMOS_130)
         nl.add_model("PB", "bsim3", -1, **BSIM3_PMOS_130)
-    nl.add_vsrc("VDD", "vdd", "0", vdd)
+    sv = supply(vdd, ramp)
+    nl.add_vsrc("VDD", "vdd", "0", sv["dc"], sv.get("pulse"))
     caps = rng.uniform(1e-15, 3e-15, stages)
     for i in range(stages):
         inp = "out%d" % ((i - 1) % stages)

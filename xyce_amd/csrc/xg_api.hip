@@ -35,6 +35,9 @@ struct Handle {
   // linear devices and the time their sources are evaluated at
   LinArgs lin{};
   double time = 0.0, bp_tol = 0.0;
+  // per-component dense solve
+  DenseSolveArgs ds{};
+  int32_t *d_rowptr = nullptr, *d_colidx = nullptr;
   // reduction programs
   ReduceDev red_vec[4]{}, red_mat[2]{};
   double *slots = nullptr;
@@ -253,6 +256,87 @@ int xg_add_vsource(xg_circuit *c, int32_t pos, int32_t neg, int32_t branch,
   return add_lin(c, xg::LIN_V, pos, neg, branch, dc, pulse, 0, "xg_add_vsource");
 }
 
+int xg_components(const xg_handle *H, int32_t *n_comp, int32_t *comp_of_node) {
+  if (!H || !n_comp) return fail(XG_E_ARG, "xg_components: bad argument");
+  const xg::Plan &P = H->h.plan;
+  *n_comp = (int32_t)P.comp_start.size() - 1;
+  if (comp_of_node)
+    for (int32_t k = 0; k < *n_comp; ++k)
+      for (int32_t j = P.comp_start[k]; j < P.comp_start[k + 1]; ++j)
+        comp_of_node[P.comp_nodes[j]] = k;
+  return XG_OK;
+}
+
+int xg_tia_assemble(xg_handle *H, int32_t transient, int32_t order, double dt,
+                    const double *qn0, const double *fprev, const double *F,
+                    const double *Q, const double *dFdxdVp,
+                    const double *dQdxdVp, const double *dFdx,
+                    const double *dQdx, double *rhs, double *J) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !F || !dFdxdVp || !dFdx || !rhs || !J ||
+      (transient && (!qn0 || !Q || !dQdxdVp || !dQdx || !(dt > 0.0) ||
+                     (order == 2 && !fprev))) ||
+      (order != 1 && order != 2))
+    return fail(XG_E_ARG, "xg_tia_assemble: bad argument");
+  xg::Handle &h = H->h;
+  xg::TiaAssembleArgs a{};
+  a.n1 = h.plan.n_nodes + 1;
+  a.nnz = (int64_t)h.plan.colidx.size();
+  a.transient = transient != 0;
+  a.order = order;
+  a.limiter = h.plan.limiter;
+  a.inv_dt = transient ? 1.0 / dt : 0.0;
+  a.qn0 = qn0;
+  a.fprev = fprev ? fprev : F;
+  a.F = F;
+  a.Q = Q ? Q : F;
+  a.FL = dFdxdVp;
+  a.QL = dQdxdVp ? dQdxdVp : dFdxdVp;
+  a.dFdx = dFdx;
+  a.dQdx = dQdx ? dQdx : dFdx;
+  a.rhs = rhs;
+  a.J = J;
+  hipError_t e = xg::launch_tia_assemble(a, h.stream);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "assemble");
+}
+
+int xg_dense_solve(xg_handle *H, const double *J, const double *rhs, double *x,
+                   const double *wa, const double *wb, double reltol,
+                   double abstol, const int32_t *active, double *stats) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !J || !rhs || !x || !wa || !wb || !stats)
+    return fail(XG_E_ARG, "xg_dense_solve: bad argument");
+  xg::Handle &h = H->h;
+  xg::DenseSolveArgs a = h.ds;
+  a.J = J;
+  a.rhs = rhs;
+  a.x = x;
+  a.wa = wa;
+  a.wb = wb;
+  a.reltol = reltol;
+  a.abstol = abstol;
+  a.active = active;
+  a.stats = stats;
+  hipError_t e = xg::launch_dense_solve(a, h.stream);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "dense solve");
+}
+
+int xg_rhs_norms(xg_handle *H, const double *rhs, double *stats) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !rhs || !stats) return fail(XG_E_ARG, "xg_rhs_norms: bad argument");
+  xg::Handle &h = H->h;
+  xg::DenseSolveArgs a = h.ds;
+  a.rhs = rhs;
+  a.stats = stats;
+  a.norm_only = 1;
+  a.active = nullptr;
+  hipError_t e = xg::launch_dense_solve(a, h.stream);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "rhs norms");
+}
+
 int xg_set_time(xg_handle *H, double time, double bp_tol) {
   if (!H) return fail(XG_E_ARG, "xg_set_time: bad argument");
   H->h.time = time;
@@ -341,6 +425,28 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
   h.n_limited_slots = P.m1_slots + xg::bsim3_eval_slots(P);
   chk(h.limited_all = h.alloc<int32_t>((size_t)h.n_limited_slots));
   chk(h.count_buf = h.alloc<int64_t>(128));
+  // circuit CSR and components for the Newton solve
+  chk(h.d_rowptr = h.upload(P.rowptr));
+  chk(h.d_colidx = h.upload(P.colidx));
+  {
+    auto &d = h.ds;
+    d.n_comp = (int32_t)P.comp_start.size() - 1;
+    d.any_large = P.comp_work > 0;
+    int32_t *cs, *cn, *cl;
+    int64_t *ao;
+    chk(cs = h.upload(P.comp_start));
+    chk(cn = h.upload(P.comp_nodes));
+    chk(cl = h.upload(P.comp_local));
+    chk(ao = h.upload(P.comp_aoff));
+    d.cstart = cs;
+    d.nodes = cn;
+    d.local = cl;
+    d.aoff = ao;
+    d.rowptr = h.d_rowptr;
+    d.colidx = h.d_colidx;
+    chk(d.work = h.alloc<double>((size_t)(P.comp_work > 0 ? P.comp_work : 1)));
+    chk(d.bwork = h.alloc<double>((size_t)P.n_nodes + 1));
+  }
   // linear devices
   h.lin.n_slots = P.lin_slots;
   chk(h.lin.type = h.upload(P.lin_type));
@@ -488,9 +594,12 @@ int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
                     double *sto_next) {
   if (H && H->h.plan_only)
     return fail(XG_E_STATE, "handle was finalized without a device");
-  if (!H || !x_next || !sta_curr || !sto_curr || !sta_next || !sto_next)
-    return fail(XG_E_ARG, "xg_update_state: null pointer");
+  if (!H || !x_next) return fail(XG_E_ARG, "xg_update_state: null pointer");
   xg::Handle &h = H->h;
+  // state / store pointers may be null when the circuit has none
+  if ((h.plan.n_state > 0 && (!sta_curr || !sta_next)) ||
+      (h.plan.n_store > 0 && (!sto_curr || !sto_next)))
+    return fail(XG_E_ARG, "xg_update_state: null pointer");
   xg::LoadPtrs p{x_next, sta_curr, sto_curr, sta_next, sto_next};
   hipError_t e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
   if (e != hipSuccess) return hip_fail(e, "mos1 eval");

@@ -152,6 +152,38 @@ hipError_t launch_mos1_eval(const Mos1Args &a, const SolverStateDev &ss,
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
                          const double *slots, double *partial, bool overwrite,
                          hipStream_t s);
+// Newton-iteration assembly (N_TIA_OneStep.C:215-330).
+struct TiaAssembleArgs {
+  int64_t n1 = 0, nnz = 0;
+  int32_t transient = 0, order = 1, limiter = 1;
+  double inv_dt = 0.0;
+  const double *qn0, *fprev, *F, *Q, *FL, *QL, *dFdx, *dQdx;
+  double *rhs, *J;
+};
+
+// Dense LU solve per connected component (ground excluded).
+struct DenseSolveArgs {
+  int32_t n_comp = 0;
+  int32_t any_large = 0;          // some component exceeds the LDS size
+  const int32_t *cstart;          // n_comp + 1: component -> first node
+  const int32_t *nodes;           // nodes grouped by component
+  const int32_t *local;           // node -> index within its component (-1 ground)
+  const int64_t *aoff;            // component -> offset of its matrix in work
+  const int32_t *rowptr, *colidx; // circuit CSR (ground last)
+  const double *J, *rhs;
+  double *x;                      // Newton iterate, updated in place
+  const double *wa, *wb;          // weights: reltol * max(|wa|, |wb|) + abstol
+  double reltol, abstol;
+  const int32_t *active;          // n_comp flags or null (all)
+  int32_t norm_only = 0;          // only stats[0] = max |rhs| per component
+  double *work, *bwork;           // dense matrices / right-hand sides
+  double *stats;                  // n_comp x 4: max|rhs|, wRMS(dx), singular, n
+};
+
+hipError_t launch_tia_assemble(const TiaAssembleArgs &a, hipStream_t s);
+hipError_t launch_dense_solve(const DenseSolveArgs &a, hipStream_t s);
+int dense_lds_max();
+
 hipError_t launch_linear_eval(const LinArgs &a, const SolverStateDev &ss,
                               const LoadPtrs &p, double time, double bp_tol,
                               hipStream_t s);

@@ -12,6 +12,7 @@
 #include "xg_setup.h"
 
 #include <algorithm>
+#include <functional>
 #include <cctype>
 #include <cmath>
 #include <cstring>
@@ -712,6 +713,49 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     for (int32_t j = plan.rowptr[r]; j < plan.rowptr[r + 1]; ++j)
       mskip[j] = (r == G || plan.colidx[j] == G) ? 1 : 0;
   for (int k = 0; k < 2; ++k) finish_program(mat[k], nnz, mskip, plan.red_mat[k]);
+  plan.limiter = c.opt.limiter;
+  // ---- connected components without ground (union-find over the pattern)
+  {
+    std::vector<int32_t> par(n);
+    std::iota(par.begin(), par.end(), 0);
+    std::function<int32_t(int32_t)> find = [&](int32_t a) {
+      while (par[a] != a) a = par[a] = par[par[a]];
+      return a;
+    };
+    for (int32_t r = 0; r < n; ++r)
+      for (int32_t j = plan.rowptr[r]; j < plan.rowptr[r + 1]; ++j) {
+        const int32_t col = plan.colidx[j];
+        if (col >= n) continue;
+        const int32_t a = find(r), b = find(col);
+        if (a != b) par[std::max(a, b)] = std::min(a, b);
+      }
+    std::vector<int32_t> id(n, -1);
+    int32_t nc = 0;
+    for (int32_t v = 0; v < n; ++v) {
+      const int32_t root = find(v);
+      if (id[root] < 0) id[root] = nc++;
+    }
+    std::vector<int32_t> cnt(nc + 1, 0);
+    for (int32_t v = 0; v < n; ++v) cnt[id[find(v)] + 1]++;
+    for (int32_t k = 0; k < nc; ++k) cnt[k + 1] += cnt[k];
+    plan.comp_start = cnt;
+    plan.comp_nodes.assign(n, 0);
+    plan.comp_local.assign(n + 1, -1);
+    std::vector<int32_t> at(cnt.begin(), cnt.end() - 1);
+    for (int32_t v = 0; v < n; ++v) {
+      const int32_t k = id[find(v)];
+      plan.comp_local[v] = at[k] - cnt[k];
+      plan.comp_nodes[at[k]++] = v;
+    }
+    plan.comp_aoff.assign(nc, 0);
+    int64_t w = 0;
+    for (int32_t k = 0; k < nc; ++k) {
+      const int64_t m = cnt[k + 1] - cnt[k];
+      plan.comp_aoff[k] = w;
+      if (m > dense_lds_max()) w += m * m;
+    }
+    plan.comp_work = w;
+  }
   int32_t pbase = 0;
   for (auto *prog : {&plan.red_vec[0], &plan.red_vec[1], &plan.red_vec[2],
                      &plan.red_vec[3], &plan.red_mat[0], &plan.red_mat[1]}) {

@@ -111,6 +111,12 @@ struct Plan {
   ReduceProgram red_vec[4];  // F, Q, dFdxdVp, dQdxdVp
   ReduceProgram red_mat[2];  // dFdx, dQdx
   int32_t n_partials = 0;    // chunk partial sums over all six programs
+  int32_t limiter = 1;
+  // connected components of the circuit graph without ground (independent
+  // sub-circuits), for the per-component dense solve
+  std::vector<int32_t> comp_start, comp_nodes, comp_local;
+  std::vector<int64_t> comp_aoff;
+  int64_t comp_work = 0;     // doubles of dense workspace (large components)
   ~Plan();
 };
 

@@ -1,0 +1,205 @@
+"""Transient analysis on the GPU: DC operating point, then fixed-step
+ONESTEP integration (backward Euler for the first two steps, trapezoidal
+after, as N_TIA_OneStep::completeStep raises the order,
+N_TIA_OneStep.C:1945-2010), with a full Newton solve per step.
+
+Every Newton iteration runs on the device through the C ABI:
+  xg_update_state + ordered reduction (xg_load_all, overwrite mode),
+  xg_tia_assemble (N_TIA_OneStep::obtainResidual / obtainJacobian),
+  xg_dense_solve (LU per independent sub-circuit + update + norms),
+  xg_rhs_norms (residual norm of the convergence test).
+The host only keeps the step sequence and reads one small stats array per
+iteration.
+
+Newton control follows N_NLS_DampedNewton / N_NLS_NLParams:
+  weights   reltol * max(|x_old|, |x_new|) + abstol (DampedNewton.C:311-335),
+            fixed per step in a transient, refreshed every DC iteration;
+  converged every device converged (no junction limiting in the last load)
+            and  max|rhs| <= RHSTol  and  wRMS(dx) <= deltaXTol,
+            or wRMS(dx) <= 1e-6 (DampedNewton.C:1787-1830);
+  DC:       reltol 1e-3, abstol 1e-12, RHSTol 1e-6, deltaXTol 1, 200 steps;
+  TRAN:     reltol 1e-2, abstol 1e-6, RHSTol 1e-2, deltaXTol 0.33, 20 steps
+            (N_NLS_NLParams.C:108-125, NLParams.h defaults).
+The predictor is ONESTEP's (N_TIA_OneStep.C:115-133 with the constant-step
+coefficients of updateCoeffs, N_TIA_OneStep.C:1355-1410).  Time steps are
+fixed (the reference's LTE step control is not reproduced); a step whose
+Newton solve fails is retried with half the step.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .loader import DeviceLoad, _ptr
+
+DC_PARAMS = dict(reltol=1e-3, abstol=1e-12, rhstol=1e-6, dxtol=1.0, maxit=200)
+TRAN_PARAMS = dict(reltol=1e-2, abstol=1e-6, rhstol=1e-2, dxtol=0.33, maxit=20)
+SMALL_UPDATE = 1e-6
+
+
+class NewtonFailure(RuntimeError):
+    pass
+
+
+class Transient:
+    def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda"):
+        import torch
+        self.torch = torch
+        self.nl = nl.layout()
+        self.dev = DeviceLoad(nl, gmin=gmin, limiter=limiter)
+        self.dev.set_output_mode(True)
+        n1 = nl.n_nodes + 1
+        nc = ctypes.c_int32()
+        check(lib().xg_components(self.dev.h, ctypes.byref(nc), None), "components")
+        self.n_comp = nc.value
+        z = lambda n: torch.zeros(n, dtype=torch.float64, device=device)
+        self.x = z(n1)            # Newton iterate (next solution)
+        self.x_curr = z(n1)       # accepted solution at t_n
+        self.xh1 = z(n1)          # x_n - x_{n-1}
+        self.xh2 = z(n1)          # x_{n-1} - x_{n-2}
+        self.xpred = z(n1)
+        self.sta_c, self.sta_n = z(nl.n_state), z(nl.n_state)
+        self.sto_c, self.sto_n = z(nl.n_store), z(nl.n_store)
+        self.vec = z(4 * n1)
+        self.F, self.Q, self.FL, self.QL = (self.vec[k * n1:(k + 1) * n1] for k in range(4))
+        nnz = self.dev.nnz
+        self.JF, self.JQ, self.J = z(nnz), z(nnz), z(nnz)
+        self.rhs = z(n1)
+        self.qn0, self.fprev = z(n1), z(n1)
+        self.stats = z(4 * self.n_comp)
+        self.nstats = z(4 * self.n_comp)
+        self.bp_tol = bp_tol
+        self.t = 0.0
+        self.n_steps = 0
+        self.newton_total = 0
+        self.limiter = limiter
+
+    # ---- one device load + assembly
+    def _load(self, it, transient, first_tran=False, dcop=False, init_jct=False, t=0.0):
+        dev = self.dev
+        dev.set_solver_state(dcop=int(dcop), tranop=int(dcop), initJct=int(init_jct),
+                             initTran=int(first_tran), newtonIter=it,
+                             transient=int(transient))
+        dev.set_time(t, self.bp_tol if self.bp_tol is not None else 0.0)
+        dev.load_all(self.x, self.sta_c, self.sto_c, self.sta_n, self.sto_n,
+                     self.F, self.Q, self.FL, self.QL, self.JF, self.JQ)
+
+    def _assemble(self, transient, order=1, dt=0.0):
+        check(lib().xg_tia_assemble(
+            self.dev.h, int(transient), order, dt,
+            *[_ptr(t) for t in (self.qn0, self.fprev, self.F, self.Q, self.FL, self.QL,
+                               self.JF, self.JQ, self.rhs, self.J)]), "assemble")
+
+    def _solve(self, prm, wa, wb):
+        check(lib().xg_dense_solve(self.dev.h, _ptr(self.J), _ptr(self.rhs), _ptr(self.x),
+                                   _ptr(wa), _ptr(wb), prm["reltol"], prm["abstol"],
+                                   None, _ptr(self.stats)), "dense solve")
+
+    def _newton(self, prm, transient, order, dt, t, first_tran=False, dcop=False,
+                wa=None, wb=None):
+        """Full Newton from self.x; returns the iteration count."""
+        torch = self.torch
+        self._load(0, transient, first_tran=first_tran, dcop=dcop, init_jct=dcop, t=t)
+        self._assemble(transient, order, dt)
+        it = 0
+        while True:
+            if dcop:
+                # DC weights follow the iterate (DampedNewton.C:520-528); the
+                # solve reads each weight before it updates that entry
+                wb = self.x
+            self._solve(prm, wa, wb)
+            it += 1
+            self._load(it, transient, first_tran=first_tran, dcop=dcop, t=t)
+            self._assemble(transient, order, dt)
+            check(lib().xg_rhs_norms(self.dev.h, _ptr(self.rhs), _ptr(self.nstats)), "norms")
+            st = self.stats.view(-1, 4).cpu().numpy()
+            nrm = self.nstats.view(-1, 4).cpu().numpy()
+            if st[:, 2].any():
+                raise NewtonFailure("singular Jacobian at t=%g" % t)
+            max_rhs = float(nrm[:, 0].max())
+            upd = float(st[:, 1].max())
+            n_lim, n_bad = self.dev.stats()
+            if n_bad:
+                raise NewtonFailure("non-finite device values at t=%g" % t)
+            devices_ok = n_lim == 0 or it >= prm["maxit"]
+            if devices_ok and max_rhs <= prm["rhstol"] and upd <= prm["dxtol"]:
+                break
+            if devices_ok and upd <= SMALL_UPDATE:
+                break
+            if it >= prm["maxit"]:
+                raise NewtonFailure("Newton did not converge at t=%g (|rhs| %g, dx %g)"
+                                    % (t, max_rhs, upd))
+        self.newton_total += it
+        return it
+
+    # ---- analyses
+    def dcop(self):
+        """DC operating point at t = 0 (the tranop before a transient)."""
+        self.x.zero_()
+        self.sta_c.zero_()
+        self.sto_c.zero_()
+        self._newton(DC_PARAMS, transient=False, order=1, dt=0.0, t=0.0, dcop=True,
+                     wa=self.x_curr)
+        # the converged point starts the history (N_TIA_OneStep::initialize)
+        self.x_curr.copy_(self.x)
+        self.sta_c.copy_(self.sta_n)
+        self.sto_c.copy_(self.sto_n)
+        self.qn0.copy_(self.Q)
+        self.fprev.copy_(self.F)
+        self.xh1.zero_()
+        self.xh2.zero_()
+        self.t = 0.0
+        self.n_steps = 0
+
+    def step(self, dt):
+        """One ONESTEP time step of size dt (retried with dt/2 on failure)."""
+        order = 2 if self.n_steps >= 2 else 1
+        t_next = self.t + dt
+        # predictor xn0 = x_n + beta1 (x_n - x_{n-1}) [+ beta2 (x_{n-1} - x_{n-2})]
+        self.xpred.copy_(self.x_curr)
+        if order == 2:
+            self.xpred.add_(self.xh1, alpha=1.5).add_(self.xh2, alpha=-0.5)
+        else:
+            self.xpred.add_(self.xh1, alpha=1.0)
+        self.x.copy_(self.xpred)
+        try:
+            self._newton(TRAN_PARAMS, transient=True, order=order, dt=dt, t=t_next,
+                         first_tran=self.n_steps == 0, wa=self.x_curr, wb=self.xpred)
+        except NewtonFailure:
+            if dt < 1e-18:
+                raise
+            self.x.copy_(self.x_curr)
+            self.step(dt / 2)
+            self.step(dt / 2)
+            return
+        # completeStep: history, states
+        self.xh2.copy_(self.xh1)
+        self.xh1.copy_(self.x).sub_(self.x_curr)
+        self.x_curr.copy_(self.x)
+        self.qn0.copy_(self.Q)
+        self.fprev.copy_(self.F)
+        self.sta_c.copy_(self.sta_n)
+        self.sto_c.copy_(self.sto_n)
+        self.t = t_next
+        self.n_steps += 1
+
+    def run(self, tstop, dt, probes, every=1):
+        """DC operating point, then steps of dt to tstop.  Returns (times,
+        waveforms[probe, time]) for the given node names."""
+        torch = self.torch
+        idx = torch.tensor([self.nl.node(p) for p in probes], device=self.x.device)
+        self.dcop()
+        n = int(round(tstop / dt))
+        out = torch.zeros(len(probes), n // every + 1, dtype=torch.float64,
+                          device=self.x.device)
+        times = [0.0]
+        out[:, 0] = self.x_curr[idx]
+        k = 1
+        for s in range(1, n + 1):
+            self.step(dt)
+            if s % every == 0:
+                out[:, k] = self.x_curr[idx]
+                times.append(self.t)
+                k += 1
+        return np.array(times), out[:, :k].cpu().numpy()
