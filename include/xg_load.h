@@ -272,6 +272,18 @@ int xg_dense_solve(xg_handle *h, const double *J, const double *rhs, double *x,
                    const double *wa, const double *wb, double reltol,
                    double abstol, const int32_t *active, double *stats);
 
+/* ONESTEP predictor (N_TIA_OneStep.C:115-133): xpred = xh0 + beta1 xh1
+   (+ beta2 xh2 at order 2), with xh1 = x_n - x_{n-1}, xh2 = x_{n-1} -
+   x_{n-2}; beta from updateCoeffs (N_TIA_OneStep.C:1380-1405). */
+int xg_tia_predict(xg_handle *h, int32_t order, double beta1, double beta2,
+                   const double *xh0, const double *xh1, const double *xh2,
+                   double *xpred);
+
+/* counts[c] (device array, n_comp) = instances of component c whose
+   junction voltages the last xg_update_state limited (per-component
+   "all devices converged", N_NLS_DampedNewton.C:1797-1805). */
+int xg_component_limited(xg_handle *h, int32_t *counts);
+
 /* stats[4c] = max |rhs| over component c's rows (the Newton convergence
    test's residual norm, N_NLS_DampedNewton.C:1787-1810). */
 int xg_rhs_norms(xg_handle *h, const double *rhs, double *stats);

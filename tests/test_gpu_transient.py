@@ -119,3 +119,30 @@ def test_dense_solve_against_numpy():
         assert np.allclose(x[idx], ref, rtol=1e-12, atol=1e-14), c
     st = stats.cpu().numpy().reshape(-1, 4)
     assert not st[:, 2].any()
+
+
+def test_batched_monte_carlo_equals_each_alone():
+    """Monte-Carlo BSIM3 rings of uneven sizes (per-device VTH0/TOX) run as
+    one batch -- one device-load launch and one dense-solve launch per
+    Newton iteration for all samples -- give each sample the waveform it
+    gets when simulated alone, bit for bit: each sample's entries sum only
+    its own devices' statements in the same order, and each converges on
+    its own test."""
+    from xyce_amd import circuits
+    from xyce_amd.transient import Transient
+    parts = circuits.monte_carlo_rings(4, seed=9, stages=(11, 41))
+    batch = Transient(circuits.merge(parts))
+    probes = ["s%d_out0" % k for k in range(len(parts))] + \
+             ["s%d_out3" % k for k in range(len(parts))]
+    tb, wb = batch.run(0.3e-9, 1e-12, probes, every=3)
+    print("batch: %d Newton iterations, %d step failures"
+          % (batch.newton_total, batch.step_failures), flush=True)
+    for k, p in enumerate(parts):
+        alone = Transient(circuits.monte_carlo_rings(4, seed=9, stages=(11, 41))[k])
+        ta, wa = alone.run(0.3e-9, 1e-12, ["out0", "out3"], every=3)
+        print("sample %d alone: %d Newton iterations, %d step failures, max diff %.3g"
+              % (k, alone.newton_total, alone.step_failures,
+                 np.abs(wa[0] - wb[k]).max()), flush=True)
+        assert np.array_equal(tb, ta)
+        assert wa[0].tobytes() == wb[k].tobytes(), k
+        assert wa[1].tobytes() == wb[len(parts) + k].tobytes(), k

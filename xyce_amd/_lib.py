@@ -70,6 +70,9 @@ SIGNATURES = [
                                        ctypes.POINTER(ctypes.c_double)]),
     ("xg_set_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     ("xg_rhs_norms", ctypes.c_int, [ctypes.c_void_p] * 3),
+    ("xg_tia_predict", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                       ctypes.c_double] + [ctypes.c_void_p] * 4),
+    ("xg_component_limited", ctypes.c_int, [ctypes.c_void_p] * 2),
     ("xg_components", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                       ctypes.c_void_p]),
     ("xg_tia_assemble", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,

@@ -459,3 +459,43 @@ def random_node_voltages(nl, seed=0, lo=-0.2, hi=2.0):
     x = rng.uniform(lo, hi, nl.n_nodes + 1)
     x[-1] = 0.0
     return x
+
+
+def merge(parts, prefixes=None):
+    """One netlist holding several independent circuits (they share only
+    ground): node, model and device names get a per-part prefix.  The
+    device load and the Newton solve then run every part in the same
+    launches, one dense solve per part."""
+    prefixes = prefixes or ["s%d_" % k for k in range(len(parts))]
+    nl = Netlist("batch of %d circuits" % len(parts))
+    for src, pre in zip(parts, prefixes):
+        ren = lambda n, pre=pre: n if n in GROUND_NAMES else pre + n
+        # keep each part's node order, so every sub-circuit has the same
+        # unknown ordering (and so the same pivots) as when it runs alone
+        for name in sorted(src.node_ids, key=src.node_ids.get):
+            nl.node(ren(name))
+        for name, m in src.models.items():
+            nl.add_model(pre + name, m.kind, m.dtype, **m.params)
+        for m in src.mos:
+            nl.add_mos(pre + m.name, ren(m.d), ren(m.g), ren(m.s), ren(m.b), pre + m.model,
+                       **m.params)
+        for name, a, b, r in src.res:
+            nl.add_res(pre + name, ren(a), ren(b), r)
+        for name, a, b, c in src.caps:
+            nl.add_cap(pre + name, ren(a), ren(b), c)
+        for v in src.vsrc:
+            nl.add_vsrc(pre + v.name, ren(v.p), ren(v.n), v.dc, v.pulse)
+    return nl
+
+
+def monte_carlo_rings(n_samples=256, seed=3, stages=(11, 301), dvth0=0.030, dtox=0.02,
+                      ramp=50e-12):
+    """Monte-Carlo samples of the BSIM3 ring oscillator: stage counts drawn
+    uniformly from `stages` (inclusive), every device with its own card
+    (VTH0 += N(0, dvth0), TOX *= U[1 - dtox, 1 + dtox]), seeded.  Returns
+    the list of sample netlists."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(stages[0], stages[1] + 1, n_samples)
+    return [ring_oscillator_bsim3(int(s), dvth0=dvth0, dtox=dtox, ramp=ramp, rng=rng,
+                                  prefix="m")
+            for s in counts]

@@ -38,6 +38,7 @@ struct Handle {
   // per-component dense solve
   DenseSolveArgs ds{};
   int32_t *d_rowptr = nullptr, *d_colidx = nullptr;
+  int32_t *slot_comp = nullptr;
   // reduction programs
   ReduceDev red_vec[4]{}, red_mat[2]{};
   double *slots = nullptr;
@@ -323,6 +324,29 @@ int xg_dense_solve(xg_handle *H, const double *J, const double *rhs, double *x,
   return e == hipSuccess ? XG_OK : hip_fail(e, "dense solve");
 }
 
+int xg_tia_predict(xg_handle *H, int32_t order, double beta1, double beta2,
+                   const double *xh0, const double *xh1, const double *xh2,
+                   double *xpred) {
+  if (!H || !xh0 || !xh1 || !xpred || (order == 2 && !xh2) ||
+      (order != 1 && order != 2))
+    return fail(XG_E_ARG, "xg_tia_predict: bad argument");
+  xg::Handle &h = H->h;
+  hipError_t e = xg::launch_predict(h.plan.n_nodes + 1, order, beta1, beta2, xh0,
+                                    xh1, xh2 ? xh2 : xh1, xpred, h.stream);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "predict");
+}
+
+int xg_component_limited(xg_handle *H, int32_t *counts) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !counts) return fail(XG_E_ARG, "xg_component_limited: bad argument");
+  xg::Handle &h = H->h;
+  hipError_t e = xg::launch_component_limited(h.limited_all, h.slot_comp,
+                                              h.n_limited_slots, counts,
+                                              h.ds.n_comp, h.stream);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "component limited");
+}
+
 int xg_rhs_norms(xg_handle *H, const double *rhs, double *stats) {
   if (H && H->h.plan_only)
     return fail(XG_E_STATE, "handle was finalized without a device");
@@ -427,6 +451,7 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
   chk(h.count_buf = h.alloc<int64_t>(128));
   // circuit CSR and components for the Newton solve
   chk(h.d_rowptr = h.upload(P.rowptr));
+  chk(h.slot_comp = h.upload(P.slot_comp));
   chk(h.d_colidx = h.upload(P.colidx));
   {
     auto &d = h.ds;

@@ -1471,33 +1471,46 @@ __global__ void __launch_bounds__(kBlock)
 // dependent and junction fields, kRecN x 64 doubles, 50.7 KB) are copied to
 // LDS with every load in flight at once, so the evaluation never waits on
 // HBM for a parameter.
-// kSplit: two blocks per eval wave, the even one the kDC part and the odd
-// one the kCV part, so that twice as many SIMDs each run a shorter wave.
+// kSplit: one workgroup of two waves per eval wave, wave 0 the kDC part and
+// wave 1 the kCV part of the same 64 instances, sharing one LDS copy of the
+// records (each wave stages half of the fields).  Both waves read their
+// inputs -- including the store the previous Newton iteration wrote, which
+// the kDC wave overwrites -- before the workgroup barrier, and write only
+// after it, so the in-place store update is race-free.
 constexpr int kRecN = B3R_N + B3I_N;
 
 template <bool kSplit>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kSplit ? 128 : 64)
     bsim3_eval_staged_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p) {
   __shared__ double rec[kRecN * 64];
   __shared__ double card[B3C_N];
-  const int lane = threadIdx.x;
-  const int w = kSplit ? (blockIdx.x >> 1) : blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = kSplit ? (int)(threadIdx.x >> 6) : 0;
+  constexpr int kWaves = kSplit ? 2 : 1;
+  constexpr int kPer = (kRecN + kWaves - 1) / kWaves;
+  const int w = blockIdx.x;
   const int slot = w * 64 + lane;
   const double *src = a.rec + (size_t)w * kRecN * 64 + lane;
-  double v[kRecN];
+  double v[kPer];
 #pragma unroll
-  for (int f = 0; f < kRecN; ++f) v[f] = src[f * 64];
+  for (int j = 0; j < kPer; ++j) {
+    const int f = j * kWaves + wave;
+    v[j] = f < kRecN ? src[f * 64] : 0.0;
+  }
   B3Inputs in;
   b3_inputs(a, ss, p, slot, in);
   const bool live = a.inst_of[slot] >= 0;
 #pragma unroll
-  for (int f = 0; f < kRecN; ++f) rec[f * 64 + lane] = v[f];
-  if (lane < B3C_N) card[lane] = a.cards[a.wave_card[w] * B3C_N + lane];
+  for (int j = 0; j < kPer; ++j) {
+    const int f = j * kWaves + wave;
+    if (f < kRecN) rec[f * 64 + lane] = v[j];
+  }
+  if (threadIdx.x < B3C_N) card[threadIdx.x] = a.cards[a.wave_card[w] * B3C_N + threadIdx.x];
   __syncthreads();
   if (!live) return;
   const double *R = rec + lane, *Gp = rec + B3R_N * 64 + lane;
   if constexpr (kSplit) {
-    if (blockIdx.x & 1)
+    if (wave)
       b3_eval<true, kCV>(a, ss, p, slot, card, R, Gp, in);
     else
       b3_eval<true, kDC>(a, ss, p, slot, card, R, Gp, in);
@@ -1508,8 +1521,8 @@ __global__ void __launch_bounds__(64)
 
 }  // namespace
 
-// Launched waves up to which the LDS-staged kernel is used: its 50.7 KB of
-// LDS per wave allows three waves per CU on the 256 CUs.
+// Eval waves up to which the LDS-staged kernels are used: their 50.7 KB of
+// LDS per workgroup allows three workgroups per CU on the 256 CUs.
 #ifndef XG_B3_STAGED_MAX_WAVES
 #define XG_B3_STAGED_MAX_WAVES (3 * 256)
 #endif
@@ -1520,12 +1533,10 @@ hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
   const int waves = d.n_slots / 64;
   int v = d.variant;
   if (v == XG_EVAL_AUTO)
-    v = 2 * waves <= XG_B3_STAGED_MAX_WAVES ? XG_EVAL_STAGED_SPLIT
-        : waves <= XG_B3_STAGED_MAX_WAVES   ? XG_EVAL_STAGED
-                                            : XG_EVAL_HBM;
+    v = waves <= XG_B3_STAGED_MAX_WAVES ? XG_EVAL_STAGED_SPLIT : XG_EVAL_HBM;
   if (v == XG_EVAL_STAGED_SPLIT) {
-    hipLaunchKernelGGL(bsim3_eval_staged_kernel<true>, dim3(2 * waves),
-                       dim3(64), 0, s, d, ss, p);
+    hipLaunchKernelGGL(bsim3_eval_staged_kernel<true>, dim3(waves), dim3(128), 0,
+                       s, d, ss, p);
   } else if (v == XG_EVAL_STAGED) {
     hipLaunchKernelGGL(bsim3_eval_staged_kernel<false>, dim3(waves), dim3(64),
                        0, s, d, ss, p);

@@ -755,6 +755,19 @@ int bsim3_build(Circuit &c, Plan &plan, int64_t slot_base,
   return XG_OK;
 }
 
+void bsim3_slot_nodes(const Plan &plan, int32_t ground, std::vector<int32_t> &out) {
+  if (!plan.b3) return;
+  const Bsim3Plan &b = *plan.b3;
+  const int32_t S = b.slots;
+  for (int32_t s = 0; s < S; ++s) {
+    int32_t node = -1;
+    if (b.inst_of[s] >= 0)
+      for (int k = 0; k < 4 && node < 0; ++k)
+        if (b.lid4[(size_t)k * S + s] != ground) node = b.lid4[(size_t)k * S + s];
+    out.push_back(node);
+  }
+}
+
 int64_t bsim3_slot_count(const Plan &plan) {
   return plan.b3 ? (int64_t)B3S_N * plan.b3->slots : 0;
 }

@@ -183,10 +183,16 @@ struct DenseSolveArgs {
 hipError_t launch_tia_assemble(const TiaAssembleArgs &a, hipStream_t s);
 hipError_t launch_dense_solve(const DenseSolveArgs &a, hipStream_t s);
 int dense_lds_max();
+hipError_t launch_predict(int64_t n1, int32_t order, double beta1, double beta2,
+                          const double *xh0, const double *xh1, const double *xh2,
+                          double *out, hipStream_t s);
 
 hipError_t launch_linear_eval(const LinArgs &a, const SolverStateDev &ss,
                               const LoadPtrs &p, double time, double bp_tol,
                               hipStream_t s);
+hipError_t launch_component_limited(const int32_t *flags, const int32_t *slot_comp,
+                                    int32_t n, int32_t *cnt, int32_t n_comp,
+                                    hipStream_t s);
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s);
 

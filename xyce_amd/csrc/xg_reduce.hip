@@ -183,7 +183,34 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+__global__ void zero_i32_kernel(int32_t *p, int32_t n) {
+  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Limited instances per component (a count, so the order of the atomic
+// increments does not matter).
+__global__ void comp_count_kernel(const int32_t *flags, const int32_t *slot_comp,
+                                  int32_t n, int32_t *cnt) {
+  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c = slot_comp[i];
+  if (c >= 0 && (flags[i] & 1)) atomicAdd(cnt + c, 1);
+}
+
 }  // namespace
+
+hipError_t launch_component_limited(const int32_t *flags, const int32_t *slot_comp,
+                                    int32_t n, int32_t *cnt, int32_t n_comp,
+                                    hipStream_t s) {
+  if (n_comp > 0)
+    hipLaunchKernelGGL(zero_i32_kernel, dim3((n_comp + 255) / 256), dim3(256), 0,
+                       s, cnt, n_comp);
+  if (n > 0)
+    hipLaunchKernelGGL(comp_count_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                       flags, slot_comp, n, cnt);
+  return hipGetLastError();
+}
 
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
                          const double *slots, double *partial, bool overwrite,

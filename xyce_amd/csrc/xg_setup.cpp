@@ -755,6 +755,20 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
       if (m > dense_lds_max()) w += m * m;
     }
     plan.comp_work = w;
+    // eval slot -> component (an instance's nodes all lie in one component)
+    std::vector<int32_t> node_of;
+    for (int32_t s = 0; s < plan.m1_slots; ++s) {
+      int32_t node = -1;
+      if (plan.m1_inst_of[s] >= 0)
+        for (int k = 0; k < 4 && node < 0; ++k)
+          if (plan.m1_lid4[(size_t)k * plan.m1_slots + s] != G)
+            node = plan.m1_lid4[(size_t)k * plan.m1_slots + s];
+      node_of.push_back(node);
+    }
+    bsim3_slot_nodes(plan, G, node_of);
+    plan.slot_comp.assign(node_of.size(), -1);
+    for (size_t s = 0; s < node_of.size(); ++s)
+      if (node_of[s] >= 0) plan.slot_comp[s] = id[find(node_of[s])];
   }
   int32_t pbase = 0;
   for (auto *prog : {&plan.red_vec[0], &plan.red_vec[1], &plan.red_vec[2],

@@ -116,6 +116,7 @@ struct Plan {
   // sub-circuits), for the per-component dense solve
   std::vector<int32_t> comp_start, comp_nodes, comp_local;
   std::vector<int64_t> comp_aoff;
+  std::vector<int32_t> slot_comp;  // MOSFET1 then BSIM3 eval slots -> component
   int64_t comp_work = 0;     // doubles of dense workspace (large components)
   ~Plan();
 };
@@ -151,6 +152,8 @@ int bsim3_build(Circuit &c, Plan &plan, int64_t slot_base,
                 std::vector<Contrib> *vec, std::vector<Contrib> *mat,
                 std::string &err);
 int64_t bsim3_slot_count(const Plan &plan);
+// For every BSIM3 eval slot, one of its non-ground nodes (-1: none/padding).
+void bsim3_slot_nodes(const Plan &plan, int32_t ground, std::vector<int32_t> &out);
 void bsim3_plan_free(Plan &plan);
 
 }  // namespace xg

@@ -56,6 +56,22 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ONESTEP predictor (N_TIA_OneStep.C:115-133): xn0 = xHistory[0], then
+// xn0 = beta_i xHistory[i] + 1.0 xn0 for i = 1..order, each product and sum
+// rounded separately (the linearCombo sequence), so the result does not
+// depend on how many circuits share the vector.
+__global__ void __launch_bounds__(kBlock)
+    predict_kernel(int64_t n1, int32_t order, double beta1, double beta2,
+                   const double *xh0, const double *xh1, const double *xh2,
+                   double *out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n1) return;
+  double v = xh0[i];
+  v = beta1 * xh1[i] + 1.0 * v;
+  if (order == 2) v = beta2 * xh2[i] + 1.0 * v;
+  out[i] = v;
+}
+
 // Block-wide max of (|v|, index) with the smallest index on ties.
 __device__ __forceinline__ void argmax_abs(double v, int idx, double *sv, int *si,
                                            double &best, int &bidx) {
@@ -258,5 +274,14 @@ hipError_t launch_dense_solve(const DenseSolveArgs &a, hipStream_t s) {
 }
 
 int dense_lds_max() { return kLdsN; }
+
+hipError_t launch_predict(int64_t n1, int32_t order, double beta1, double beta2,
+                          const double *xh0, const double *xh1, const double *xh2,
+                          double *out, hipStream_t s) {
+  if (n1 == 0) return hipSuccess;
+  hipLaunchKernelGGL(predict_kernel, dim3((unsigned)((n1 + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, n1, order, beta1, beta2, xh0, xh1, xh2, out);
+  return hipGetLastError();
+}
 
 }  // namespace xg

@@ -19,10 +19,14 @@ from . import _lib
 from ._lib import check, keys_vals, lib
 
 
-def _ptr(t):
-    if not t.is_cuda or str(t.dtype) != "torch.float64" or not t.is_contiguous():
-        raise ValueError("expected a contiguous float64 CUDA tensor")
+def _ptr(t, dtype="torch.float64"):
+    if not t.is_cuda or str(t.dtype) != dtype or not t.is_contiguous():
+        raise ValueError("expected a contiguous %s CUDA tensor" % dtype[6:])
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _iptr(t):
+    return _ptr(t, "torch.int32")
 
 
 class DeviceLoad:

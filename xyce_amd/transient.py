@@ -31,7 +31,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .loader import DeviceLoad, _ptr
+from .loader import DeviceLoad, _iptr, _ptr
 
 DC_PARAMS = dict(reltol=1e-3, abstol=1e-12, rhstol=1e-6, dxtol=1.0, maxit=200)
 TRAN_PARAMS = dict(reltol=1e-2, abstol=1e-6, rhstol=1e-2, dxtol=0.33, maxit=20)
@@ -69,10 +69,13 @@ class Transient:
         self.qn0, self.fprev = z(n1), z(n1)
         self.stats = z(4 * self.n_comp)
         self.nstats = z(4 * self.n_comp)
+        self.lim = torch.zeros(self.n_comp, dtype=torch.int32, device=device)
+        self.active = torch.ones(self.n_comp, dtype=torch.int32, device=device)
         self.bp_tol = bp_tol
         self.t = 0.0
         self.n_steps = 0
         self.newton_total = 0
+        self.step_failures = 0
         self.limiter = limiter
 
     # ---- one device load + assembly
@@ -94,12 +97,17 @@ class Transient:
     def _solve(self, prm, wa, wb):
         check(lib().xg_dense_solve(self.dev.h, _ptr(self.J), _ptr(self.rhs), _ptr(self.x),
                                    _ptr(wa), _ptr(wb), prm["reltol"], prm["abstol"],
-                                   None, _ptr(self.stats)), "dense solve")
+                                   _iptr(self.active), _ptr(self.stats)), "dense solve")
 
     def _newton(self, prm, transient, order, dt, t, first_tran=False, dcop=False,
                 wa=None, wb=None):
-        """Full Newton from self.x; returns the iteration count."""
-        torch = self.torch
+        """Full Newton from self.x, one convergence test per independent
+        sub-circuit: a converged component is left out of later solves
+        (its iterate, and so its load, no longer changes), so each circuit
+        of a batch follows exactly the iterations it would take alone.
+        Returns the iteration count."""
+        self.active.fill_(1)
+        conv = np.zeros(self.n_comp, bool)
         self._load(0, transient, first_tran=first_tran, dcop=dcop, init_jct=dcop, t=t)
         self._assemble(transient, order, dt)
         it = 0
@@ -112,24 +120,30 @@ class Transient:
             it += 1
             self._load(it, transient, first_tran=first_tran, dcop=dcop, t=t)
             self._assemble(transient, order, dt)
-            check(lib().xg_rhs_norms(self.dev.h, _ptr(self.rhs), _ptr(self.nstats)), "norms")
+            L = lib()
+            check(L.xg_rhs_norms(self.dev.h, _ptr(self.rhs), _ptr(self.nstats)), "norms")
+            check(L.xg_component_limited(self.dev.h, _iptr(self.lim)), "limited")
             st = self.stats.view(-1, 4).cpu().numpy()
             nrm = self.nstats.view(-1, 4).cpu().numpy()
+            lim = self.lim.cpu().numpy()
             if st[:, 2].any():
                 raise NewtonFailure("singular Jacobian at t=%g" % t)
-            max_rhs = float(nrm[:, 0].max())
-            upd = float(st[:, 1].max())
-            n_lim, n_bad = self.dev.stats()
+            _, n_bad = self.dev.stats()
             if n_bad:
                 raise NewtonFailure("non-finite device values at t=%g" % t)
-            devices_ok = n_lim == 0 or it >= prm["maxit"]
-            if devices_ok and max_rhs <= prm["rhstol"] and upd <= prm["dxtol"]:
-                break
-            if devices_ok and upd <= SMALL_UPDATE:
+            max_rhs, upd = nrm[:, 0], st[:, 1]
+            devices_ok = (lim == 0) | (it >= prm["maxit"])
+            now = devices_ok & (((max_rhs <= prm["rhstol"]) & (upd <= prm["dxtol"])) |
+                                (upd <= SMALL_UPDATE))
+            conv |= now
+            if conv.all():
                 break
             if it >= prm["maxit"]:
-                raise NewtonFailure("Newton did not converge at t=%g (|rhs| %g, dx %g)"
-                                    % (t, max_rhs, upd))
+                bad = np.flatnonzero(~conv)
+                raise NewtonFailure("Newton did not converge at t=%g in %d of %d sub-circuits "
+                                    "(|rhs| %g, dx %g)" % (t, len(bad), self.n_comp,
+                                                           max_rhs[bad].max(), upd[bad].max()))
+            self.active.copy_(self.torch.from_numpy((~conv).astype(np.int32)))
         self.newton_total += it
         return it
 
@@ -156,12 +170,13 @@ class Transient:
         """One ONESTEP time step of size dt (retried with dt/2 on failure)."""
         order = 2 if self.n_steps >= 2 else 1
         t_next = self.t + dt
-        # predictor xn0 = x_n + beta1 (x_n - x_{n-1}) [+ beta2 (x_{n-1} - x_{n-2})]
-        self.xpred.copy_(self.x_curr)
-        if order == 2:
-            self.xpred.add_(self.xh1, alpha=1.5).add_(self.xh2, alpha=-0.5)
-        else:
-            self.xpred.add_(self.xh1, alpha=1.0)
+        # predictor xn0 = x_n + beta1 (x_n - x_{n-1}) [+ beta2 (x_{n-1} - x_{n-2})],
+        # constant-step coefficients: order 1 beta1 = 1; order 2 beta1 = 1.5,
+        # beta2 = -0.5
+        b1, b2 = (1.5, -0.5) if order == 2 else (1.0, 0.0)
+        check(lib().xg_tia_predict(self.dev.h, order, b1, b2, _ptr(self.x_curr),
+                                   _ptr(self.xh1), _ptr(self.xh2), _ptr(self.xpred)),
+              "predict")
         self.x.copy_(self.xpred)
         try:
             self._newton(TRAN_PARAMS, transient=True, order=order, dt=dt, t=t_next,
@@ -169,6 +184,7 @@ class Transient:
         except NewtonFailure:
             if dt < 1e-18:
                 raise
+            self.step_failures += 1
             self.x.copy_(self.x_curr)
             self.step(dt / 2)
             self.step(dt / 2)
