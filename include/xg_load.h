@@ -272,6 +272,33 @@ int xg_dense_solve(xg_handle *h, const double *J, const double *rhs, double *x,
                    const double *wa, const double *wb, double reltol,
                    double abstol, const int32_t *active, double *stats);
 
+/* The Newton linear solve per component with a static-pivot sparse LU
+   (replaces the KLU refactor-and-solve of N_LAS_AmesosSolver.C:231-470;
+   KLU's threshold rule, tol 0.001, is checked on every factorisation and a
+   component that fails it is refactored by the dense partial-pivoting LU in
+   the same call).  Arguments, update and stats as xg_dense_solve. */
+int xg_lu_solve(xg_handle *h, const double *J, const double *rhs, double *x,
+                const double *wa, const double *wb, double reltol,
+                double abstol, const int32_t *active, double *stats);
+
+/* Sparse LU statistics: info[0] filled factor entries, [1] multiply-adds
+   per factorisation, [2] most factor levels of a component, [3] most
+   triangular-solve levels, [4] components without a static factorisation,
+   [5] components whose factors exceed LDS, [6] components refactored with
+   pivoting by the last xg_lu_solve (synchronises; -1 without a device),
+   [7] factor and solve tasks.  No counterpart in the reference. */
+int xg_lu_info(xg_handle *h, int64_t info[8]);
+
+/* Host copy of the sparse LU plan, for inspection: sizes[0..5] = n_nodes,
+   CSR entries, components, levels + 1, tasks, multiply-add terms; arrays
+   (may be NULL, then only sizes) = prow[n_nodes], qcol[n_nodes],
+   lu_pos[CSR entries], ok[components], lv_comp[3 components + 1],
+   lv_ptr[levels + 1], t_dst[tasks], t_div[tasks], t_op[tasks + 1],
+   ops[2 terms]; voff[components + 1] (may be NULL).  Layout as described
+   for LuPlan in the library sources.  No counterpart in the reference. */
+int xg_lu_export(const xg_handle *h, int64_t sizes[6], int32_t *const *arrays,
+                 int64_t *voff);
+
 /* ONESTEP predictor (N_TIA_OneStep.C:115-133): xpred = xh0 + beta1 xh1
    (+ beta2 xh2 at order 2), with xh1 = x_n - x_{n-1}, xh2 = x_{n-1} -
    x_{n-2}; beta from updateCoeffs (N_TIA_OneStep.C:1380-1405). */

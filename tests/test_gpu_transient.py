@@ -146,3 +146,86 @@ def test_batched_monte_carlo_equals_each_alone():
         assert np.array_equal(tb, ta)
         assert wa[0].tobytes() == wb[k].tobytes(), k
         assert wa[1].tobytes() == wb[len(parts) + k].tobytes(), k
+
+
+def _two_rings():
+    from xyce_amd import circuits
+    return circuits.merge([circuits.ring_oscillator_mos1(21, seed=1),
+                           circuits.ring_oscillator_bsim3(301, seed=2)])
+
+
+@pytest.mark.parametrize("mode", ["dominant", "fallback"])
+def test_sparse_lu_against_numpy(mode):
+    """xg_lu_solve solves per component like LAPACK.  "fallback": one
+    component gets a tiny pivot under large off-diagonal entries, so its
+    static pivots fail the threshold test and the partial-pivoting LU takes
+    it over in the same call; the other stays on the sparse path."""
+    import ctypes
+    import torch
+    from xyce_amd._lib import check, lib
+    from xyce_amd.loader import DeviceLoad, _ptr
+    nl = _two_rings()
+    nl.layout()
+    dev = DeviceLoad(nl)
+    rp, ci = dev.pattern()
+    n = nl.n_nodes
+    rng = np.random.default_rng(4)
+    vals = rng.uniform(-1, 1, len(ci))
+    rows = np.repeat(np.arange(n + 1), np.diff(rp))
+    vals[rows == ci] += 20.0
+    nc = ctypes.c_int32()
+    comp = np.zeros(n, np.int32)
+    check(lib().xg_components(dev.h, ctypes.byref(nc), comp.ctypes.data), "components")
+    assert nc.value == 2
+    if mode == "fallback":
+        # component 0's first static pivot (nothing updates it): 1e-9, the
+        # rest of its column 5.0, so |l| = 5e9 fails the threshold test
+        P = dev.lu_plan()
+        r0, c0 = int(P["prow"][0]), int(P["qcol"][0])  # position 0 of component 0
+        assert comp[r0] == 0
+        vals[(rows == r0) & (ci == c0)] = 1e-9
+        col = (ci == c0) & (rows != r0) & (rows < n)
+        assert col.any()
+        vals[col] = 5.0
+    rhs = rng.uniform(-1, 1, n + 1)
+    rhs[n] = 0.0
+    cu = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda")
+    J, R, X = cu(vals), cu(rhs), cu(np.zeros(n + 1))
+    W = cu(np.ones(n + 1))
+    stats = cu(np.zeros(4 * nc.value))
+    check(lib().xg_lu_solve(dev.h, _ptr(J), _ptr(R), _ptr(X), _ptr(W), _ptr(W), 0.0, 1.0,
+                            None, _ptr(stats)), "solve")
+    info = dev.lu_info()
+    print(mode, info, flush=True)
+    assert info["fallbacks"] == (1 if mode == "fallback" else 0)
+    x = X.cpu().numpy()
+    for c in range(nc.value):
+        idx = np.flatnonzero(comp == c)
+        loc = -np.ones(n + 1, int)
+        loc[idx] = np.arange(len(idx))
+        A = np.zeros((len(idx), len(idx)))
+        m = (loc[rows] >= 0) & (loc[ci] >= 0)
+        A[loc[rows[m]], loc[ci[m]]] = vals[m]
+        ref = np.linalg.solve(A, rhs[idx])
+        assert np.allclose(x[idx], ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max()), c
+    st = stats.cpu().numpy().reshape(-1, 4)
+    assert not st[:, 2].any()
+    assert np.array_equal(st[:, 3], np.bincount(comp))
+
+
+def test_sparse_and_dense_solvers_give_the_same_waveforms():
+    """A ring transient with the sparse LU and with the dense pivoting LU:
+    the solvers round differently, the waveforms agree to far below the
+    tolerance."""
+    from xyce_amd import circuits
+    from xyce_amd.transient import Transient
+    mk = lambda: circuits.ring_oscillator_bsim3(31, seed=3, ramp=50e-12)
+    a = Transient(mk(), solver="lu")
+    ta, wa = a.run(0.3e-9, 1e-12, ["out0", "out7"], every=5)
+    b = Transient(mk(), solver="dense")
+    tb, wb = b.run(0.3e-9, 1e-12, ["out0", "out7"], every=5)
+    err = np.abs(wa - wb).max()
+    print("sparse vs dense: max difference %.3g V, Newton %d / %d, fallbacks now %d"
+          % (err, a.newton_total, b.newton_total, a.dev.lu_info()["fallbacks"]), flush=True)
+    assert np.array_equal(ta, tb)
+    assert err <= 1e-6

@@ -1,0 +1,204 @@
+"""Static-pivot sparse LU plan (xg_lu_export), checked on the CPU.
+
+The task lists are executed here level by level exactly as the device
+kernel runs them (one value per task, terms in list order), with every
+index bounds-checked and every read checked to come from an earlier level.
+The solutions are compared with LAPACK (numpy.linalg.solve) on random
+diagonally dominant values and on real Newton Jacobians of the oracle's
+transients; the threshold test (|l_ik| <= 1000) is evaluated as the device
+evaluates it."""
+import numpy as np
+import pytest
+
+from xyce_amd import circuits
+from xyce_amd.loader import DeviceLoad
+
+
+def _comp_starts(dev):
+    import ctypes
+    from xyce_amd._lib import check, lib
+    n = dev.n_rows - 1
+    nc = ctypes.c_int32()
+    comp = np.zeros(max(n, 1), np.int32)
+    check(lib().xg_components(dev.h, ctypes.byref(nc), comp.ctypes.data), "components")
+    cstart = np.concatenate([[0], np.cumsum(np.bincount(comp[:n], minlength=nc.value))])
+    return comp[:n], cstart
+
+
+def run_plan(dev, vals, rhs):
+    """Executes the plan; returns (x, components that fail the pivot test)."""
+    P = dev.lu_plan()
+    rp, _ = dev.pattern()
+    n = dev.n_rows - 1
+    comp, cstart = _comp_starts(dev)
+    x = np.zeros(n + 1)
+    failed = []
+    for c in range(len(cstart) - 1):
+        beg, m = cstart[c], cstart[c + 1] - cstart[c]
+        assert P["ok"][c] == 1
+        nv = int(P["voff"][c + 1] - P["voff"][c])
+        prow, qcol = P["prow"][beg:beg + m], P["qcol"][beg:beg + m]
+        members = np.flatnonzero(comp == c)
+        assert np.array_equal(np.sort(prow), members)
+        assert np.array_equal(np.sort(qcol), members)
+        val = np.zeros(nv)
+        for g in prow:
+            for k in range(rp[g], rp[g + 1]):
+                p = P["lu_pos"][k]
+                if p >= 0:
+                    assert p < nv
+                    val[p] = vals[k]
+        vec = rhs[prow].astype(float)
+        lv = P["lv_comp"][3 * c:3 * c + 4]
+        bad = False
+        for ph in range(3):
+            for L in range(lv[ph], lv[ph + 1]):
+                new = {}
+                for k in range(P["lv_ptr"][L], P["lv_ptr"][L + 1]):
+                    d, dv = P["t_dst"][k], P["t_div"][k]
+                    src, lim = (val, nv) if ph == 0 else (vec, m)
+                    assert 0 <= d < lim and d not in new
+                    s = src[d]
+                    for o in range(P["t_op"][k], P["t_op"][k + 1]):
+                        a, b = P["ops"][o]
+                        assert 0 <= a < nv and 0 <= b < lim
+                        assert b not in new and (ph > 0 or a not in new)
+                        s = s - val[a] * src[b]
+                    if ph == 0 and dv >= 0:
+                        assert dv < nv and dv not in new
+                        s = s / val[dv]
+                        bad |= not abs(s) <= 1000.0
+                    elif ph == 0 and dv == -2:
+                        bad |= not (abs(s) > 0.0 and np.isfinite(s))
+                    elif ph == 2:
+                        s = s / val[dv]
+                    new[d] = s
+                for d, s in new.items():
+                    (val if ph == 0 else vec)[d] = s
+        if bad:
+            failed.append(c)
+        x[qcol] = vec
+    return x, failed
+
+
+def _dense(dev, vals, idx):
+    rp, ci = dev.pattern()
+    n = dev.n_rows - 1
+    rows = np.repeat(np.arange(n + 1), np.diff(rp))
+    loc = -np.ones(n + 1, int)
+    loc[idx] = np.arange(len(idx))
+    A = np.zeros((len(idx), len(idx)))
+    m = (loc[rows] >= 0) & (loc[ci] >= 0)
+    A[loc[rows[m]], loc[ci[m]]] = vals[m]
+    return A
+
+
+def _rc_ladder():
+    nl = circuits.Netlist("rc ladder")
+    nl.add_vsrc("V1", "n0", "0", 1.0)
+    for k in range(40):
+        nl.add_res("R%d" % k, "n%d" % k, "n%d" % (k + 1), 100.0 + k)
+        nl.add_cap("C%d" % k, "n%d" % (k + 1), "0", 1e-15)
+    return nl
+
+
+CASES = [
+    lambda: circuits.ring_oscillator_mos1(11, seed=1),
+    lambda: circuits.ring_oscillator_mos1(101, seed=1),
+    lambda: circuits.ring_oscillator_bsim3(301, seed=3),
+    lambda: circuits.merge(circuits.monte_carlo_rings(4, seed=9, stages=(11, 41))),
+    _rc_ladder,
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_plan_solves_like_lapack(case):
+    nl = CASES[case]()
+    nl.layout()
+    dev = DeviceLoad(nl, plan_only=True)
+    rp, ci = dev.pattern()
+    n = dev.n_rows - 1
+    rng = np.random.default_rng(case)
+    vals = rng.uniform(-1, 1, len(ci))
+    rows = np.repeat(np.arange(n + 1), np.diff(rp))
+    vals[rows == ci] += 20.0
+    # a source's branch row has no diagonal: keep its +-1 entries dominant
+    rhs = rng.uniform(-1, 1, n + 1)
+    rhs[n] = 0.0
+    x, failed = run_plan(dev, vals, rhs)
+    comp, cstart = _comp_starts(dev)
+    info = dev.lu_info()
+    assert info["static_fail"] == 0 and info["fallbacks"] == -1
+    for c in range(len(cstart) - 1):
+        if c in failed:
+            continue
+        idx = np.flatnonzero(comp == c)
+        ref = np.linalg.solve(_dense(dev, vals, idx), rhs[idx])
+        assert np.allclose(x[idx], ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+
+
+def test_levels_stay_shallow():
+    """Nested dissection keeps a 301-stage ring's factorisation to a few
+    dozen dependent levels (the natural order would need ~300)."""
+    nl = circuits.ring_oscillator_bsim3(301, seed=3)
+    nl.layout()
+    info = DeviceLoad(nl, plan_only=True).lu_info()
+    assert info["max_factor_levels"] <= 48
+    assert info["fnnz"] < 12 * 303
+
+
+def test_same_plan_alone_and_in_batch():
+    parts = circuits.monte_carlo_rings(3, seed=5, stages=(11, 31))
+    batch = circuits.merge(parts)
+    batch.layout()
+    B = DeviceLoad(batch, plan_only=True)
+    PB = B.lu_plan()
+    _, cs = _comp_starts(B)
+    for k in range(3):
+        p = circuits.monte_carlo_rings(3, seed=5, stages=(11, 31))[k]
+        p.layout()
+        A = DeviceLoad(p, plan_only=True)
+        PA = A.lu_plan()
+        la = PA["lv_comp"]
+        lb = PB["lv_comp"][3 * k:3 * k + 4]
+        assert np.array_equal(np.diff(la), np.diff(lb))
+        ta = slice(PA["lv_ptr"][la[0]], PA["lv_ptr"][la[3]])
+        tb = slice(PB["lv_ptr"][lb[0]], PB["lv_ptr"][lb[3]])
+        assert np.array_equal(PA["t_dst"][ta], PB["t_dst"][tb])
+        assert np.array_equal(PA["t_div"][ta], PB["t_div"][tb])
+        oa = PA["ops"][PA["t_op"][ta.start]:PA["t_op"][ta.stop]]
+        ob = PB["ops"][PB["t_op"][tb.start]:PB["t_op"][tb.stop]]
+        assert np.array_equal(oa, ob)
+
+
+@pytest.mark.parametrize("which", ["mos1", "bsim3"])
+def test_real_jacobians(which):
+    """Every Newton Jacobian of a short ring transient (DC operating point
+    included): the static pivots either pass the threshold test and solve
+    like LAPACK, or fail it (then the device refactors with pivoting)."""
+    from oracle.transient_ref import RefTransient
+    mk = {"mos1": lambda: circuits.ring_oscillator_mos1(11, seed=1, ramp=50e-12),
+          "bsim3": lambda: circuits.ring_oscillator_bsim3(11, seed=3, ramp=50e-12)}[which]
+    r = RefTransient(mk())
+    mats = []
+    orig = r._assemble
+
+    def hook(transient, order, dt):
+        orig(transient, order, dt)
+        mats.append((r.J.copy(), r.rhs.copy()))
+    r._assemble = hook
+    r.run(0.1e-9, 1e-12, ["out0"], every=50)
+    nl = mk()
+    nl.layout()
+    dev = DeviceLoad(nl, plan_only=True)
+    n = dev.n_rows - 1
+    idx = np.arange(n)
+    passed = 0
+    for J, rhs in mats[::5]:
+        x, failed = run_plan(dev, J, rhs)
+        if failed:
+            continue
+        passed += 1
+        ref = np.linalg.solve(_dense(dev, J, idx), rhs[idx])
+        assert np.max(np.abs(x[idx] - ref)) <= 1e-12 * np.abs(ref).max()
+    assert passed >= 0.8 * len(mats[::5])

@@ -79,6 +79,10 @@ SIGNATURES = [
                                         ctypes.c_double] + [ctypes.c_void_p] * 10),
     ("xg_dense_solve", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 +
      [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_lu_solve", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 +
+     [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_lu_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_lu_export", ctypes.c_int, [ctypes.c_void_p] * 4),
     ("xg_set_eval_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_set_output_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),

@@ -24,6 +24,8 @@ SOURCES = [
     "xg_reduce.hip",
     "xg_linear.hip",
     "xg_tia.hip",
+    "xg_lu.hip",
+    "xg_lu_setup.cpp",
     "xg_setup.cpp",
     "xg_bsim3_setup.cpp",
 ]

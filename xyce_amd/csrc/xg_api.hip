@@ -35,8 +35,9 @@ struct Handle {
   // linear devices and the time their sources are evaluated at
   LinArgs lin{};
   double time = 0.0, bp_tol = 0.0;
-  // per-component dense solve
+  // per-component dense solve, and the static-pivot sparse LU
   DenseSolveArgs ds{};
+  SparseLuArgs lu{};
   int32_t *d_rowptr = nullptr, *d_colidx = nullptr;
   int32_t *slot_comp = nullptr;
   // reduction programs
@@ -324,6 +325,96 @@ int xg_dense_solve(xg_handle *H, const double *J, const double *rhs, double *x,
   return e == hipSuccess ? XG_OK : hip_fail(e, "dense solve");
 }
 
+int xg_lu_solve(xg_handle *H, const double *J, const double *rhs, double *x,
+                const double *wa, const double *wb, double reltol, double abstol,
+                const int32_t *active, double *stats) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !J || !rhs || !x || !wa || !wb || !stats)
+    return fail(XG_E_ARG, "xg_lu_solve: bad argument");
+  xg::Handle &h = H->h;
+  xg::SparseLuArgs u = h.lu;
+  u.J = J;
+  u.rhs = rhs;
+  u.x = x;
+  u.wa = wa;
+  u.wb = wb;
+  u.reltol = reltol;
+  u.abstol = abstol;
+  u.active = active;
+  u.stats = stats;
+  hipError_t e = xg::launch_sparse_lu(u, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "sparse LU");
+  // components whose static pivots fail the threshold test: dense LU
+  // with partial pivoting, in the same stream, no host round trip
+  xg::DenseSolveArgs a = h.ds;
+  a.J = J;
+  a.rhs = rhs;
+  a.x = x;
+  a.wa = wa;
+  a.wb = wb;
+  a.reltol = reltol;
+  a.abstol = abstol;
+  a.active = h.lu.fallback;
+  a.only_flagged = 1;
+  a.stats = stats;
+  e = xg::launch_dense_solve(a, h.stream);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "pivoting LU");
+}
+
+int xg_lu_info(xg_handle *H, int64_t *info) {
+  if (!H || !info) return fail(XG_E_ARG, "xg_lu_info: bad argument");
+  xg::Handle &h = H->h;
+  const xg::LuPlan &L = h.plan.lu;
+  int64_t n_global = 0;
+  const int32_t nc = (int32_t)h.plan.comp_start.size() - 1;
+  for (int32_t k = 0; k < nc; ++k)
+    if (!xg::sparse_lu_fits_lds(h.plan.comp_start[k + 1] - h.plan.comp_start[k],
+                                L.voff[k + 1] - L.voff[k]))
+      n_global++;
+  info[0] = L.fnnz;
+  info[1] = L.factor_ops;
+  info[2] = L.max_factor_levels;
+  info[3] = L.max_solve_levels;
+  info[4] = L.n_static_fail;
+  info[5] = n_global;
+  info[6] = -1;
+  if (!h.plan_only && nc > 0) {
+    std::vector<int32_t> f(nc);
+    if (hipStreamSynchronize(h.stream) != hipSuccess ||
+        hipMemcpy(f.data(), h.lu.fallback, sizeof(int32_t) * nc, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+      return fail(XG_E_HIP, "xg_lu_info: copy failed");
+    info[6] = 0;
+    for (int32_t v : f) info[6] += v != 0;
+  } else if (nc == 0) {
+    info[6] = 0;
+  }
+  info[7] = (int64_t)L.t_dst.size();
+  return XG_OK;
+}
+
+int xg_lu_export(const xg_handle *H, int64_t *sizes, int32_t *const *arrays,
+                 int64_t *voff) {
+  if (!H || !sizes) return fail(XG_E_ARG, "xg_lu_export: bad argument");
+  const xg::Plan &P = H->h.plan;
+  const xg::LuPlan &L = P.lu;
+  sizes[0] = P.n_nodes;
+  sizes[1] = (int64_t)P.colidx.size();
+  sizes[2] = (int64_t)P.comp_start.size() - 1;
+  sizes[3] = (int64_t)L.lv_ptr.size();
+  sizes[4] = (int64_t)L.t_dst.size();
+  sizes[5] = (int64_t)L.ops.size() / 2;
+  if (!arrays) return XG_OK;
+  const std::vector<int32_t> *src[9] = {&L.prow, &L.qcol, &L.lu_pos, &L.ok, &L.lv_comp,
+                                        &L.lv_ptr, &L.t_dst, &L.t_div, &L.t_op};
+  for (int k = 0; k < 9; ++k)
+    if (arrays[k]) std::memcpy(arrays[k], src[k]->data(), src[k]->size() * sizeof(int32_t));
+  if (arrays[9]) std::memcpy(arrays[9], L.ops.data(), L.ops.size() * sizeof(int32_t));
+  if (voff) std::memcpy(voff, L.voff.data(), L.voff.size() * sizeof(int64_t));
+  return XG_OK;
+}
+
 int xg_tia_predict(xg_handle *H, int32_t order, double beta1, double beta2,
                    const double *xh0, const double *xh1, const double *xh2,
                    double *xpred) {
@@ -471,6 +562,41 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
     d.colidx = h.d_colidx;
     chk(d.work = h.alloc<double>((size_t)(P.comp_work > 0 ? P.comp_work : 1)));
     chk(d.bwork = h.alloc<double>((size_t)P.n_nodes + 1));
+    // sparse LU task lists; factor blocks of components beyond LDS
+    auto &u = h.lu;
+    const xg::LuPlan &L = P.lu;
+    u.n_comp = d.n_comp;
+    int64_t gwork = 0;
+    std::vector<int64_t> voff(L.voff);
+    for (int32_t k = 0; k < u.n_comp; ++k) {
+      const int64_t m = P.comp_start[k + 1] - P.comp_start[k];
+      if (!xg::sparse_lu_fits_lds(m, L.voff[k + 1] - L.voff[k])) u.any_global = 1;
+    }
+    if (u.any_global) gwork = L.voff[u.n_comp];
+    u.cstart = cs;
+    chk(u.prow = h.upload(L.prow));
+    chk(u.qcol = h.upload(L.qcol));
+    chk(u.lu_pos = h.upload(L.lu_pos));
+    chk(u.voff = h.upload(voff));
+    chk(u.ok = h.upload(L.ok));
+    chk(u.lv_comp = h.upload(L.lv_comp));
+    chk(u.lv_ptr = h.upload(L.lv_ptr));
+    chk(u.t_dst = h.upload(L.t_dst));
+    chk(u.t_div = h.upload(L.t_div));
+    chk(u.t_op = h.upload(L.t_op));
+    {
+      int32_t *o;
+      chk(o = h.upload(L.ops));
+      u.ops = (const int2 *)o;
+    }
+    u.rowptr = h.d_rowptr;
+    u.inv_tau = 1.0 / 0.001;
+    chk(u.val_work = h.alloc<double>((size_t)(gwork > 0 ? gwork : 1)));
+    u.vec_work = d.bwork;
+    chk(u.fallback = h.alloc<int32_t>((size_t)(u.n_comp > 0 ? u.n_comp : 1)));
+    if (ok && hipMemset(u.fallback, 0, sizeof(int32_t) * (u.n_comp > 0 ? u.n_comp : 1)) !=
+                  hipSuccess)
+      ok = false;
   }
   // linear devices
   h.lin.n_slots = P.lin_slots;

@@ -176,13 +176,40 @@ struct DenseSolveArgs {
   double reltol, abstol;
   const int32_t *active;          // n_comp flags or null (all)
   int32_t norm_only = 0;          // only stats[0] = max |rhs| per component
+  int32_t only_flagged = 0;       // inactive components keep their stats
   double *work, *bwork;           // dense matrices / right-hand sides
   double *stats;                  // n_comp x 4: max|rhs|, wRMS(dx), singular, n
+};
+
+// Static-pivot sparse LU per component (task lists of lu_analyse).
+struct SparseLuArgs {
+  int32_t n_comp = 0;
+  int32_t any_global = 0;         // some component's factors exceed LDS
+  const int32_t *cstart;          // n_comp + 1
+  const int32_t *prow, *qcol;     // position -> global equation / unknown
+  const int32_t *lu_pos;          // CSR entry -> position in its block (-1)
+  const int64_t *voff;            // n_comp + 1 value offsets
+  const int32_t *ok;              // static factorisation available
+  const int32_t *lv_comp, *lv_ptr, *t_dst, *t_div, *t_op;
+  const int2 *ops;
+  const int32_t *rowptr;
+  const double *J, *rhs;
+  double *x;
+  const double *wa, *wb;
+  double reltol, abstol;
+  double inv_tau;                 // largest accepted |l_ik|
+  const int32_t *active;
+  double *val_work, *vec_work;    // factors / solution of global components
+  double *stats;
+  int32_t *fallback;              // n_comp: 1 = refactor with pivoting
 };
 
 hipError_t launch_tia_assemble(const TiaAssembleArgs &a, hipStream_t s);
 hipError_t launch_dense_solve(const DenseSolveArgs &a, hipStream_t s);
 int dense_lds_max();
+constexpr int kDenseMax = 2048;  // largest component with a dense form
+hipError_t launch_sparse_lu(const SparseLuArgs &a, hipStream_t s);
+bool sparse_lu_fits_lds(int64_t n, int64_t fnnz);
 hipError_t launch_predict(int64_t n1, int32_t order, double beta1, double beta2,
                           const double *xh0, const double *xh1, const double *xh2,
                           double *out, hipStream_t s);

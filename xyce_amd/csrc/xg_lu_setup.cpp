@@ -1,0 +1,475 @@
+// Symbolic analysis of the fixed-pattern sparse LU used for the Newton
+// solve of each independent sub-circuit.
+//
+// The reference solves J dx = -F with KLU (N_LAS_AmesosSolver.C:231-470):
+// a fill-reducing ordering, then partial pivoting with a preference for the
+// diagonal (a diagonal entry is kept while |a_kk| >= 0.001 max_i |a_ik|).
+// Here the pivot sequence is frozen at setup from the structure alone:
+//
+//  1. matching: each row is assigned a column so that the diagonal is
+//     structurally nonzero.  A voltage source's branch row takes its
+//     positive (else negative) node's column and that node's row takes the
+//     branch column (the +-1 entries of N_DEV_Vsrc.C:1023-1036); the other
+//     rows keep their own column when they can, and augmenting paths
+//     settle the rest.
+//  2. ordering: nested dissection of the symmetrised graph (BFS level-set
+//     separators from a pseudo-peripheral node), with rows of very high
+//     degree (supply rails) last.  It keeps the elimination tree shallow,
+//     so the factorisation has few dependent levels.
+//  3. fill: the row structure of L from the elimination tree (the filled
+//     pattern is made structurally symmetric); U is its transpose.
+//  4. tasks: every value of L and U is one task (a Crout dot product,
+//     terms in ascending order), grouped into levels of independent tasks;
+//     the two triangular solves likewise, one task per row.
+//
+// The device checks every factorisation against KLU's threshold rule
+// (|l_ik| <= 1 / 0.001, nonzero finite pivots) and refactors a component
+// that fails with the dense partial-pivoting kernel.
+//
+// Everything here uses component-local indices only, so a sub-circuit gets
+// the same factorisation alone and inside a batch.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "xg_internal.h"
+#include "xg_setup.h"
+
+namespace xg {
+
+namespace {
+
+using Graph = std::vector<std::vector<int32_t>>;
+
+// Breadth-first levels of the vertices with in[v] == tag, from root.
+int bfs_levels(const Graph &g, const std::vector<int32_t> &in, int32_t tag,
+               int32_t root, std::vector<int32_t> &lev,
+               std::vector<int32_t> &order) {
+  order.clear();
+  order.push_back(root);
+  lev[root] = 0;
+  int depth = 0;
+  for (size_t h = 0; h < order.size(); ++h) {
+    const int32_t v = order[h];
+    for (int32_t u : g[v])
+      if (in[u] == tag && lev[u] < 0) {
+        lev[u] = lev[v] + 1;
+        depth = std::max(depth, lev[u]);
+        order.push_back(u);
+      }
+  }
+  return depth + 1;
+}
+
+struct Dissector {
+  const Graph &g;
+  std::vector<int32_t> in;     // current tag of each vertex (-1: ordered)
+  std::vector<int32_t> lev;
+  std::vector<int32_t> &out;   // elimination order
+  int32_t next_tag = 1;
+  Dissector(const Graph &g_, std::vector<int32_t> &o)
+      : g(g_), in(g_.size(), -1), lev(g_.size(), -1), out(o) {}
+
+  // Orders the vertex set `set` (all tagged `tag`).
+  void run(std::vector<int32_t> set, int32_t tag) {
+    if (set.size() <= 8) {
+      for (int32_t v : set) {
+        out.push_back(v);
+        in[v] = -1;
+      }
+      return;
+    }
+    // connected pieces are ordered one after the other
+    std::vector<int32_t> order;
+    for (int32_t v : set) lev[v] = -1;
+    std::vector<std::vector<int32_t>> pieces;
+    for (int32_t v : set)
+      if (lev[v] < 0) {
+        bfs_levels(g, in, tag, v, lev, order);
+        pieces.push_back(order);
+      }
+    if (pieces.size() > 1) {
+      for (auto &p : pieces) {
+        const int32_t t = next_tag++;
+        for (int32_t v : p) in[v] = t;
+        run(p, t);
+      }
+      return;
+    }
+    // pseudo-peripheral root: the last vertex of a BFS, twice
+    int32_t root = set[0];
+    int depth = 0;
+    for (int it = 0; it < 2; ++it) {
+      for (int32_t v : set) lev[v] = -1;
+      depth = bfs_levels(g, in, tag, root, lev, order);
+      root = order.back();
+    }
+    for (int32_t v : set) lev[v] = -1;
+    depth = bfs_levels(g, in, tag, root, lev, order);
+    if (depth < 3) {
+      for (int32_t v : set) {
+        out.push_back(v);
+        in[v] = -1;
+      }
+      return;
+    }
+    std::vector<int32_t> cnt(depth, 0);
+    for (int32_t v : set) cnt[lev[v]]++;
+    // the smallest level in the middle third is the separator
+    int lo = std::max(1, depth / 3), hi = std::min(depth - 2, (2 * depth) / 3);
+    if (hi < lo) lo = hi = depth / 2;
+    int sep = lo;
+    for (int l = lo; l <= hi; ++l)
+      if (cnt[l] < cnt[sep]) sep = l;
+    std::vector<int32_t> a, b, s;
+    for (int32_t v : set) {
+      if (lev[v] < sep)
+        a.push_back(v);
+      else if (lev[v] > sep)
+        b.push_back(v);
+      else
+        s.push_back(v);
+    }
+    const int32_t ta = next_tag++, tb = next_tag++;
+    for (int32_t v : a) in[v] = ta;
+    for (int32_t v : b) in[v] = tb;
+    for (int32_t v : s) in[v] = -2;  // kept out of both halves
+    run(a, ta);
+    run(b, tb);
+    for (int32_t v : s) {
+      out.push_back(v);
+      in[v] = -1;
+    }
+  }
+};
+
+// Row -> column matching with a structurally nonzero diagonal.
+// Returns false if the pattern is structurally singular.
+bool match_rows(const Graph &rows, int32_t n, const std::vector<int32_t> &pair_r,
+                const std::vector<int32_t> &pair_c, std::vector<int32_t> &col_of_row,
+                std::vector<int32_t> &row_of_col) {
+  col_of_row.assign(n, -1);
+  row_of_col.assign(n, -1);
+  auto has = [&](int32_t r, int32_t c) {
+    return std::binary_search(rows[r].begin(), rows[r].end(), c);
+  };
+  for (size_t k = 0; k < pair_r.size(); ++k) {
+    const int32_t br = pair_r[k], nd = pair_c[k];
+    if (col_of_row[br] >= 0 || col_of_row[nd] >= 0 || row_of_col[nd] >= 0 ||
+        row_of_col[br] >= 0 || !has(br, nd) || !has(nd, br))
+      continue;
+    col_of_row[br] = nd;
+    row_of_col[nd] = br;
+    col_of_row[nd] = br;
+    row_of_col[br] = nd;
+  }
+  for (int32_t r = 0; r < n; ++r)
+    if (col_of_row[r] < 0 && row_of_col[r] < 0 && has(r, r)) {
+      col_of_row[r] = r;
+      row_of_col[r] = r;
+    }
+  std::vector<int32_t> seen(n, -1);
+  std::vector<std::pair<int32_t, size_t>> stack;
+  std::vector<int32_t> path_col;
+  for (int32_t r0 = 0; r0 < n; ++r0) {
+    if (col_of_row[r0] >= 0) continue;
+    // depth-first augmenting path from row r0
+    stack.assign(1, {r0, 0});
+    path_col.clear();
+    bool found = false;
+    while (!stack.empty() && !found) {
+      auto &top = stack.back();
+      const int32_t r = top.first;
+      if (top.second >= rows[r].size()) {
+        stack.pop_back();
+        if (!path_col.empty()) path_col.pop_back();
+        continue;
+      }
+      const int32_t c = rows[r][top.second++];
+      if (seen[c] == r0) continue;
+      seen[c] = r0;
+      path_col.push_back(c);
+      if (row_of_col[c] < 0) {
+        found = true;
+      } else {
+        stack.push_back({row_of_col[c], 0});
+      }
+    }
+    if (!found) return false;
+    // flip the path: stack rows pair with path columns
+    for (size_t k = 0; k < stack.size(); ++k) {
+      const int32_t r = stack[k].first, c = path_col[k];
+      col_of_row[r] = c;
+      row_of_col[c] = r;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+int lu_analyse(Plan &plan, std::string &err) {
+  const int32_t n_all = plan.n_nodes, G = plan.n_nodes;
+  const int32_t nc = (int32_t)plan.comp_start.size() - 1;
+  LuPlan &L = plan.lu;
+  L = LuPlan{};
+  L.prow.assign(n_all, 0);
+  L.qcol.assign(n_all, 0);
+  L.lu_pos.assign(plan.colidx.size(), -1);
+  L.voff.assign(nc + 1, 0);
+  L.ok.assign(nc, 0);
+  L.lv_comp.assign((size_t)3 * nc + 1, 0);
+  L.lv_ptr.assign(1, 0);
+  L.t_op.assign(1, 0);
+  // voltage-source pairings (branch row, node column), global ids
+  std::vector<int32_t> vs_of_node(n_all + 1, -1);
+  std::vector<std::pair<int32_t, int32_t>> vpairs;
+  for (int32_t s = 0; s < plan.lin_slots; ++s) {
+    if (plan.lin_type[s] != LIN_V) continue;
+    const size_t S = (size_t)plan.lin_slots;
+    const int32_t p = plan.lin_lid[s], q = plan.lin_lid[S + s],
+                  br = plan.lin_lid[2 * S + s];
+    const int32_t nd = p != G ? p : q;
+    if (nd != G && br != G) vpairs.push_back({br, nd});
+  }
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> comp_pairs(nc);
+  std::vector<int32_t> comp_of(n_all, -1);
+  for (int32_t c = 0; c < nc; ++c)
+    for (int32_t j = plan.comp_start[c]; j < plan.comp_start[c + 1]; ++j)
+      comp_of[plan.comp_nodes[j]] = c;
+  for (auto &pr : vpairs) comp_pairs[comp_of[pr.first]].push_back(pr);
+
+  for (int32_t c = 0; c < nc; ++c) {
+    const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
+    const int32_t *nodes = plan.comp_nodes.data() + beg;
+    auto loc = [&](int32_t g) { return plan.comp_local[g]; };
+    // local structure of the component's rows
+    Graph rows(n);
+    for (int32_t r = 0; r < n; ++r) {
+      const int32_t g = nodes[r];
+      for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
+        const int32_t col = plan.colidx[k];
+        if (col < G) rows[r].push_back(loc(col));
+      }
+      std::sort(rows[r].begin(), rows[r].end());
+      rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
+    }
+    std::vector<int32_t> pr, pc;
+    for (auto &p : comp_pairs[c]) {
+      pr.push_back(loc(p.first));
+      pc.push_back(loc(p.second));
+    }
+    std::vector<int32_t> col_of_row, row_of_col;
+    const bool matched = match_rows(rows, n, pr, pc, col_of_row, row_of_col);
+    // L.voff stays flat for a component without a static factorisation
+    if (!matched) {
+      L.n_static_fail++;
+      for (int ph = 0; ph < 3; ++ph)
+        L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
+      L.voff[c + 1] = L.voff[c];
+      for (int32_t i = 0; i < n; ++i) {
+        L.prow[beg + i] = nodes[i];
+        L.qcol[beg + i] = nodes[i];
+      }
+      continue;
+    }
+    // symmetrised graph of the matched matrix M(c, j) = A(row_of_col[c], j)
+    Graph sym(n);
+    for (int32_t cc = 0; cc < n; ++cc)
+      for (int32_t j : rows[row_of_col[cc]])
+        if (j != cc) {
+          sym[cc].push_back(j);
+          sym[j].push_back(cc);
+        }
+    size_t edges = 0;
+    for (auto &a : sym) {
+      std::sort(a.begin(), a.end());
+      a.erase(std::unique(a.begin(), a.end()), a.end());
+      edges += a.size();
+    }
+    // ordering: nested dissection; rows of very high degree last
+    const double avg = n ? (double)edges / n : 0.0;
+    const size_t hub = (size_t)std::max(32.0, std::max(8.0 * avg, 4.0 * std::sqrt((double)n)));
+    std::vector<int32_t> order;
+    order.reserve(n);
+    {
+      Dissector d(sym, order);
+      std::vector<int32_t> rest, hubs;
+      for (int32_t v = 0; v < n; ++v) (sym[v].size() > hub ? hubs : rest).push_back(v);
+      for (int32_t v : rest) d.in[v] = 0;
+      d.next_tag = 1;
+      d.run(rest, 0);
+      std::stable_sort(hubs.begin(), hubs.end(), [&](int32_t a, int32_t b) {
+        return sym[a].size() < sym[b].size();
+      });
+      for (int32_t v : hubs) order.push_back(v);
+    }
+    std::vector<int32_t> pinv(n);
+    for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
+    for (int32_t i = 0; i < n; ++i) {
+      L.qcol[beg + i] = nodes[order[i]];
+      L.prow[beg + i] = nodes[row_of_col[order[i]]];
+    }
+    // elimination tree and row structure of L (Liu's row-subtree walk)
+    std::vector<std::vector<int32_t>> lrow(n), ucol(n);
+    std::vector<int32_t> parent(n, -1), mark(n, -1);
+    for (int32_t i = 0; i < n; ++i) {
+      mark[i] = i;
+      for (int32_t u : sym[order[i]]) {
+        int32_t j = pinv[u];
+        if (j >= i) continue;
+        while (mark[j] != i) {
+          lrow[i].push_back(j);
+          mark[j] = i;
+          if (parent[j] < 0) {
+            parent[j] = i;
+            break;
+          }
+          j = parent[j];
+        }
+      }
+      std::sort(lrow[i].begin(), lrow[i].end());
+      for (int32_t j : lrow[i]) ucol[j].push_back(i);
+    }
+    // filled row-major pattern: row i = lrow[i], i, ucol[i] (all sorted)
+    std::vector<int64_t> rp(n + 1, 0);
+    for (int32_t i = 0; i < n; ++i)
+      rp[i + 1] = rp[i] + (int64_t)lrow[i].size() + 1 + (int64_t)ucol[i].size();
+    const int64_t fn = rp[n];
+    if (fn > INT32_MAX) {
+      err = "sparse LU: filled pattern of a component exceeds 2^31 entries";
+      return XG_E_ARG;
+    }
+    auto pos = [&](int32_t i, int32_t j) -> int32_t {
+      if (j < i) {
+        auto it = std::lower_bound(lrow[i].begin(), lrow[i].end(), j);
+        return (int32_t)(rp[i] + (it - lrow[i].begin()));
+      }
+      if (j == i) return (int32_t)(rp[i] + (int64_t)lrow[i].size());
+      auto it = std::lower_bound(ucol[i].begin(), ucol[i].end(), j);
+      return (int32_t)(rp[i] + (int64_t)lrow[i].size() + 1 + (it - ucol[i].begin()));
+    };
+    // CSR entry -> position: entry (g, col) of row g lands at
+    // (position of g's equation, position of col's unknown)
+    std::vector<int32_t> rpos(n);
+    for (int32_t i = 0; i < n; ++i) rpos[row_of_col[order[i]]] = i;
+    for (int32_t r = 0; r < n; ++r) {
+      const int32_t g = nodes[r];
+      for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
+        const int32_t col = plan.colidx[k];
+        if (col < G) L.lu_pos[k] = pos(rpos[r], pinv[loc(col)]);
+      }
+    }
+    L.voff[c + 1] = L.voff[c] + fn;
+    L.fnnz += fn;
+    L.ok[c] = 1;
+    // ---- factor tasks: values in the order diag, column of L, row of U
+    std::vector<int32_t> lev(fn, -1);
+    struct Task {
+      int32_t dst, div, lev;
+      std::vector<int32_t> ops;
+    };
+    std::vector<Task> tasks;
+    auto intersect = [&](int32_t i, int32_t j, std::vector<int32_t> &ops, int &lv) {
+      const auto &a = lrow[i], &b = lrow[j];
+      size_t x = 0, y = 0;
+      while (x < a.size() && y < b.size()) {
+        if (a[x] < b[y]) {
+          ++x;
+        } else if (b[y] < a[x]) {
+          ++y;
+        } else {
+          const int32_t m = a[x];
+          const int32_t pa = pos(i, m), pb = pos(m, j);
+          ops.push_back(pa);
+          ops.push_back(pb);
+          lv = std::max(lv, std::max(lev[pa], lev[pb]) + 1);
+          ++x;
+          ++y;
+        }
+      }
+    };
+    for (int32_t p = 0; p < n; ++p) {
+      {
+        Task t{pos(p, p), -2, 0, {}};
+        intersect(p, p, t.ops, t.lev);
+        lev[t.dst] = t.lev;
+        tasks.push_back(std::move(t));
+      }
+      const int32_t dp = pos(p, p);
+      for (int32_t i : ucol[p]) {  // L(i, p), i > p
+        Task t{pos(i, p), dp, lev[dp] + 1, {}};
+        intersect(i, p, t.ops, t.lev);
+        lev[t.dst] = t.lev;
+        tasks.push_back(std::move(t));
+      }
+      for (int32_t j : ucol[p]) {  // U(p, j), j > p
+        Task t{pos(p, j), -1, 0, {}};
+        intersect(p, j, t.ops, t.lev);
+        if (t.ops.empty()) continue;  // the scattered value is final
+        lev[t.dst] = t.lev;
+        tasks.push_back(std::move(t));
+      }
+    }
+    // ---- forward solve (L y = b, unit diagonal) and backward (U x = y)
+    std::vector<Task> fwd, bwd;
+    {
+      std::vector<int32_t> lf(n, -1), lb(n, -1);
+      for (int32_t i = 0; i < n; ++i) {
+        if (lrow[i].empty()) continue;
+        Task t{i, -1, 0, {}};
+        for (int32_t k : lrow[i]) {
+          t.ops.push_back(pos(i, k));
+          t.ops.push_back(k);
+          t.lev = std::max(t.lev, lf[k] + 1);
+        }
+        lf[i] = t.lev;
+        fwd.push_back(std::move(t));
+      }
+      for (int32_t i = n - 1; i >= 0; --i) {
+        Task t{i, pos(i, i), 0, {}};
+        for (int32_t j : ucol[i]) {
+          t.ops.push_back(pos(i, j));
+          t.ops.push_back(j);
+          t.lev = std::max(t.lev, lb[j] + 1);
+        }
+        lb[i] = t.lev;
+        bwd.push_back(std::move(t));
+      }
+    }
+    // ---- emit the three phases, tasks bucketed by level
+    auto emit = [&](std::vector<Task> &ts, int ph) {
+      int nlev = 0;
+      for (auto &t : ts) nlev = std::max(nlev, t.lev + 1);
+      std::vector<std::vector<int32_t>> by(nlev);
+      for (size_t k = 0; k < ts.size(); ++k) by[ts[k].lev].push_back((int32_t)k);
+      for (auto &lv : by) {
+        for (int32_t k : lv) {
+          const Task &t = ts[k];
+          L.t_dst.push_back(t.dst);
+          L.t_div.push_back(t.div);
+          L.ops.insert(L.ops.end(), t.ops.begin(), t.ops.end());
+          L.t_op.push_back((int32_t)(L.ops.size() / 2));
+          if (ph == 0) L.factor_ops += (int64_t)t.ops.size() / 2;
+        }
+        L.lv_ptr.push_back((int32_t)L.t_dst.size());
+      }
+      L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
+      return nlev;
+    };
+    const int f = emit(tasks, 0);
+    const int a = emit(fwd, 1);
+    const int b = emit(bwd, 2);
+    L.max_factor_levels = std::max(L.max_factor_levels, f);
+    L.max_solve_levels = std::max(L.max_solve_levels, a + b);
+    if (L.ops.size() / 2 > (size_t)INT32_MAX || L.t_dst.size() > (size_t)INT32_MAX) {
+      err = "sparse LU: task lists exceed 2^31 entries";
+      return XG_E_ARG;
+    }
+  }
+  return XG_OK;
+}
+
+}  // namespace xg

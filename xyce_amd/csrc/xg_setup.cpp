@@ -751,8 +751,8 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     int64_t w = 0;
     for (int32_t k = 0; k < nc; ++k) {
       const int64_t m = cnt[k + 1] - cnt[k];
-      plan.comp_aoff[k] = w;
-      if (m > dense_lds_max()) w += m * m;
+      plan.comp_aoff[k] = m <= kDenseMax ? w : -1;
+      if (m > dense_lds_max() && m <= kDenseMax) w += m * m;
     }
     plan.comp_work = w;
     // eval slot -> component (an instance's nodes all lie in one component)
@@ -777,7 +777,7 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     pbase += (int32_t)prog->cdst.size();
   }
   plan.n_partials = pbase;
-  return XG_OK;
+  return lu_analyse(plan, err);
 }
 
 }  // namespace xg

@@ -87,6 +87,35 @@ struct Circuit {
   ~Circuit();
 };
 
+// Fixed-pattern sparse LU of each component (built by lu_analyse).
+//
+// Position i of component c (0 <= i < n_c) holds equation prow[i] and
+// unknown qcol[i] (global ids): rows are first matched to columns so that
+// every diagonal entry is structurally nonzero (a voltage source's branch
+// row pairs with its node's column), then both are ordered by nested
+// dissection of the symmetrised graph.  The factors live in a filled,
+// structurally symmetric pattern, row-major with sorted columns; positions
+// are relative to the component's block (voff).  Three task phases per
+// component -- factor, forward solve, backward solve -- each a list of
+// levels whose tasks are independent.  A task computes one value:
+//   factor   v[dst] = (v[dst] - sum v[a] * v[b]) [/ v[div]]
+//   solve    y[dst] = (y[dst] - sum v[a] * y[b]) [/ v[div]]
+// with its terms in ascending order, so every result is computed by one
+// thread in one fixed order.
+struct LuPlan {
+  std::vector<int32_t> prow, qcol;    // by comp_start offsets
+  std::vector<int32_t> lu_pos;        // CSR entry -> position in its block, -1
+  std::vector<int64_t> voff;          // n_comp + 1 value offsets
+  std::vector<int32_t> ok;            // 1: matching found (static factorisation)
+  std::vector<int32_t> lv_comp;       // 3 n_comp + 1: phase -> first level
+  std::vector<int32_t> lv_ptr;        // level -> first task (n_levels + 1)
+  std::vector<int32_t> t_dst, t_div;  // t_div: -1 none, -2 pivot (check only)
+  std::vector<int32_t> t_op;          // task -> first op (n_tasks + 1)
+  std::vector<int32_t> ops;           // pairs (a, b)
+  int64_t fnnz = 0, factor_ops = 0;
+  int32_t max_factor_levels = 0, max_solve_levels = 0, n_static_fail = 0;
+};
+
 // A device-ready plan (host copies of everything that gets uploaded).
 struct Plan {
   int32_t n_nodes, n_state, n_store;
@@ -118,6 +147,7 @@ struct Plan {
   std::vector<int64_t> comp_aoff;
   std::vector<int32_t> slot_comp;  // MOSFET1 then BSIM3 eval slots -> component
   int64_t comp_work = 0;     // doubles of dense workspace (large components)
+  LuPlan lu;                 // static-pivot sparse factorisation per component
   ~Plan();
 };
 
@@ -134,6 +164,7 @@ int mos1_model_process(Mos1Model &m, const Options &opt, std::string &err);
 int mos1_instance_setup(const Mos1Model &m, Mos1Inst &in, const Options &opt,
                         std::string &err);
 int build_plan(Circuit &c, Plan &plan, std::string &err);
+int lu_analyse(Plan &plan, std::string &err);
 
 // BSIM3 hooks (xg_bsim3_setup.cpp).
 int bsim3_add_model(Circuit &c, int dtype, const ParamSet &p, int32_t *id,

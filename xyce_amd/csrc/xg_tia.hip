@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kBlock)
   double *stats = a.stats + 4 * (size_t)c;
   if ((kLds && n > kLdsN) || (!kLds && n <= kLdsN)) return;  // other variant
   if (a.active && !a.active[c]) {
-    if (t == 0) {
+    if (t == 0 && !a.only_flagged) {
       stats[0] = 0.0;
       stats[1] = 0.0;
       stats[2] = 0.0;
@@ -143,7 +143,17 @@ __global__ void __launch_bounds__(kBlock)
     }
     return;
   }
-  double *A = kLds ? lds_a : a.work + a.aoff[c];
+  if (!kLds && a.aoff[c] < 0 && !a.norm_only) {
+    // no dense form for a component this large
+    if (t == 0) {
+      stats[0] = 0.0;
+      stats[1] = 0.0;
+      stats[2] = 1.0;
+      stats[3] = (double)n;
+    }
+    return;
+  }
+  double *A = kLds ? lds_a : a.work + (a.aoff[c] < 0 ? 0 : a.aoff[c]);
   double *b = kLds ? sb : a.bwork + beg;
   const int32_t *nodes = a.nodes + beg;
   // gather the component's rows of J (and its RHS) into dense form

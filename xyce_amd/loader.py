@@ -108,6 +108,32 @@ class DeviceLoad:
                                    ci.ctypes.data_as(_lib.c_int32_p)), "get pattern")
         return rp, ci[: self.nnz]
 
+    def lu_plan(self):
+        """Host copy of the static-pivot sparse LU plan (xg_lu_export) as a
+        dict of numpy arrays."""
+        sz = np.zeros(6, np.int64)
+        check(lib().xg_lu_export(self.h, sz.ctypes.data, None, None), "lu export")
+        n, nnz, nc, nlev, nt, nops = (int(v) for v in sz)
+        names = ["prow", "qcol", "lu_pos", "ok", "lv_comp", "lv_ptr", "t_dst", "t_div",
+                 "t_op", "ops"]
+        lens = [n, n, nnz, nc, 3 * nc + 1, nlev, nt, nt, nt + 1, 2 * nops]
+        arrs = [np.zeros(max(k, 1), np.int32) for k in lens]
+        ptrs = (ctypes.c_void_p * 10)(*[a.ctypes.data for a in arrs])
+        voff = np.zeros(nc + 1, np.int64)
+        check(lib().xg_lu_export(self.h, sz.ctypes.data, ptrs, voff.ctypes.data), "lu export")
+        out = {k: a[:m] for k, a, m in zip(names, arrs, lens)}
+        out["ops"] = out["ops"].reshape(-1, 2)
+        out["voff"] = voff
+        return out
+
+    def lu_info(self):
+        """xg_lu_info as a dict."""
+        info = np.zeros(8, np.int64)
+        check(lib().xg_lu_info(self.h, info.ctypes.data), "lu info")
+        keys = ["fnnz", "factor_ops", "max_factor_levels", "max_solve_levels",
+                "static_fail", "global_components", "fallbacks", "tasks"]
+        return dict(zip(keys, (int(v) for v in info)))
+
     def mos1_offsets(self):
         off = np.zeros((max(self.n_mos1, 1), 22), np.int32)
         check(lib().xg_get_mos1_offsets(self.h, off.ctypes.data_as(_lib.c_int32_p)), "offsets")

@@ -6,7 +6,9 @@ N_TIA_OneStep.C:1945-2010), with a full Newton solve per step.
 Every Newton iteration runs on the device through the C ABI:
   xg_update_state + ordered reduction (xg_load_all, overwrite mode),
   xg_tia_assemble (N_TIA_OneStep::obtainResidual / obtainJacobian),
-  xg_dense_solve (LU per independent sub-circuit + update + norms),
+  xg_lu_solve (static-pivot sparse LU per independent sub-circuit, with
+  the partial-pivoting dense LU for components that fail the threshold
+  test; + update + norms),
   xg_rhs_norms (residual norm of the convergence test).
 The host only keeps the step sequence and reads one small stats array per
 iteration.
@@ -43,8 +45,14 @@ class NewtonFailure(RuntimeError):
 
 
 class Transient:
-    def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda"):
+    def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda",
+                 solver="lu"):
+        """solver: "lu" (static-pivot sparse LU with pivoting fallback) or
+        "dense" (dense partial-pivoting LU for every component)."""
         import torch
+        if solver not in ("lu", "dense"):
+            raise ValueError("solver must be 'lu' or 'dense'")
+        self.solver = solver
         self.torch = torch
         self.nl = nl.layout()
         self.dev = DeviceLoad(nl, gmin=gmin, limiter=limiter)
@@ -95,9 +103,10 @@ class Transient:
                                self.JF, self.JQ, self.rhs, self.J)]), "assemble")
 
     def _solve(self, prm, wa, wb):
-        check(lib().xg_dense_solve(self.dev.h, _ptr(self.J), _ptr(self.rhs), _ptr(self.x),
-                                   _ptr(wa), _ptr(wb), prm["reltol"], prm["abstol"],
-                                   _iptr(self.active), _ptr(self.stats)), "dense solve")
+        fn = lib().xg_lu_solve if self.solver == "lu" else lib().xg_dense_solve
+        check(fn(self.dev.h, _ptr(self.J), _ptr(self.rhs), _ptr(self.x),
+                 _ptr(wa), _ptr(wb), prm["reltol"], prm["abstol"],
+                 _iptr(self.active), _ptr(self.stats)), "linear solve")
 
     def _newton(self, prm, transient, order, dt, t, first_tran=False, dcop=False,
                 wa=None, wb=None):
