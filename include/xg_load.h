@@ -273,10 +273,11 @@ int xg_dense_solve(xg_handle *h, const double *J, const double *rhs, double *x,
                    double abstol, const int32_t *active, double *stats);
 
 /* The Newton linear solve per component with a static-pivot sparse LU
-   (replaces the KLU refactor-and-solve of N_LAS_AmesosSolver.C:231-470;
-   KLU's threshold rule, tol 0.001, is checked on every factorisation and a
-   component that fails it is refactored by the dense partial-pivoting LU in
-   the same call).  Arguments, update and stats as xg_dense_solve. */
+   (replaces the KLU refactor-and-solve of N_LAS_AmesosSolver.C:231-470).
+   Every solve is checked for backward stability (finite factors, nonzero
+   pivots, ||J dx - rhs||_inf <= 1e-12 (||J||_inf ||dx||_inf + ||rhs||_inf));
+   a component that fails is refactored by the dense partial-pivoting LU in
+   the same call.  Arguments, update and stats as xg_dense_solve. */
 int xg_lu_solve(xg_handle *h, const double *J, const double *rhs, double *x,
                 const double *wa, const double *wb, double reltol,
                 double abstol, const int32_t *active, double *stats);

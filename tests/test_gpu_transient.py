@@ -158,8 +158,8 @@ def _two_rings():
 def test_sparse_lu_against_numpy(mode):
     """xg_lu_solve solves per component like LAPACK.  "fallback": one
     component gets a tiny pivot under large off-diagonal entries, so its
-    static pivots fail the threshold test and the partial-pivoting LU takes
-    it over in the same call; the other stays on the sparse path."""
+    static pivots fail the backward-error test and the partial-pivoting LU
+    takes it over in the same call; the other stays on the sparse path."""
     import ctypes
     import torch
     from xyce_amd._lib import check, lib
@@ -178,12 +178,13 @@ def test_sparse_lu_against_numpy(mode):
     check(lib().xg_components(dev.h, ctypes.byref(nc), comp.ctypes.data), "components")
     assert nc.value == 2
     if mode == "fallback":
-        # component 0's first static pivot (nothing updates it): 1e-9, the
-        # rest of its column 5.0, so |l| = 5e9 fails the threshold test
+        # component 0's first static pivot (nothing updates it): 1e-14, the
+        # rest of its column 5.0: element growth ~5e14 ruins the static
+        # factorisation's backward error
         P = dev.lu_plan()
         r0, c0 = int(P["prow"][0]), int(P["qcol"][0])  # position 0 of component 0
         assert comp[r0] == 0
-        vals[(rows == r0) & (ci == c0)] = 1e-9
+        vals[(rows == r0) & (ci == c0)] = 1e-14
         col = (ci == c0) & (rows != r0) & (rows < n)
         assert col.any()
         vals[col] = 5.0
@@ -211,6 +212,47 @@ def test_sparse_lu_against_numpy(mode):
     st = stats.cpu().numpy().reshape(-1, 4)
     assert not st[:, 2].any()
     assert np.array_equal(st[:, 3], np.bincount(comp))
+
+
+def test_sparse_lu_global_memory_form():
+    """A component too large for LDS (an RC ladder of 6000 nodes) takes
+    the global-memory form of the sparse LU; checked against a sparse
+    direct solve."""
+    import ctypes
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd._lib import check, lib
+    from xyce_amd.loader import DeviceLoad, _ptr
+    nl = circuits.Netlist("long ladder")
+    nl.add_vsrc("V1", "n0", "0", 1.0)
+    for k in range(6000):
+        nl.add_res("R%d" % k, "n%d" % k, "n%d" % (k + 1), 100.0)
+        nl.add_cap("C%d" % k, "n%d" % (k + 1), "0", 1e-15)
+    nl.layout()
+    dev = DeviceLoad(nl)
+    info = dev.lu_info()
+    assert info["global_components"] == 1
+    rp, ci = dev.pattern()
+    n = nl.n_nodes
+    rng = np.random.default_rng(7)
+    vals = rng.uniform(-1, 1, len(ci))
+    rows = np.repeat(np.arange(n + 1), np.diff(rp))
+    vals[rows == ci] += 20.0
+    rhs = rng.uniform(-1, 1, n + 1)
+    rhs[n] = 0.0
+    cu = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda")
+    J, R, X, W = cu(vals), cu(rhs), cu(np.zeros(n + 1)), cu(np.ones(n + 1))
+    stats = cu(np.zeros(4))
+    check(lib().xg_lu_solve(dev.h, _ptr(J), _ptr(R), _ptr(X), _ptr(W), _ptr(W), 0.0, 1.0,
+                            None, _ptr(stats)), "solve")
+    assert dev.lu_info()["fallbacks"] == 0
+    m = (rows < n) & (ci < n)
+    A = sp.csr_matrix((vals[m], (rows[m], ci[m])), shape=(n, n))
+    ref = spla.spsolve(A.tocsc(), rhs[:n])
+    x = X.cpu().numpy()[:n]
+    assert np.allclose(x, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
 
 
 def test_sparse_and_dense_solvers_give_the_same_waveforms():

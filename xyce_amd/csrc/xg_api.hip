@@ -368,10 +368,7 @@ int xg_lu_info(xg_handle *H, int64_t *info) {
   const xg::LuPlan &L = h.plan.lu;
   int64_t n_global = 0;
   const int32_t nc = (int32_t)h.plan.comp_start.size() - 1;
-  for (int32_t k = 0; k < nc; ++k)
-    if (!xg::sparse_lu_fits_lds(h.plan.comp_start[k + 1] - h.plan.comp_start[k],
-                                L.voff[k + 1] - L.voff[k]))
-      n_global++;
+  for (int32_t k = 0; k < nc; ++k) n_global += L.in_lds[k] ? 0 : 1;
   info[0] = L.fnnz;
   info[1] = L.factor_ops;
   info[2] = L.max_factor_levels;
@@ -568,10 +565,8 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
     u.n_comp = d.n_comp;
     int64_t gwork = 0;
     std::vector<int64_t> voff(L.voff);
-    for (int32_t k = 0; k < u.n_comp; ++k) {
-      const int64_t m = P.comp_start[k + 1] - P.comp_start[k];
-      if (!xg::sparse_lu_fits_lds(m, L.voff[k + 1] - L.voff[k])) u.any_global = 1;
-    }
+    for (int32_t k = 0; k < u.n_comp; ++k)
+      if (!L.in_lds[k]) u.any_global = 1;
     if (u.any_global) gwork = L.voff[u.n_comp];
     u.cstart = cs;
     chk(u.prow = h.upload(L.prow));
@@ -581,16 +576,28 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
     chk(u.ok = h.upload(L.ok));
     chk(u.lv_comp = h.upload(L.lv_comp));
     chk(u.lv_ptr = h.upload(L.lv_ptr));
-    chk(u.t_dst = h.upload(L.t_dst));
-    chk(u.t_div = h.upload(L.t_div));
-    chk(u.t_op = h.upload(L.t_op));
+    chk(u.in_lds = h.upload(L.in_lds));
+    {
+      std::vector<int32_t> t4(4 * L.t_dst.size());
+      for (size_t k = 0; k < L.t_dst.size(); ++k) {
+        t4[4 * k] = L.t_dst[k];
+        t4[4 * k + 1] = L.t_div[k];
+        t4[4 * k + 2] = L.t_op[k];
+        t4[4 * k + 3] = L.t_op[k + 1];
+      }
+      int32_t *o;
+      chk(o = h.upload(t4));
+      u.tasks = (const int4 *)o;
+    }
     {
       int32_t *o;
       chk(o = h.upload(L.ops));
       u.ops = (const int2 *)o;
     }
     u.rowptr = h.d_rowptr;
-    u.inv_tau = 1.0 / 0.001;
+    u.colidx = h.d_colidx;
+    chk(u.qpos = h.upload(L.qpos));
+    u.max_berr = 1e-12;
     chk(u.val_work = h.alloc<double>((size_t)(gwork > 0 ? gwork : 1)));
     u.vec_work = d.bwork;
     chk(u.fallback = h.alloc<int32_t>((size_t)(u.n_comp > 0 ? u.n_comp : 1)));

@@ -190,14 +190,17 @@ struct SparseLuArgs {
   const int32_t *lu_pos;          // CSR entry -> position in its block (-1)
   const int64_t *voff;            // n_comp + 1 value offsets
   const int32_t *ok;              // static factorisation available
-  const int32_t *lv_comp, *lv_ptr, *t_dst, *t_div, *t_op;
+  const int32_t *in_lds;          // factors, vector and tasks fit in LDS
+  const int32_t *lv_comp, *lv_ptr;
+  const int4 *tasks;              // {dst, div, first op, end op}
   const int2 *ops;
-  const int32_t *rowptr;
+  const int32_t *rowptr, *colidx;
+  const int32_t *qpos;            // global unknown -> position (-1 ground)
   const double *J, *rhs;
   double *x;
   const double *wa, *wb;
   double reltol, abstol;
-  double inv_tau;                 // largest accepted |l_ik|
+  double max_berr;                // largest accepted normwise backward error
   const int32_t *active;
   double *val_work, *vec_work;    // factors / solution of global components
   double *stats;
@@ -209,7 +212,10 @@ hipError_t launch_dense_solve(const DenseSolveArgs &a, hipStream_t s);
 int dense_lds_max();
 constexpr int kDenseMax = 2048;  // largest component with a dense form
 hipError_t launch_sparse_lu(const SparseLuArgs &a, hipStream_t s);
-bool sparse_lu_fits_lds(int64_t n, int64_t fnnz);
+// LDS bytes a component needs for the in-LDS form, and the budget
+int64_t sparse_lu_lds_bytes(int64_t n, int64_t fnnz, int64_t tasks, int64_t ops,
+                            int64_t levels);
+int64_t sparse_lu_lds_budget();
 hipError_t launch_predict(int64_t n1, int32_t order, double beta1, double beta2,
                           const double *xh0, const double *xh1, const double *xh2,
                           double *out, hipStream_t s);

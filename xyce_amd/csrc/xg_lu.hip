@@ -11,10 +11,16 @@
 // is a dot product of earlier values, summed by one thread in ascending
 // order, so results do not depend on the launch or on other components.
 //
-// Each factorisation is checked against KLU's threshold pivoting rule
-// (klu defaults: tol = 0.001): every multiplier |l_ik| <= 1/tol, every
-// pivot nonzero and finite.  A component that fails is flagged; the dense
-// partial-pivoting kernel launched right after refactors just those.
+// Static pivots are accepted when the solve is backward stable: every
+// pivot nonzero and every factor finite, and the normwise backward error of
+// the solution, ||J dx - rhs|| / (||J|| ||dx|| + ||rhs||) (infinity norms,
+// the residual taken with the unfactored values), at most 1e-12 -- the
+// bound partial pivoting gives in practice (static pivoting with a
+// backward-error check, as in Li and Demmel's GESP for SuperLU_DIST).  A
+// component that fails is flagged, its iterate left as it was, and the
+// dense partial-pivoting kernel launched right after refactors just those.
+#include <cfloat>
+
 #include "xg_internal.h"
 
 namespace xg {
@@ -22,8 +28,7 @@ namespace xg {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kLuVal = 16384;  // 128 KB of factor values in LDS
-constexpr int kLuVec = 2048;   //  16 KB of solution vector in LDS
+constexpr int kLdsBytes = 144 * 1024;  // factors, vector and task lists
 
 __device__ __forceinline__ double blk_max(double v, double *sv) {
   const int t = threadIdx.x;
@@ -51,16 +56,57 @@ __device__ __forceinline__ double blk_sum(double v, double *sv) {
   return r;
 }
 
+__host__ __device__ constexpr int64_t align16(int64_t b) { return (b + 15) & ~(int64_t)15; }
+
+// The task lists of one component, with op ranges relative to `ops` and
+// level pointers relative to `tasks`.
+struct Lists {
+  const int4 *tasks;
+  const int2 *ops;
+  const int32_t *lvp;
+  int obase;  // subtracted from the ops fields of tasks (0 once staged)
+};
+
+template <int kPhase>
+__device__ __forceinline__ void run_level(const Lists &ls, int L, double *val,
+                                          double *vec, int &bad) {
+  const int k0 = ls.lvp[L], k1 = ls.lvp[L + 1];
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += kBlock) {
+    const int4 tk = ls.tasks[k];
+    const int o0 = tk.z - ls.obase, o1 = tk.w - ls.obase;
+    if (kPhase == 0) {
+      double s = val[tk.x];
+      for (int o = o0; o < o1; ++o) {
+        const int2 q = ls.ops[o];
+        s = s - val[q.x] * val[q.y];
+      }
+      if (tk.y >= 0) {
+        s = s / val[tk.y];
+        bad |= !(fabs(s) <= DBL_MAX);  // infinite or NaN
+      } else if (tk.y == -2) {
+        bad |= !(fabs(s) > 0.0) || isinf(s);
+      }
+      val[tk.x] = s;
+    } else {
+      double s = vec[tk.x];
+      for (int o = o0; o < o1; ++o) {
+        const int2 q = ls.ops[o];
+        s = s - val[q.x] * vec[q.y];
+      }
+      vec[tk.x] = kPhase == 2 ? s / val[tk.y] : s;
+    }
+  }
+}
+
 template <bool kLds>
 __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
-  __shared__ double lval[kLds ? kLuVal : 1];
-  __shared__ double lvec[kLds ? kLuVec : 1];
+  __shared__ __align__(16) char lds[kLds ? kLdsBytes : 16];
   __shared__ double sv[kBlock];
   const int c = blockIdx.x;
   const int t = threadIdx.x;
+  if ((a.in_lds[c] != 0) != kLds) return;  // the other form
   const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
   const int64_t nv = a.voff[c + 1] - a.voff[c];
-  if ((n <= kLuVec && nv <= kLuVal) != kLds) return;  // the other form
   double *stats = a.stats + 4 * (size_t)c;
   if (a.active && !a.active[c]) {
     if (t == 0) {
@@ -76,8 +122,35 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
     if (t == 0) a.fallback[c] = 1;
     return;
   }
-  double *val = kLds ? lval : a.val_work + a.voff[c];
-  double *vec = kLds ? lvec : a.vec_work + beg;
+  const int32_t *lv = a.lv_comp + 3 * (size_t)c;
+  const int L0 = lv[0], nlev = lv[3] - lv[0];
+  const int tbase = a.lv_ptr[L0], ntask = a.lv_ptr[lv[3]] - tbase;
+  const int obase = ntask ? a.tasks[tbase].z : 0;
+  const int nops = ntask ? a.tasks[tbase + ntask - 1].w - obase : 0;
+  double *val, *vec;
+  Lists ls;
+  if (kLds) {
+    // stage: [values][vector][tasks][ops][level pointers]
+    char *p = lds;
+    val = (double *)p;
+    p += align16(nv * 8);
+    vec = (double *)p;
+    p += align16((int64_t)n * 8);
+    int4 *tk = (int4 *)p;
+    p += (int64_t)ntask * 16;
+    int2 *op = (int2 *)p;
+    p += align16((int64_t)nops * 8);
+    int32_t *lvp = (int32_t *)p;
+    for (int k = t; k < ntask; k += kBlock) tk[k] = a.tasks[tbase + k];
+    for (int k = t; k < nops; k += kBlock) op[k] = a.ops[obase + k];
+    for (int k = t; k <= nlev; k += kBlock) lvp[k] = a.lv_ptr[L0 + k] - tbase;
+    ls = Lists{tk, op, lvp, obase};
+  } else {
+    val = a.val_work + a.voff[c];
+    vec = a.vec_work + beg;
+    // global lists: level pointers stay absolute, ops absolute
+    ls = Lists{a.tasks, a.ops, a.lv_ptr + L0, 0};
+  }
   const int32_t *prow = a.prow + beg, *qcol = a.qcol + beg;
   for (int64_t e = t; e < nv; e += kBlock) val[e] = 0.0;
   __syncthreads();
@@ -92,26 +165,12 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
     vec[i] = r;
     rmax = fmax(rmax, fabs(r));
   }
-  const double max_rhs = blk_max(rmax, sv);  // includes the barrier
+  const double max_rhs = blk_max(rmax, sv);  // includes the barriers
   // ---- factor
-  const int32_t *lv = a.lv_comp + 3 * (size_t)c;
+  const int f1 = lv[1] - L0, s1 = lv[2] - L0;
   int bad = 0;
-  for (int L = lv[0]; L < lv[1]; ++L) {
-    for (int k = a.lv_ptr[L] + t; k < a.lv_ptr[L + 1]; k += kBlock) {
-      const int d = a.t_dst[k], dv = a.t_div[k];
-      double s = val[d];
-      for (int o = a.t_op[k]; o < a.t_op[k + 1]; ++o) {
-        const int2 q = a.ops[o];
-        s = s - val[q.x] * val[q.y];
-      }
-      if (dv >= 0) {
-        s = s / val[dv];
-        bad |= !(fabs(s) <= a.inv_tau);  // also catches NaN
-      } else if (dv == -2) {
-        bad |= !(fabs(s) > 0.0) || isinf(s);
-      }
-      val[d] = s;
-    }
+  for (int L = 0; L < f1; ++L) {
+    run_level<0>(ls, L, val, vec, bad);
     __syncthreads();
   }
   if (__syncthreads_or(bad)) {
@@ -119,29 +178,36 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
     return;
   }
   // ---- L y = P b (unit diagonal), then U x' = y, in place
-  for (int L = lv[1]; L < lv[2]; ++L) {
-    for (int k = a.lv_ptr[L] + t; k < a.lv_ptr[L + 1]; k += kBlock) {
-      const int d = a.t_dst[k];
-      double s = vec[d];
-      for (int o = a.t_op[k]; o < a.t_op[k + 1]; ++o) {
-        const int2 q = a.ops[o];
-        s = s - val[q.x] * vec[q.y];
-      }
-      vec[d] = s;
-    }
+  for (int L = f1; L < s1; ++L) {
+    run_level<1>(ls, L, val, vec, bad);
     __syncthreads();
   }
-  for (int L = lv[2]; L < lv[3]; ++L) {
-    for (int k = a.lv_ptr[L] + t; k < a.lv_ptr[L + 1]; k += kBlock) {
-      const int d = a.t_dst[k];
-      double s = vec[d];
-      for (int o = a.t_op[k]; o < a.t_op[k + 1]; ++o) {
-        const int2 q = a.ops[o];
-        s = s - val[q.x] * vec[q.y];
-      }
-      vec[d] = s / val[a.t_div[k]];
-    }
+  for (int L = s1; L < nlev; ++L) {
+    run_level<2>(ls, L, val, vec, bad);
     __syncthreads();
+  }
+  // ---- backward error of the solution against the unfactored rows
+  double rm = 0.0, am = 0.0, xm = 0.0;
+  for (int i = t; i < n; i += kBlock) {
+    const int g = prow[i];
+    double r = a.rhs[g], as = 0.0;
+    for (int k = a.rowptr[g]; k < a.rowptr[g + 1]; ++k) {
+      const int p = a.qpos[a.colidx[k]];
+      if (p >= 0) {
+        r = r - a.J[k] * vec[p];
+        as = as + fabs(a.J[k]);
+      }
+    }
+    rm = fmax(rm, fabs(r));
+    am = fmax(am, as);
+    xm = fmax(xm, fabs(vec[i]));
+  }
+  rm = blk_max(rm, sv);
+  am = blk_max(am, sv);
+  xm = blk_max(xm, sv);
+  if (!(rm <= a.max_berr * (am * xm + max_rhs))) {  // also NaN / infinite
+    if (t == 0) a.fallback[c] = 1;
+    return;
   }
   // ---- Newton update x += dx and the weighted RMS norm of dx
   // (N_NLS_DampedNewton.C:311-335)
@@ -166,9 +232,13 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
 
 }  // namespace
 
-bool sparse_lu_fits_lds(int64_t n, int64_t fnnz) {
-  return n <= kLuVec && fnnz <= kLuVal;
+int64_t sparse_lu_lds_bytes(int64_t n, int64_t fnnz, int64_t tasks, int64_t ops,
+                            int64_t levels) {
+  return align16(fnnz * 8) + align16(n * 8) + tasks * 16 + align16(ops * 8) +
+         (levels + 1) * 4;
 }
+
+int64_t sparse_lu_lds_budget() { return kLdsBytes; }
 
 hipError_t launch_sparse_lu(const SparseLuArgs &a, hipStream_t s) {
   if (a.n_comp == 0) return hipSuccess;

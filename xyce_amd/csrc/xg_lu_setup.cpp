@@ -22,9 +22,9 @@
 //     terms in ascending order), grouped into levels of independent tasks;
 //     the two triangular solves likewise, one task per row.
 //
-// The device checks every factorisation against KLU's threshold rule
-// (|l_ik| <= 1 / 0.001, nonzero finite pivots) and refactors a component
-// that fails with the dense partial-pivoting kernel.
+// The device checks every solve's backward error (nonzero finite pivots,
+// ||J dx - rhs|| <= 1e-12 (||J|| ||dx|| + ||rhs||)) and refactors a
+// component that fails with the dense partial-pivoting kernel.
 //
 // Everything here uses component-local indices only, so a sub-circuit gets
 // the same factorisation alone and inside a batch.
@@ -217,9 +217,11 @@ int lu_analyse(Plan &plan, std::string &err) {
   L = LuPlan{};
   L.prow.assign(n_all, 0);
   L.qcol.assign(n_all, 0);
+  L.qpos.assign(n_all + 1, -1);
   L.lu_pos.assign(plan.colidx.size(), -1);
   L.voff.assign(nc + 1, 0);
   L.ok.assign(nc, 0);
+  L.in_lds.assign(nc, 1);
   L.lv_comp.assign((size_t)3 * nc + 1, 0);
   L.lv_ptr.assign(1, 0);
   L.t_op.assign(1, 0);
@@ -272,6 +274,7 @@ int lu_analyse(Plan &plan, std::string &err) {
       for (int32_t i = 0; i < n; ++i) {
         L.prow[beg + i] = nodes[i];
         L.qcol[beg + i] = nodes[i];
+        L.qpos[nodes[i]] = i;
       }
       continue;
     }
@@ -310,6 +313,7 @@ int lu_analyse(Plan &plan, std::string &err) {
     for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
     for (int32_t i = 0; i < n; ++i) {
       L.qcol[beg + i] = nodes[order[i]];
+      L.qpos[nodes[order[i]]] = i;
       L.prow[beg + i] = nodes[row_of_col[order[i]]];
     }
     // elimination tree and row structure of L (Liu's row-subtree walk)
@@ -459,9 +463,13 @@ int lu_analyse(Plan &plan, std::string &err) {
       L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
       return nlev;
     };
+    const int64_t t0 = (int64_t)L.t_dst.size(), o0 = (int64_t)L.ops.size() / 2;
     const int f = emit(tasks, 0);
     const int a = emit(fwd, 1);
     const int b = emit(bwd, 2);
+    L.in_lds[c] = sparse_lu_lds_bytes(n, fn, (int64_t)L.t_dst.size() - t0,
+                                      (int64_t)L.ops.size() / 2 - o0, f + a + b) <=
+                  sparse_lu_lds_budget();
     L.max_factor_levels = std::max(L.max_factor_levels, f);
     L.max_solve_levels = std::max(L.max_solve_levels, a + b);
     if (L.ops.size() / 2 > (size_t)INT32_MAX || L.t_dst.size() > (size_t)INT32_MAX) {

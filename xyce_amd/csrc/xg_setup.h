@@ -104,9 +104,12 @@ struct Circuit {
 // thread in one fixed order.
 struct LuPlan {
   std::vector<int32_t> prow, qcol;    // by comp_start offsets
+  std::vector<int32_t> qpos;          // unknown -> position in its component
+                                      // (n_nodes + 1 entries, ground -1)
   std::vector<int32_t> lu_pos;        // CSR entry -> position in its block, -1
   std::vector<int64_t> voff;          // n_comp + 1 value offsets
   std::vector<int32_t> ok;            // 1: matching found (static factorisation)
+  std::vector<int32_t> in_lds;        // 1: factors, vector and tasks fit LDS
   std::vector<int32_t> lv_comp;       // 3 n_comp + 1: phase -> first level
   std::vector<int32_t> lv_ptr;        // level -> first task (n_levels + 1)
   std::vector<int32_t> t_dst, t_div;  // t_div: -1 none, -2 pivot (check only)
