@@ -41,6 +41,14 @@ HBM_PEAK_GBS = 8000.0
 B3_EVAL_TRAFFIC_BYTES = 38.81e6
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print("[bench %.0fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
+
+
+T_START = time.perf_counter()
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +60,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches only (for counter collection)")
+    ap.add_argument("--mc-samples", type=int, default=256,
+                    help="Monte-Carlo ring samples in the batched-transient leg")
+    ap.add_argument("--mc-plain", type=int, default=16,
+                    help="samples (evenly spaced) timed one circuit per launch")
+    ap.add_argument("--mc-tstop", type=float, default=0.5e-9)
+    ap.add_argument("--mc-dt", type=float, default=1e-12)
+    ap.add_argument("--no-mc", action="store_true", help="skip the batched-transient leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 512 MiB device copy, the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -84,6 +99,46 @@ def graph_timing(torch, dev, stream, step, timed, args):
     return timed(graph.replay, args.steps)
 
 
+def batched_transient_leg(torch, args):
+    """Monte-Carlo BSIM3 ring oscillators (stage counts U{11..301}, per-device
+    VTH0 += N(0, 30 mV), TOX *= U[0.98, 1.02], seed 3): every sample's full
+    transient (DC operating point, then fixed steps) in one batch, against
+    the same samples one circuit per launch.  Setup (netlist analysis and
+    upload) is outside both timings; state stays resident on the device."""
+    from xyce_amd import circuits
+    from xyce_amd.montecarlo import run_batched, run_plain
+    parts = circuits.monte_carlo_rings(args.mc_samples, seed=3)
+    probes = ["out0"]
+    sync = torch.cuda.synchronize
+    progress("batched transient: %d samples" % len(parts))
+    b = run_batched(parts, args.mc_tstop, args.mc_dt, probes, every=10, sync=sync)
+    progress("batched transient: %.3f s; plain form on %d samples"
+             % (b["seconds"], args.mc_plain))
+    pick = np.unique(np.linspace(0, len(parts) - 1, args.mc_plain).round().astype(int))
+    p = run_plain([parts[k] for k in pick], args.mc_tstop, args.mc_dt, probes, every=10,
+                  sync=sync)
+    progress("plain form: %.3f s" % float(p["seconds"].sum()))
+    diff = float(np.abs(p["waves"] - b["waves"][pick]).max())
+    plain_s = p["seconds"]
+    return {
+        "workload": "%d Monte-Carlo BSIM3 ring oscillators, 11-301 stages, %d FETs; "
+                    ".TRAN %.3g s at %.3g s steps after the DC operating point"
+                    % (len(parts), sum(len(q.mos) for q in parts), args.mc_tstop, args.mc_dt),
+        "solves_per_s": len(parts) / b["seconds"],
+        "plain_solves_per_s": len(pick) / float(plain_s.sum()),
+        "speedup_vs_plain": (len(parts) / b["seconds"]) / (len(pick) / float(plain_s.sum())),
+        "batch_latency_s": b["seconds"],
+        "plain_latency_p50_s": float(np.percentile(plain_s, 50)),
+        "plain_latency_p99_s": float(np.percentile(plain_s, 99)),
+        "plain_samples_timed": int(len(pick)),
+        "batch_newton_iterations": int(b["newton"]),
+        "plain_newton_iterations_mean": float(p["newton"].mean()),
+        "batch_step_failures": int(b["step_failures"]),
+        "batch_setup_s": b["setup_s"],
+        "max_waveform_diff_batch_vs_plain_V": diff,
+    }
+
+
 def cpu_baseline(nl, x, sta, sto, budget_s):
     """The oracle's restatement of the reference Master loops (array of
     structs, pointer read-modify-write stamping), serial on one core."""
@@ -113,10 +168,12 @@ def main():
               "graph partition across GPUs, which this build does not implement "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    progress("importing torch")
     import torch
     from xyce_amd import circuits
     from xyce_amd.loader import DeviceLoad
 
+    progress("building the %d-gate netlist" % args.gates)
     nl = circuits.random_logic_bsim3(n_gates=args.gates, seed=2)
     nl.layout()
     n_b3 = len(nl.bsim3)
@@ -154,6 +211,7 @@ def main():
         out.zero_()
         dev.load_all(X, SC, TC, SN, TN, F, Q, FL, QL, JF, JQ)
 
+    progress("device load: warmup and timed steps")
     dev.set_output_mode(False)
     for _ in range(args.warmup):
         step_accumulate()
@@ -208,7 +266,7 @@ def main():
 
     value = n_b3 / (ms_step * 1e-3)
     res = {
-        "metric": "BSIM3 instance-evals/s",
+        "metric": "BSIM3 instance-evals/s; batched transient solves/s + p99 latency vs plain",
         "value": value,
         "unit": "instance-evals/s",
         "n_gpus": 1,
@@ -237,7 +295,11 @@ def main():
         "ms_per_step_zero_then_accumulate": ms_accumulate,
         "reduction_ms": reduce_ms,
     }
+    progress("device load: %.4f ms per step" % ms_step)
+    if not args.no_mc:
+        res["batched_transient"] = batched_transient_leg(torch, args)
     if not args.no_cpu:
+        progress("CPU baseline (%.0f s sample)" % args.cpu_seconds)
         sta_h, sto_h = sta.copy(), sto.copy()
         res["cpu_baseline"] = cpu_baseline(nl, x, sta_h, sto_h, args.cpu_seconds)
     print(json.dumps(res), flush=True)

@@ -465,7 +465,7 @@ def merge(parts, prefixes=None):
     """One netlist holding several independent circuits (they share only
     ground): node, model and device names get a per-part prefix.  The
     device load and the Newton solve then run every part in the same
-    launches, one dense solve per part."""
+    launches, one LU per part."""
     prefixes = prefixes or ["s%d_" % k for k in range(len(parts))]
     nl = Netlist("batch of %d circuits" % len(parts))
     for src, pre in zip(parts, prefixes):

@@ -41,6 +41,11 @@ namespace xg {
 
 namespace {
 
+// Per-component limits of the static factorisation: strictly-lower fill
+// entries and multiply-adds per factorisation.
+constexpr int64_t kLuMaxFill = 4 << 20;
+constexpr int64_t kLuMaxOps = 64 << 20;
+
 using Graph = std::vector<std::vector<int32_t>>;
 
 // Breadth-first levels of the vertices with in[v] == tag, from root.
@@ -316,10 +321,14 @@ int lu_analyse(Plan &plan, std::string &err) {
       L.qpos[nodes[order[i]]] = i;
       L.prow[beg + i] = nodes[row_of_col[order[i]]];
     }
-    // elimination tree and row structure of L (Liu's row-subtree walk)
+    // elimination tree and row structure of L (Liu's row-subtree walk),
+    // abandoned when the fill or the factorisation work exceeds a budget
+    // (the component then has no static factorisation)
     std::vector<std::vector<int32_t>> lrow(n), ucol(n);
     std::vector<int32_t> parent(n, -1), mark(n, -1);
-    for (int32_t i = 0; i < n; ++i) {
+    int64_t fill = 0;
+    bool too_big = false;
+    for (int32_t i = 0; i < n && !too_big; ++i) {
       mark[i] = i;
       for (int32_t u : sym[order[i]]) {
         int32_t j = pinv[u];
@@ -336,6 +345,21 @@ int lu_analyse(Plan &plan, std::string &err) {
       }
       std::sort(lrow[i].begin(), lrow[i].end());
       for (int32_t j : lrow[i]) ucol[j].push_back(i);
+      fill += (int64_t)lrow[i].size();
+      too_big = fill > kLuMaxFill;
+    }
+    if (!too_big) {
+      int64_t work = 0;  // multiply-adds: sum over pivots of (column count)^2
+      for (int32_t k = 0; k < n; ++k) work += (int64_t)ucol[k].size() * (int64_t)ucol[k].size();
+      too_big = work > kLuMaxOps;
+    }
+    if (too_big) {
+      L.n_static_fail++;
+      for (int ph = 0; ph < 3; ++ph)
+        L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
+      L.voff[c + 1] = L.voff[c];
+      L.in_lds[c] = 0;
+      continue;
     }
     // filled row-major pattern: row i = lrow[i], i, ucol[i] (all sorted)
     std::vector<int64_t> rp(n + 1, 0);
