@@ -300,6 +300,15 @@ int xg_lu_info(xg_handle *h, int64_t info[8]);
 int xg_lu_export(const xg_handle *h, int64_t sizes[6], int32_t *const *arrays,
                  int64_t *voff);
 
+/* Step parameters from device memory (for graph-captured step loops, whose
+   kernel arguments are fixed at capture): when dstep is not NULL, the
+   linear sources read time = dstep[0] and breakpoint tolerance = dstep[1],
+   and transient assembly reads 1/dt = dstep[2] and order = dstep[3], at
+   kernel run time, instead of the values of xg_set_time / xg_tia_assemble.
+   NULL restores those.  No counterpart in the reference (its time
+   integrator passes them through N_TIA_TimeIntInfo on the host). */
+int xg_set_step_source(xg_handle *h, const double *dstep);
+
 /* ONESTEP predictor (N_TIA_OneStep.C:115-133): xpred = xh0 + beta1 xh1
    (+ beta2 xh2 at order 2), with xh1 = x_n - x_{n-1}, xh2 = x_{n-1} -
    x_{n-2}; beta from updateCoeffs (N_TIA_OneStep.C:1380-1405). */
@@ -324,6 +333,57 @@ int xg_set_time(xg_handle *h, double time, double bp_tol);
 /* Sizes the library was finalized with. */
 int xg_sizes(const xg_handle *h, int32_t *n_nodes, int32_t *n_state,
              int32_t *n_store, int32_t *n_mos1, int32_t *n_bsim3);
+
+/* ---- Transient driver: the DC operating point and ONESTEP steps with the
+   whole Newton loop on the device (replaces the host-side loop of
+   N_NLS_DampedNewton::solve, N_NLS_DampedNewton.C:400-700, and the step
+   sequence of N_TIA_OneStep with fixed steps; no LTE step control).  Every
+   iteration is one hipGraph launch (xg_lu_solve, xg_load_all,
+   xg_tia_assemble, xg_rhs_norms, xg_component_limited, the non-finite
+   count and the convergence test of N_NLS_DampedNewton.C:1787-1830 per
+   independent sub-circuit); the host reads four control words per
+   iteration and halves a step whose Newton solve fails.  Parameters:
+   DC reltol 1e-3, abstol 1e-12, RHSTol 1e-6, deltaXTol 1, 200 iterations;
+   transient 1e-2, 1e-6, 1e-2, 0.33, 20 (N_NLS_NLParams.C:108-125).
+
+   The caller owns every vector (device pointers, as N_TIA_DataStore owns
+   the reference's): lengths n1 = n_nodes + 1, nnz CSR values, n_comp
+   components, n_state / n_store. */
+typedef struct xg_tran xg_tran;
+typedef struct xg_tran_buffers {
+  double *x, *x_curr, *xh1, *xh2, *xpred;  /* n1: iterate, accepted, history */
+  double *sta_c, *sta_n, *sto_c, *sto_n;   /* state / store, current / next */
+  double *F, *Q, *FL, *QL;                 /* n1 each: loaded vectors       */
+  double *JF, *JQ, *J;                     /* nnz: dF/dx, dQ/dx, Jacobian   */
+  double *rhs, *qn0, *fprev;               /* n1                            */
+  double *stats, *nstats;                  /* 4 n_comp: solve / rhs norms   */
+  int32_t *lim, *active, *conv;            /* n_comp                        */
+  int64_t *counts;                         /* 128: non-finite count partials*/
+  int32_t *ctl;                            /* 8 control words               */
+  double *step;                            /* 4: time, bp_tol, 1/dt, order  */
+} xg_tran_buffers;
+
+/* Binds a driver to a finalized handle (the handle's stream and step
+   source are the driver's until xg_tran_destroy).  bp_tol: PULSE corner
+   tolerance. */
+int xg_tran_create(xg_handle *h, const xg_tran_buffers *b, double bp_tol,
+                   xg_tran **out);
+/* DC operating point at t = 0 (N_ANP_Transient's tranop); on success the
+   history starts there.  > 0: Newton failure (1 singular Jacobian,
+   2 non-finite device values, 3 no convergence). */
+int xg_tran_dcop(xg_tran *t);
+/* One step of dt (order 1 for the first two steps, then 2); a step whose
+   Newton solve fails is replaced by two of dt/2, recursively. */
+int xg_tran_step(xg_tran *t, double dt);
+/* out[k] = x_curr[ids[k]] on the driver's stream (device arrays). */
+int xg_tran_probe(xg_tran *t, const int32_t *ids, int32_t n, double *out);
+/* Time of the accepted solution; counters[0..3] = steps, Newton
+   iterations, failed steps, last failure word. */
+int xg_tran_info(xg_tran *t, double *time, int64_t counters[4]);
+/* 0: launch every iteration's kernels one by one instead of a graph. */
+int xg_tran_set_graphs(xg_tran *t, int32_t on);
+int xg_tran_sync(xg_tran *t);
+int xg_tran_destroy(xg_tran *t);
 
 #ifdef __cplusplus
 }

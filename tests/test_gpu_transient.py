@@ -271,3 +271,29 @@ def test_sparse_and_dense_solvers_give_the_same_waveforms():
           % (err, a.newton_total, b.newton_total, a.dev.lu_info()["fallbacks"]), flush=True)
     assert np.array_equal(ta, tb)
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_native_driver_equals_python_driver(graphs):
+    """The library's device-side Newton driver (with and without hipGraph
+    replay) and the Python driver run the same kernels in the same order:
+    the waveforms, the time grid and the Newton iteration counts agree bit
+    for bit, on a ring and on a batch of Monte-Carlo samples."""
+    from xyce_amd import circuits
+    from xyce_amd.transient import Transient
+    cases = [(lambda: circuits.ring_oscillator_mos1(11, seed=1, ramp=50e-12),
+              ["out0", "out5", "vdd"]),
+             (lambda: circuits.merge(circuits.monte_carlo_rings(3, seed=9, stages=(11, 31))),
+              ["s0_out0", "s1_out2", "s2_out1"])]
+    for mk, probes in cases:
+        a = Transient(mk(), driver="native", graphs=graphs)
+        ta, wa = a.run(0.2e-9, 1e-12, probes, every=4)
+        b = Transient(mk(), driver="python")
+        tb, wb = b.run(0.2e-9, 1e-12, probes, every=4)
+        print("native (graphs %s) %d iterations, python %d"
+              % (graphs, a.newton_total, b.newton_total), flush=True)
+        assert np.array_equal(ta, tb)
+        assert wa.tobytes() == wb.tobytes()
+        assert a.newton_total == b.newton_total
+        assert a.step_failures == b.step_failures
+        a.close()

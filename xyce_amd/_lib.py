@@ -83,6 +83,17 @@ SIGNATURES = [
      [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_lu_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_lu_export", ctypes.c_int, [ctypes.c_void_p] * 4),
+    ("xg_set_step_source", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_tran_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                      ctypes.c_void_p]),
+    ("xg_tran_dcop", ctypes.c_int, [ctypes.c_void_p]),
+    ("xg_tran_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("xg_tran_probe", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                     ctypes.c_void_p]),
+    ("xg_tran_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_tran_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("xg_tran_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("xg_tran_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_set_eval_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_set_output_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),

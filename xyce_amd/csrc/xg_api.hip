@@ -35,6 +35,7 @@ struct Handle {
   // linear devices and the time their sources are evaluated at
   LinArgs lin{};
   double time = 0.0, bp_tol = 0.0;
+  const double *dstep = nullptr;  // device {time, bp_tol, 1/dt, order}
   // per-component dense solve, and the static-pivot sparse LU
   DenseSolveArgs ds{};
   SparseLuArgs lu{};
@@ -277,8 +278,9 @@ int xg_tia_assemble(xg_handle *H, int32_t transient, int32_t order, double dt,
   if (H && H->h.plan_only)
     return fail(XG_E_STATE, "handle was finalized without a device");
   if (!H || !F || !dFdxdVp || !dFdx || !rhs || !J ||
-      (transient && (!qn0 || !Q || !dQdxdVp || !dQdx || !(dt > 0.0) ||
-                     (order == 2 && !fprev))) ||
+      (transient && (!qn0 || !Q || !dQdxdVp || !dQdx ||
+                     (!(dt > 0.0) && !H->h.dstep) ||
+                     ((order == 2 || H->h.dstep) && !fprev))) ||
       (order != 1 && order != 2))
     return fail(XG_E_ARG, "xg_tia_assemble: bad argument");
   xg::Handle &h = H->h;
@@ -289,6 +291,7 @@ int xg_tia_assemble(xg_handle *H, int32_t transient, int32_t order, double dt,
   a.order = order;
   a.limiter = h.plan.limiter;
   a.inv_dt = transient ? 1.0 / dt : 0.0;
+  a.dstep = h.dstep;
   a.qn0 = qn0;
   a.fprev = fprev ? fprev : F;
   a.F = F;
@@ -447,6 +450,33 @@ int xg_rhs_norms(xg_handle *H, const double *rhs, double *stats) {
   a.active = nullptr;
   hipError_t e = xg::launch_dense_solve(a, h.stream);
   return e == hipSuccess ? XG_OK : hip_fail(e, "rhs norms");
+}
+
+}  // extern "C"
+
+namespace xg {
+hipError_t handle_count_flags(xg_handle *H, int64_t *out128) {
+  Handle &h = H->h;
+  return launch_count_flags(h.limited_all, h.n_limited_slots, out128, h.stream);
+}
+hipStream_t handle_stream(const xg_handle *H) { return H->h.stream; }
+int32_t handle_n_comp(const xg_handle *H) {
+  return (int32_t)H->h.plan.comp_start.size() - 1;
+}
+int64_t handle_n_nodes(const xg_handle *H) { return H->h.plan.n_nodes; }
+void handle_sizes(const xg_handle *H, int64_t *n_state, int64_t *n_store, int64_t *nnz) {
+  *n_state = H->h.plan.n_state;
+  *n_store = H->h.plan.n_store;
+  *nnz = (int64_t)H->h.plan.colidx.size();
+}
+}  // namespace xg
+
+extern "C" {
+
+int xg_set_step_source(xg_handle *H, const double *dstep) {
+  if (!H) return fail(XG_E_ARG, "xg_set_step_source: bad argument");
+  H->h.dstep = dstep;
+  return XG_OK;
 }
 
 int xg_set_time(xg_handle *H, double time, double bp_tol) {
@@ -763,7 +793,7 @@ int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
   if (e != hipSuccess) return hip_fail(e, "mos1 eval");
   e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream);
   if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
-  e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.stream);
+  e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.dstep, h.stream);
   if (e != hipSuccess) return hip_fail(e, "linear eval");
   return XG_OK;
 }

@@ -157,6 +157,7 @@ struct TiaAssembleArgs {
   int64_t n1 = 0, nnz = 0;
   int32_t transient = 0, order = 1, limiter = 1;
   double inv_dt = 0.0;
+  const double *dstep = nullptr;  // if set: 1/dt = dstep[2], order = dstep[3]
   const double *qn0, *fprev, *F, *Q, *FL, *QL, *dFdx, *dQdx;
   double *rhs, *J;
 };
@@ -222,11 +223,18 @@ hipError_t launch_predict(int64_t n1, int32_t order, double beta1, double beta2,
 
 hipError_t launch_linear_eval(const LinArgs &a, const SolverStateDev &ss,
                               const LoadPtrs &p, double time, double bp_tol,
-                              hipStream_t s);
+                              const double *dstep, hipStream_t s);
 hipError_t launch_component_limited(const int32_t *flags, const int32_t *slot_comp,
                                     int32_t n, int32_t *cnt, int32_t n_comp,
                                     hipStream_t s);
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s);
+
+// Handle internals for the transient driver (xg_tran.hip).
+hipError_t handle_count_flags(xg_handle *h, int64_t *out128);
+hipStream_t handle_stream(const xg_handle *h);
+int32_t handle_n_comp(const xg_handle *h);
+int64_t handle_n_nodes(const xg_handle *h);
+void handle_sizes(const xg_handle *h, int64_t *n_state, int64_t *n_store, int64_t *nnz);
 
 }  // namespace xg

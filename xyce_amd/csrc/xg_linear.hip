@@ -35,8 +35,12 @@ __device__ __forceinline__ double pulse_value(const double *pl, size_t st,
 
 __global__ void __launch_bounds__(256)
     linear_eval_kernel(LinArgs a, SolverStateDev ss, LoadPtrs p, double time,
-                       double bp_tol) {
+                       double bp_tol, const double *dstep) {
   const int s = blockIdx.x * 256 + threadIdx.x;
+  if (dstep) {  // time and breakpoint tolerance from device memory
+    time = dstep[0];
+    bp_tol = dstep[1];
+  }
   const int S = a.n_slots;
   if (s >= S) return;
   const int32_t type = a.type[s];
@@ -82,10 +86,10 @@ __global__ void __launch_bounds__(256)
 
 hipError_t launch_linear_eval(const LinArgs &a, const SolverStateDev &ss,
                               const LoadPtrs &p, double time, double bp_tol,
-                              hipStream_t s) {
+                              const double *dstep, hipStream_t s) {
   if (a.n_slots == 0) return hipSuccess;
   hipLaunchKernelGGL(linear_eval_kernel, dim3((a.n_slots + 255) / 256),
-                     dim3(256), 0, s, a, ss, p, time, bp_tol);
+                     dim3(256), 0, s, a, ss, p, time, bp_tol, dstep);
   return hipGetLastError();
 }
 

@@ -22,7 +22,11 @@ constexpr int kLdsN = 128;  // 128 x 128 doubles = 128 KB of the 160 KB LDS
 __global__ void __launch_bounds__(kBlock)
     assemble_kernel(TiaAssembleArgs a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const double inv_dt = a.inv_dt;
+  double inv_dt = a.inv_dt;
+  if (a.dstep && a.transient) {  // step size and order from device memory
+    inv_dt = a.dstep[2];
+    a.order = (int32_t)a.dstep[3];
+  }
   if (i < a.n1) {
     double r;
     if (a.transient) {

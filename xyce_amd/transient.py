@@ -3,6 +3,14 @@ ONESTEP integration (backward Euler for the first two steps, trapezoidal
 after, as N_TIA_OneStep::completeStep raises the order,
 N_TIA_OneStep.C:1945-2010), with a full Newton solve per step.
 
+Two drivers run the same kernels in the same order:
+  "native"  the library's xg_tran_* driver: each Newton iteration is one
+            hipGraph launch ending in the convergence test on the device;
+            the host reads four control words per iteration;
+  "python"  the loop below, one C-ABI call per kernel group, the
+            convergence test on the host.
+Both give bit-identical results.
+
 Every Newton iteration runs on the device through the C ABI:
   xg_update_state + ordered reduction (xg_load_all, overwrite mode),
   xg_tia_assemble (N_TIA_OneStep::obtainResidual / obtainJacobian),
@@ -44,15 +52,33 @@ class NewtonFailure(RuntimeError):
     pass
 
 
+class _TranBuffers(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "x", "x_curr", "xh1", "xh2", "xpred", "sta_c", "sta_n", "sto_c", "sto_n",
+        "F", "Q", "FL", "QL", "JF", "JQ", "J", "rhs", "qn0", "fprev", "stats", "nstats",
+        "lim", "active", "conv", "counts", "ctl", "step")]
+
+
+_FAIL_WORDS = {1: "singular Jacobian", 2: "non-finite device values", 3: "no convergence"}
+
+
 class Transient:
     def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda",
-                 solver="lu"):
+                 solver="lu", driver=None, graphs=True):
         """solver: "lu" (static-pivot sparse LU with pivoting fallback) or
-        "dense" (dense partial-pivoting LU for every component)."""
+        "dense" (dense partial-pivoting LU for every component; python
+        driver only).  driver: "native" (default with "lu") or "python"."""
         import torch
         if solver not in ("lu", "dense"):
             raise ValueError("solver must be 'lu' or 'dense'")
+        if driver is None:
+            driver = "native" if solver == "lu" else "python"
+        if driver not in ("native", "python"):
+            raise ValueError("driver must be 'native' or 'python'")
+        if driver == "native" and solver != "lu":
+            raise ValueError("the native driver solves with the sparse LU")
         self.solver = solver
+        self.driver = driver
         self.torch = torch
         self.nl = nl.layout()
         self.dev = DeviceLoad(nl, gmin=gmin, limiter=limiter)
@@ -85,6 +111,48 @@ class Transient:
         self.newton_total = 0
         self.step_failures = 0
         self.limiter = limiter
+        self._tr = None
+        if driver == "native":
+            self.conv = torch.zeros(self.n_comp, dtype=torch.int32, device=device)
+            self.counts = torch.zeros(128, dtype=torch.int64, device=device)
+            self.ctl = torch.zeros(8, dtype=torch.int32, device=device)
+            self.stepv = torch.zeros(4, dtype=torch.float64, device=device)
+            b = _TranBuffers(*[t.data_ptr() if t.numel() else None for t in (
+                self.x, self.x_curr, self.xh1, self.xh2, self.xpred, self.sta_c, self.sta_n,
+                self.sto_c, self.sto_n, self.F, self.Q, self.FL, self.QL, self.JF, self.JQ,
+                self.J, self.rhs, self.qn0, self.fprev, self.stats, self.nstats, self.lim,
+                self.active, self.conv, self.counts, self.ctl, self.stepv)])
+            torch.cuda.synchronize()
+            tr = ctypes.c_void_p()
+            check(lib().xg_tran_create(self.dev.h, ctypes.byref(b), bp_tol or 0.0,
+                                       ctypes.byref(tr)), "transient driver")
+            self._tr = tr
+            check(lib().xg_tran_set_graphs(tr, 1 if graphs else 0), "graphs")
+
+    def close(self):
+        if self._tr is not None:
+            lib().xg_tran_destroy(self._tr)
+            self._tr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _native(self, rc, what):
+        if rc > 0:
+            self._sync_counters()
+            raise NewtonFailure("%s: %s at t=%g" % (what, _FAIL_WORDS.get(rc, rc), self.t))
+        check(rc, what)
+        self._sync_counters()
+
+    def _sync_counters(self):
+        t = ctypes.c_double()
+        c = (ctypes.c_int64 * 4)()
+        check(lib().xg_tran_info(self._tr, ctypes.byref(t), c), "driver info")
+        self.t, self.n_steps, self.newton_total, self.step_failures = \
+            t.value, c[0], c[1], c[2]
 
     # ---- one device load + assembly
     def _load(self, it, transient, first_tran=False, dcop=False, init_jct=False, t=0.0):
@@ -159,6 +227,9 @@ class Transient:
     # ---- analyses
     def dcop(self):
         """DC operating point at t = 0 (the tranop before a transient)."""
+        if self._tr is not None:
+            self._native(lib().xg_tran_dcop(self._tr), "DC operating point")
+            return
         self.x.zero_()
         self.sta_c.zero_()
         self.sto_c.zero_()
@@ -177,6 +248,9 @@ class Transient:
 
     def step(self, dt):
         """One ONESTEP time step of size dt (retried with dt/2 on failure)."""
+        if self._tr is not None:
+            self._native(lib().xg_tran_step(self._tr, dt), "time step")
+            return
         order = 2 if self.n_steps >= 2 else 1
         t_next = self.t + dt
         # predictor xn0 = x_n + beta1 (x_n - x_{n-1}) [+ beta2 (x_{n-1} - x_{n-2})],
@@ -213,6 +287,8 @@ class Transient:
         """DC operating point, then steps of dt to tstop.  Returns (times,
         waveforms[probe, time]) for the given node names."""
         torch = self.torch
+        if self._tr is not None:
+            return self._run_native(tstop, dt, probes, every)
         idx = torch.tensor([self.nl.node(p) for p in probes], device=self.x.device)
         self.dcop()
         n = int(round(tstop / dt))
@@ -228,3 +304,32 @@ class Transient:
                 times.append(self.t)
                 k += 1
         return np.array(times), out[:, :k].cpu().numpy()
+
+    def _run_native(self, tstop, dt, probes, every):
+        torch = self.torch
+        L = lib()
+        ids = torch.tensor([self.nl.node(p) for p in probes], dtype=torch.int32,
+                           device=self.x.device)
+        n = int(round(tstop / dt))
+        out = torch.zeros(n // every + 1, len(probes), dtype=torch.float64,
+                          device=self.x.device)
+        torch.cuda.synchronize()
+        self.dcop()
+        check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes), out[0].data_ptr()),
+              "probe")
+        times = [0.0]
+        tnow = ctypes.c_double()
+        k = 1
+        for s in range(1, n + 1):
+            rc = L.xg_tran_step(self._tr, dt)
+            if rc:
+                self._native(rc, "time step")
+            if s % every == 0:
+                check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes),
+                                      out[k].data_ptr()), "probe")
+                check(L.xg_tran_info(self._tr, ctypes.byref(tnow), None), "driver info")
+                times.append(tnow.value)
+                k += 1
+        check(L.xg_tran_sync(self._tr), "sync")
+        self._sync_counters()
+        return np.array(times), out[:k].T.cpu().numpy()
