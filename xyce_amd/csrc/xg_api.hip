@@ -606,6 +606,8 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
     chk(u.ok = h.upload(L.ok));
     chk(u.lv_comp = h.upload(L.lv_comp));
     chk(u.lv_ptr = h.upload(L.lv_ptr));
+    chk(u.lv_mid = h.upload(L.lv_mid));
+    chk(u.lv_long = h.upload(L.lv_long));
     chk(u.in_lds = h.upload(L.in_lds));
     {
       std::vector<int32_t> t4(4 * L.t_dst.size());
@@ -624,9 +626,15 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
       chk(o = h.upload(L.ops));
       u.ops = (const int2 *)o;
     }
-    u.rowptr = h.d_rowptr;
-    u.colidx = h.d_colidx;
-    chk(u.qpos = h.upload(L.qpos));
+    chk(u.ent_off = h.upload(L.ent_off));
+    chk(u.ent_k = h.upload(L.ent_k));
+    chk(u.ent_pos = h.upload(L.ent_pos));
+    chk(u.ent_col = h.upload(L.ent_col));
+    chk(u.ent_rs = h.upload(L.ent_rs));
+    u.n_ent = (int32_t)L.ent_k.size();
+    u.n_nodes = P.n_nodes;
+    chk(u.Jp = h.alloc<double>((size_t)(u.n_ent > 0 ? u.n_ent : 1)));
+    chk(u.bp = h.alloc<double>((size_t)(P.n_nodes > 0 ? P.n_nodes : 1)));
     u.max_berr = 1e-12;
     chk(u.val_work = h.alloc<double>((size_t)(gwork > 0 ? gwork : 1)));
     u.vec_work = d.bwork;

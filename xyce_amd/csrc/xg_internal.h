@@ -192,11 +192,12 @@ struct SparseLuArgs {
   const int64_t *voff;            // n_comp + 1 value offsets
   const int32_t *ok;              // static factorisation available
   const int32_t *in_lds;          // factors, vector and tasks fit in LDS
-  const int32_t *lv_comp, *lv_ptr;
+  const int32_t *lv_comp, *lv_ptr, *lv_mid, *lv_long;
   const int4 *tasks;              // {dst, div, first op, end op}
   const int2 *ops;
-  const int32_t *rowptr, *colidx;
-  const int32_t *qpos;            // global unknown -> position (-1 ground)
+  const int32_t *ent_off, *ent_k, *ent_pos, *ent_col, *ent_rs;
+  double *Jp, *bp;                // J in entry order, rhs in position order
+  int32_t n_ent = 0, n_nodes = 0;
   const double *J, *rhs;
   double *x;
   const double *wa, *wb;

@@ -63,39 +63,95 @@ __host__ __device__ constexpr int64_t align16(int64_t b) { return (b + 15) & ~(i
 struct Lists {
   const int4 *tasks;
   const int2 *ops;
-  const int32_t *lvp;
+  const int32_t *lvp, *lvm, *lvl;  // level -> first task, first mid, first long
   int obase;  // subtracted from the ops fields of tasks (0 once staged)
 };
 
+// Finishes a task value: division by the pivot (factor and backward
+// solve) and the factor's acceptance checks.
+template <int kPhase>
+__device__ __forceinline__ void finish_task(const int4 &tk, double s, double *val,
+                                            double *vec, int &bad) {
+  if (kPhase == 0) {
+    if (tk.y >= 0) {
+      s = s / val[tk.y];
+      bad |= !(fabs(s) <= DBL_MAX);  // infinite or NaN
+    } else if (tk.y == -2) {
+      bad |= !(fabs(s) > 0.0) || isinf(s);
+    }
+    val[tk.x] = s;
+  } else {
+    vec[tk.x] = kPhase == 2 ? s / val[tk.y] : s;
+  }
+}
+
+// One level.  Short tasks (at most four terms): one thread each, the terms'
+// operands loaded together, then summed in list order.  Mid tasks: eight
+// lanes each, long tasks (rows of supply rails, which touch most of the
+// circuit): a wavefront each; lane-strided partial sums folded by a fixed
+// butterfly.  The association is fixed by the plan, so results do not
+// depend on the launch.
 template <int kPhase>
 __device__ __forceinline__ void run_level(const Lists &ls, int L, double *val,
                                           double *vec, int &bad) {
-  const int k0 = ls.lvp[L], k1 = ls.lvp[L + 1];
-  for (int k = k0 + (int)threadIdx.x; k < k1; k += kBlock) {
+  const int k0 = ls.lvp[L], km = ls.lvm[L], kl = ls.lvl[L], k1 = ls.lvp[L + 1];
+  const double *src = kPhase == 0 ? val : vec;
+  for (int k = k0 + (int)threadIdx.x; k < km; k += kBlock) {
     const int4 tk = ls.tasks[k];
-    const int o0 = tk.z - ls.obase, o1 = tk.w - ls.obase;
-    if (kPhase == 0) {
-      double s = val[tk.x];
-      for (int o = o0; o < o1; ++o) {
+    const int o0 = tk.z - ls.obase, m = tk.w - tk.z;
+    double a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j < m) {
+        const int2 q = ls.ops[o0 + j];
+        a[j] = val[q.x];
+        b[j] = src[q.y];
+      }
+    }
+    double s = src[tk.x];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (j < m) s = s - a[j] * b[j];
+    finish_task<kPhase>(tk, s, val, vec, bad);
+  }
+  {
+    const int grp = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    for (int k = km + grp; k < kl; k += kBlock / 8) {
+      const int4 tk = ls.tasks[k];
+      const int o0 = tk.z - ls.obase, o1 = tk.w - ls.obase;
+      double p = 0.0;
+      for (int o = o0 + sub; o < o1; o += 8) {
         const int2 q = ls.ops[o];
-        s = s - val[q.x] * val[q.y];
+        p = p + val[q.x] * src[q.y];
       }
-      if (tk.y >= 0) {
-        s = s / val[tk.y];
-        bad |= !(fabs(s) <= DBL_MAX);  // infinite or NaN
-      } else if (tk.y == -2) {
-        bad |= !(fabs(s) > 0.0) || isinf(s);
-      }
-      val[tk.x] = s;
-    } else {
-      double s = vec[tk.x];
-      for (int o = o0; o < o1; ++o) {
-        const int2 q = ls.ops[o];
-        s = s - val[q.x] * vec[q.y];
-      }
-      vec[tk.x] = kPhase == 2 ? s / val[tk.y] : s;
+      for (int off = 4; off >= 1; off >>= 1) p = p + __shfl_xor(p, off);
+      if (sub == 0) finish_task<kPhase>(tk, src[tk.x] - p, val, vec, bad);
     }
   }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = kl + wave; k < k1; k += kBlock / 64) {
+    const int4 tk = ls.tasks[k];
+    const int o0 = tk.z - ls.obase, o1 = tk.w - ls.obase;
+    double p = 0.0;
+    for (int o = o0 + lane; o < o1; o += 64) {
+      const int2 q = ls.ops[o];
+      p = p + val[q.x] * src[q.y];
+    }
+    for (int off = 32; off >= 1; off >>= 1) p = p + __shfl_xor(p, off);
+    if (lane == 0) finish_task<kPhase>(tk, src[tk.x] - p, val, vec, bad);
+  }
+}
+
+// Lays the Jacobian out per component in entry order and the right-hand
+// side in position order, so that each component's workgroup reads its
+// inputs contiguously.
+__global__ void __launch_bounds__(kBlock)
+    permute_kernel(int32_t n_ent, int32_t n_nodes, const int32_t *ent_k,
+                   const int32_t *prow, const double *J, const double *rhs,
+                   double *Jp, double *bp) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n_ent) Jp[i] = J[ent_k[i]];
+  if (i < n_nodes) bp[i] = rhs[prow[i]];
 }
 
 template <bool kLds>
@@ -141,27 +197,35 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
     int2 *op = (int2 *)p;
     p += align16((int64_t)nops * 8);
     int32_t *lvp = (int32_t *)p;
+    int32_t *lvm = lvp + nlev + 1;
+    int32_t *lvl = lvm + nlev;
     for (int k = t; k < ntask; k += kBlock) tk[k] = a.tasks[tbase + k];
     for (int k = t; k < nops; k += kBlock) op[k] = a.ops[obase + k];
     for (int k = t; k <= nlev; k += kBlock) lvp[k] = a.lv_ptr[L0 + k] - tbase;
-    ls = Lists{tk, op, lvp, obase};
+    for (int k = t; k < nlev; k += kBlock) {
+      lvm[k] = a.lv_mid[L0 + k] - tbase;
+      lvl[k] = a.lv_long[L0 + k] - tbase;
+    }
+    ls = Lists{tk, op, lvp, lvm, lvl, obase};
   } else {
     val = a.val_work + a.voff[c];
     vec = a.vec_work + beg;
     // global lists: level pointers stay absolute, ops absolute
-    ls = Lists{a.tasks, a.ops, a.lv_ptr + L0, 0};
+    ls = Lists{a.tasks, a.ops, a.lv_ptr + L0, a.lv_mid + L0, a.lv_long + L0, 0};
   }
-  const int32_t *prow = a.prow + beg, *qcol = a.qcol + beg;
+  const int32_t *qcol = a.qcol + beg;
+  // the component's Jacobian entries and right-hand side, already laid out
+  // contiguously by the permute kernel
+  const int eb = a.ent_off[c], ne = a.ent_off[c + 1] - eb;
+  const double *jp = a.Jp + eb, *bp = a.bp + beg;
+  const int32_t *epos = a.ent_pos + eb, *ecol = a.ent_col + eb, *ers = a.ent_rs + beg + c;
   for (int64_t e = t; e < nv; e += kBlock) val[e] = 0.0;
   __syncthreads();
+#pragma unroll 4
+  for (int j = t; j < ne; j += kBlock) val[epos[j]] = jp[j];
   double rmax = 0.0;
   for (int i = t; i < n; i += kBlock) {
-    const int g = prow[i];
-    for (int k = a.rowptr[g]; k < a.rowptr[g + 1]; ++k) {
-      const int p = a.lu_pos[k];
-      if (p >= 0) val[p] = a.J[k];
-    }
-    const double r = a.rhs[g];
+    const double r = bp[i];
     vec[i] = r;
     rmax = fmax(rmax, fabs(r));
   }
@@ -189,14 +253,13 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
   // ---- backward error of the solution against the unfactored rows
   double rm = 0.0, am = 0.0, xm = 0.0;
   for (int i = t; i < n; i += kBlock) {
-    const int g = prow[i];
-    double r = a.rhs[g], as = 0.0;
-    for (int k = a.rowptr[g]; k < a.rowptr[g + 1]; ++k) {
-      const int p = a.qpos[a.colidx[k]];
-      if (p >= 0) {
-        r = r - a.J[k] * vec[p];
-        as = as + fabs(a.J[k]);
-      }
+    double r = bp[i], as = 0.0;
+    const int j0 = ers[i], j1 = ers[i + 1];
+#pragma unroll 4
+    for (int j = j0; j < j1; ++j) {
+      const double v = jp[j];
+      r = r - v * vec[ecol[j]];
+      as = as + fabs(v);
     }
     rm = fmax(rm, fabs(r));
     am = fmax(am, as);
@@ -235,13 +298,18 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
 int64_t sparse_lu_lds_bytes(int64_t n, int64_t fnnz, int64_t tasks, int64_t ops,
                             int64_t levels) {
   return align16(fnnz * 8) + align16(n * 8) + tasks * 16 + align16(ops * 8) +
-         (levels + 1) * 4;
+         (3 * levels + 1) * 4;
 }
 
 int64_t sparse_lu_lds_budget() { return kLdsBytes; }
 
 hipError_t launch_sparse_lu(const SparseLuArgs &a, hipStream_t s) {
   if (a.n_comp == 0) return hipSuccess;
+  const int64_t nperm = a.n_ent > a.n_nodes ? a.n_ent : a.n_nodes;
+  if (nperm > 0)
+    hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nperm + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, a.n_ent, a.n_nodes, a.ent_k, a.prow, a.J,
+                       a.rhs, a.Jp, a.bp);
   hipLaunchKernelGGL(sparse_lu_kernel<true>, dim3(a.n_comp), dim3(kBlock), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !a.any_global) return e;

@@ -227,6 +227,8 @@ int lu_analyse(Plan &plan, std::string &err) {
   L.voff.assign(nc + 1, 0);
   L.ok.assign(nc, 0);
   L.in_lds.assign(nc, 1);
+  L.ent_off.assign(1, 0);
+  L.ent_rs.assign((size_t)n_all + nc, 0);
   L.lv_comp.assign((size_t)3 * nc + 1, 0);
   L.lv_ptr.assign(1, 0);
   L.t_op.assign(1, 0);
@@ -474,7 +476,17 @@ int lu_analyse(Plan &plan, std::string &err) {
       std::vector<std::vector<int32_t>> by(nlev);
       for (size_t k = 0; k < ts.size(); ++k) by[ts[k].lev].push_back((int32_t)k);
       for (auto &lv : by) {
+        // short tasks first, then mid, then long
+        auto cls = [&](int32_t k) {
+          const size_t m = ts[k].ops.size() / 2;
+          return m <= (size_t)kLuShortOps ? 0 : m <= (size_t)kLuLongOps ? 1 : 2;
+        };
+        std::stable_sort(lv.begin(), lv.end(),
+                         [&](int32_t x, int32_t y) { return cls(x) < cls(y); });
+        int32_t first_mid = -1, first_long = -1;
         for (int32_t k : lv) {
+          if (first_mid < 0 && cls(k) >= 1) first_mid = (int32_t)L.t_dst.size();
+          if (first_long < 0 && cls(k) == 2) first_long = (int32_t)L.t_dst.size();
           const Task &t = ts[k];
           L.t_dst.push_back(t.dst);
           L.t_div.push_back(t.div);
@@ -482,6 +494,9 @@ int lu_analyse(Plan &plan, std::string &err) {
           L.t_op.push_back((int32_t)(L.ops.size() / 2));
           if (ph == 0) L.factor_ops += (int64_t)t.ops.size() / 2;
         }
+        const int32_t end = (int32_t)L.t_dst.size();
+        L.lv_long.push_back(first_long < 0 ? end : first_long);
+        L.lv_mid.push_back(first_mid < 0 ? L.lv_long.back() : first_mid);
         L.lv_ptr.push_back((int32_t)L.t_dst.size());
       }
       L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
@@ -500,6 +515,29 @@ int lu_analyse(Plan &plan, std::string &err) {
       err = "sparse LU: task lists exceed 2^31 entries";
       return XG_E_ARG;
     }
+  }
+  // entry lists (every component, with or without a static factorisation)
+  for (int32_t c = 0; c < nc; ++c) {
+    const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
+    const int32_t eb = L.ent_off.back();
+    int32_t *rs = L.ent_rs.data() + beg + c;
+    for (int32_t i = 0; i < n; ++i) {
+      rs[i] = (int32_t)L.ent_k.size() - eb;
+      const int32_t g = L.prow[beg + i];
+      for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
+        const int32_t col = plan.colidx[k];
+        if (col >= G) continue;
+        L.ent_k.push_back(k);
+        L.ent_pos.push_back(L.lu_pos[k]);
+        L.ent_col.push_back(L.qpos[col]);
+      }
+    }
+    rs[n] = (int32_t)L.ent_k.size() - eb;
+    if (L.ent_k.size() > (size_t)INT32_MAX) {
+      err = "sparse LU: entry lists exceed 2^31 entries";
+      return XG_E_ARG;
+    }
+    L.ent_off.push_back((int32_t)L.ent_k.size());
   }
   return XG_OK;
 }

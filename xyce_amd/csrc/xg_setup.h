@@ -102,16 +102,29 @@ struct Circuit {
 //   solve    y[dst] = (y[dst] - sum v[a] * y[b]) [/ v[div]]
 // with its terms in ascending order, so every result is computed by one
 // thread in one fixed order.
+constexpr int kLuShortOps = 4;
+constexpr int kLuLongOps = 32;
 struct LuPlan {
   std::vector<int32_t> prow, qcol;    // by comp_start offsets
   std::vector<int32_t> qpos;          // unknown -> position in its component
                                       // (n_nodes + 1 entries, ground -1)
   std::vector<int32_t> lu_pos;        // CSR entry -> position in its block, -1
+  // the CSR entries of each component's rows in position order (ground
+  // columns left out): ent_off (n_comp + 1) -> CSR entry, factor position,
+  // column position; ent_rs: per component n_c + 1 row starts (relative),
+  // at comp_start[c] + c
+  std::vector<int32_t> ent_off, ent_k, ent_pos, ent_col, ent_rs;
   std::vector<int64_t> voff;          // n_comp + 1 value offsets
   std::vector<int32_t> ok;            // 1: matching found (static factorisation)
   std::vector<int32_t> in_lds;        // 1: factors, vector and tasks fit LDS
   std::vector<int32_t> lv_comp;       // 3 n_comp + 1: phase -> first level
   std::vector<int32_t> lv_ptr;        // level -> first task (n_levels + 1)
+  std::vector<int32_t> lv_mid, lv_long;  // level -> first task with more
+                                      // than kLuShortOps / kLuLongOps terms
+                                      // (n_levels each); a level holds its
+                                      // short tasks (one thread each), then
+                                      // mid (8 lanes each), then long (one
+                                      // wavefront each)
   std::vector<int32_t> t_dst, t_div;  // t_div: -1 none, -2 pivot (check only)
   std::vector<int32_t> t_op;          // task -> first op (n_tasks + 1)
   std::vector<int32_t> ops;           // pairs (a, b)
