@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--mc-tstop", type=float, default=0.5e-9)
     ap.add_argument("--mc-dt", type=float, default=1e-12)
     ap.add_argument("--no-mc", action="store_true", help="skip the batched-transient leg")
+    ap.add_argument("--no-grid", action="store_true",
+                    help="skip the power-grid (100k inverter) load timing")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 512 MiB device copy, the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -137,6 +139,85 @@ def batched_transient_leg(torch, args):
         "batch_setup_s": b["setup_s"],
         "max_waveform_diff_batch_vs_plain_V": diff,
     }
+
+
+def power_grid_leg(torch, args):
+    """The 500 x 500 resistive power-grid mesh driving 100,000 BSIM3
+    inverters (200,000 FETs, 100,000 clock sources, 550,004 unknowns, seed
+    5): one device load per Newton step (every device: BSIM3, resistors,
+    capacitors, sources, and the ordered reduction into F, Q, the limiter
+    vectors and both Jacobians) and the Jacobian assembly of a transient
+    step, timed at full scale with seeded node voltages."""
+    from xyce_amd import circuits
+    from xyce_amd._lib import check, lib
+    from xyce_amd.loader import DeviceLoad, _ptr
+    progress("power grid: building the netlist")
+    t0 = time.perf_counter()
+    nl = circuits.power_grid_bsim3()
+    nl.layout()
+    dev = DeviceLoad(nl)
+    setup = time.perf_counter() - t0
+    progress("power grid: %d unknowns, %d CSR entries, setup %.1f s"
+             % (nl.n_nodes, dev.nnz, setup))
+    n1 = nl.n_nodes + 1
+    rng = np.random.default_rng(13)
+    x = rng.uniform(-0.2, 1.4, n1)
+    x[-1] = 0.0
+    cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    X = cu(x)
+    SC, SN = cu(rng.uniform(-1e-14, 1e-14, nl.n_state)), cu(np.zeros(nl.n_state))
+    TC, TN = cu(rng.uniform(-0.2, 1.4, nl.n_store)), cu(np.zeros(nl.n_store))
+    out = torch.zeros(4 * n1 + 3 * dev.nnz, dtype=torch.float64, device="cuda")
+    F, Q, FL, QL = (out[i * n1:(i + 1) * n1] for i in range(4))
+    JF, JQ, J = (out[4 * n1 + i * dev.nnz:4 * n1 + (i + 1) * dev.nnz] for i in range(3))
+    qn0, fprev, rhs = cu(np.zeros(n1)), cu(np.zeros(n1)), cu(np.zeros(n1))
+    stream = torch.cuda.current_stream()
+    dev.set_stream(stream.cuda_stream)
+    dev.set_output_mode(True)
+    dev.set_solver_state(transient=1, newtonIter=1)
+    dev.set_time(0.3e-9, 0.0)
+
+    def load():
+        dev.load_all(X, SC, TC, SN, TN, F, Q, FL, QL, JF, JQ)
+
+    def assemble():
+        check(lib().xg_tia_assemble(dev.h, 1, 2, 1e-12, *[_ptr(t) for t in (
+            qn0, fprev, F, Q, FL, QL, JF, JQ, rhs, J)]), "assemble")
+
+    def timed(fn, k):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(k):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / k
+
+    k = 30
+    load_ms = timed(load, k)
+    asm_ms = timed(assemble, k)
+    eval_ms = timed(lambda: dev.update_state(X, SC, TC, SN, TN), k)
+    n_b3 = len(nl.bsim3)
+    progress("power grid: load %.3f ms, assembly %.3f ms" % (load_ms, asm_ms))
+    res = {
+        "workload": "500x500 resistive power-grid mesh + 100,000 BSIM3 inverters with "
+                    "clock sources (200,000 FETs, %d unknowns, %d CSR entries, seed 5)"
+                    % (nl.n_nodes, dev.nnz),
+        "device_load_ms_per_newton_step": load_ms,
+        "jacobian_assembly_ms": asm_ms,
+        "newton_step_load_plus_assembly_ms": load_ms + asm_ms,
+        "bsim3_evals_per_s_in_load": n_b3 / (load_ms * 1e-3),
+        "device_eval_ms": eval_ms,
+        "eval_roofline_frac": B3_BYTES_PER_EVAL * n_b3 / (eval_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "setup_s": setup,
+        "linear_solve": "not timed: one 550k-unknown component exceeds the static "
+                        "sparse-LU fill budget",
+    }
+    dev.close()
+    return res
 
 
 def cpu_baseline(nl, x, sta, sto, budget_s):
@@ -298,6 +379,8 @@ def main():
     progress("device load: %.4f ms per step" % ms_step)
     if not args.no_mc:
         res["batched_transient"] = batched_transient_leg(torch, args)
+    if not args.no_grid:
+        res["power_grid"] = power_grid_leg(torch, args)
     if not args.no_cpu:
         progress("CPU baseline (%.0f s sample)" % args.cpu_seconds)
         sta_h, sto_h = sta.copy(), sto.copy()

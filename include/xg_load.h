@@ -240,6 +240,18 @@ int xg_set_output_mode(xg_handle *h, int32_t mode);
 #define XG_EVAL_STAGED_SPLIT 3 /* as STAGED, DC and charge parts on two waves */
 int xg_set_eval_variant(xg_handle *h, int32_t variant);
 
+/* Junction diode (N_DEV_Diode.C, level 1, RS = 0 so the internal node is
+   the positive one): .MODEL parameters IS (JS), N, TT, CJO (CJ, CJ0), VJ,
+   M, EG, XTI, FC, BV (VB), IBV, IRF, TNOM (Celsius), KF, AF (ignored:
+   noise); instance parameters AREA, IC, TEMP (Celsius), OFF.  sto: the
+   store slot of the junction voltage (li_storevd).  Loads in the Master
+   loops of N_DEV_Diode.C:2007-2108 (updateState, loadDAEVectors,
+   loadDAEMatrices), after capacitors and before sources.  XG_E_PARAM for
+   level 2, RS != 0 or LAMBERTW != 0. */
+int xg_add_diode(xg_circuit *c, int32_t pos, int32_t neg, int32_t n_model,
+                 const char *const *mkeys, const double *mvals, int32_t n_inst,
+                 const char *const *ikeys, const double *ivals, int32_t sto);
+
 /* ---- Newton iteration linear algebra (the rows after the device load) */
 
 /* Independent sub-circuits: connected components of the circuit graph with
@@ -299,6 +311,14 @@ int xg_lu_info(xg_handle *h, int64_t info[8]);
    for LuPlan in the library sources.  No counterpart in the reference. */
 int xg_lu_export(const xg_handle *h, int64_t sizes[6], int32_t *const *arrays,
                  int64_t *voff);
+
+/* Device-held instance state that is not in the state / store vectors
+   (each diode's junction voltage of its last evaluation, which
+   Master::updateState stores at the next call, N_DEV_Diode.C:2016), for
+   checkpoint and restart (N_IO_RestartMgr.C:285-870 saves the device
+   instances' state with the vectors).  mode 0: *n = number of doubles;
+   1: copy to buf; 2: copy from buf (*n must match).  Synchronises. */
+int xg_instance_state(xg_handle *h, int32_t mode, double *buf, int64_t *n);
 
 /* Step parameters from device memory (for graph-captured step loops, whose
    kernel arguments are fixed at capture): when dstep is not NULL, the
@@ -380,6 +400,10 @@ int xg_tran_probe(xg_tran *t, const int32_t *ids, int32_t n, double *out);
 /* Time of the accepted solution; counters[0..3] = steps, Newton
    iterations, failed steps, last failure word. */
 int xg_tran_info(xg_tran *t, double *time, int64_t counters[4]);
+/* Restart: the accepted time, step count and counters of a checkpoint
+   (the caller restores the vectors it owns). */
+int xg_tran_restore(xg_tran *t, double time, int64_t n_steps, int64_t newton_total,
+                    int64_t step_failures);
 /* 0: launch every iteration's kernels one by one instead of a graph. */
 int xg_tran_set_graphs(xg_tran *t, int32_t on);
 int xg_tran_sync(xg_tran *t);

@@ -14,13 +14,15 @@ in Python floats, which are IEEE doubles):
   F[br] += (v+ - v-), F[br] -= source; dF/dx +1 (+,br), -1 (-,br),
   +1 (br,+), -1 (br,-).
 * PULSE     N_DEV_SourceData.C:928-1000 (PulseData::updateSource).
+* Diode     oracle/diode_ref.py (level 1, RS = 0).
 
 Device types load after the MOSFETs, in the order resistor, capacitor,
-source, each in netlist order.
+diode, source, each in netlist order.
 """
 import math
 
 from . import stamp_ref
+from .diode_ref import RefDiode, diode_fields
 
 
 def pulse_value(p, t, bp_tol):
@@ -40,12 +42,27 @@ def pulse_value(p, t, bp_tol):
 
 
 class RefLinear:
-    def __init__(self, nl, rowptr, colidx, rowmaps=None):
+    def __init__(self, nl, rowptr, colidx, rowmaps=None, gmin=1e-12, limiter=True,
+                 temp=300.15, tnom=300.15):
         self.n = nl.n_nodes
         gl = lambda g: self.n if g < 0 else g
         self.dev = []
+        self.diodes = []
+        self.gmin, self.limiter = gmin, limiter
         rowmaps = {} if rowmaps is None else rowmaps
         for kind, p, q, br, val, pulse, st in nl.linear_devices():
+            if kind == "D":
+                off = stamp_ref.stamp_offsets(rowptr, colidx, self.n, [p, q], [[0, 1], [0, 1]],
+                                              rowmaps)
+                P, N = gl(p), gl(q)
+                pp, pn = rowptr[P] + off[0][0], rowptr[P] + off[0][1]
+                np_, nn = rowptr[N] + off[1][0], rowptr[N] + off[1][1]
+                # dF/dx: (P,P) (P,Pri) (N,N) (N,Pri) (Pri,P) (Pri,N) (Pri,Pri), Pri = P
+                pos_f = [pp, pp, nn, np_, pp, pn, pp]
+                pos_q = [nn, np_, pn, pp]
+                self.diodes.append(RefDiode(diode_fields(val[0], val[1], temp, tnom), P, N, st,
+                                            pos_f, pos_q))
+                continue
             if kind == "V":
                 gids = [p, q, br]
                 stamp = [[2], [2], [0, 1]]
@@ -62,17 +79,20 @@ class RefLinear:
             g = 1.0 / val if kind == "R" else val
             self.dev.append(dict(kind=kind, p=gl(p), q=gl(q), br=gl(br) if br >= 0 else -1,
                                  val=g, pulse=pulse, sta=st, pos=pos))
-        self.flags = dict(tranop=0, transient=0)
+        self.flags = dict(tranop=0, transient=0, dcop=0, initJct=0, newtonIter=0)
         self.time = 0.0
         self.bp_tol = 0.0
 
-    def set_solver_state(self, tranop=0, transient=0, **_):
-        self.flags = dict(tranop=tranop, transient=transient)
+    def set_solver_state(self, tranop=0, transient=0, dcop=0, initJct=0, newtonIter=0, **_):
+        self.flags = dict(tranop=tranop, transient=transient, dcop=dcop, initJct=initJct,
+                          newtonIter=newtonIter)
 
     def set_time(self, t, bp_tol=0.0):
         self.time, self.bp_tol = float(t), float(bp_tol)
 
-    def update_state(self, x, sta_n):
+    def update_state(self, x, sta_n, sto_c=None, sto_n=None):
+        for dio in self.diodes:
+            dio.update_state(x, sto_c, sto_n, self.flags, self.gmin, self.limiter)
         for d in self.dev:
             vp, vn = float(x[d["p"]]), float(x[d["q"]])
             if d["kind"] == "R":
@@ -87,7 +107,7 @@ class RefLinear:
                 d["ibr"] = float(x[d["br"]])
                 d["drop"] = vp - vn
 
-    def load_vectors(self, F, Q):
+    def load_vectors(self, F, Q, FL=None, QL=None):
         for d in self.dev:
             if d["kind"] == "R":
                 F[d["p"]] += d["i0"]
@@ -96,6 +116,8 @@ class RefLinear:
             if d["kind"] == "C":
                 Q[d["p"]] += d["q0"]
                 Q[d["q"]] += -d["q0"]
+        for dio in self.diodes:
+            dio.load_vectors(F, Q, FL, QL, self.limiter)
         for d in self.dev:
             if d["kind"] == "V":
                 F[d["p"]] += d["ibr"]
@@ -104,7 +126,11 @@ class RefLinear:
                 F[d["br"]] -= d["src"]
 
     def load_matrices(self, dFdx, dQdx):
-        for kind in ("R", "C", "V"):
+        for kind in ("R", "C", "D", "V"):
+            if kind == "D":
+                for dio in self.diodes:
+                    dio.load_matrices(dFdx, dQdx)
+                continue
             for d in self.dev:
                 if d["kind"] != kind:
                     continue

@@ -161,7 +161,8 @@ class RefLoad:
             from . import bsim3_bind
             self.b3 = bsim3_bind.RefBsim3(self, nl, gmin, limiter, temp)
         from .linear_ref import RefLinear
-        self.lin = RefLinear(nl, self.rowptr, self.colidx)
+        self.lin = RefLinear(nl, self.rowptr, self.colidx, gmin=gmin, limiter=limiter,
+                             temp=temp)
 
     def set_solver_state(self, dcop=0, initJct=0, initFix=0, initTran=0, newtonIter=0,
                          tranop=0, acop=0, transient=0):
@@ -171,7 +172,8 @@ class RefLoad:
         lib().ref_mos1_set_solver_state(self.m1, _i(f))
         if self.b3:
             self.b3.set_solver_state(f)
-        self.lin.set_solver_state(tranop=tranop, transient=transient)
+        self.lin.set_solver_state(tranop=tranop, transient=transient, dcop=dcop,
+                                  initJct=initJct, newtonIter=newtonIter)
 
     def set_time(self, t, bp_tol=0.0):
         self.lin.set_time(t, bp_tol)
@@ -180,13 +182,13 @@ class RefLoad:
         lib().ref_mos1_update_state(self.m1, _d(x), _d(sta_c), _d(sto_c), _d(sta_n), _d(sto_n))
         if self.b3:
             self.b3.update_state(x, sta_c, sto_c, sta_n, sto_n)
-        self.lin.update_state(x, sta_n)
+        self.lin.update_state(x, sta_n, sto_c, sto_n)
 
     def load_vectors(self, F, Q, dFdxdVp, dQdxdVp):
         lib().ref_mos1_load_vectors(self.m1, _d(F), _d(Q), _d(dFdxdVp), _d(dQdxdVp))
         if self.b3:
             self.b3.load_vectors(F, Q, dFdxdVp, dQdxdVp)
-        self.lin.load_vectors(F, Q)
+        self.lin.load_vectors(F, Q, dFdxdVp, dQdxdVp)
 
     def load_matrices(self):
         """Adds into self.dFdx / self.dQdx (zero them first, as the

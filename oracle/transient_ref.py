@@ -85,6 +85,7 @@ class RefTransient:
         if self.ref.b3:
             l3, _ = self.ref.b3.flags()
             n_lim += int(l3.sum())
+        n_lim += sum(0 if d.orig else 1 for d in self.ref.lin.diodes)
         self.n_lim = n_lim
 
     def _assemble(self, transient, order, dt):

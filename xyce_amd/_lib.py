@@ -79,6 +79,9 @@ SIGNATURES = [
                                         ctypes.c_double] + [ctypes.c_void_p] * 10),
     ("xg_dense_solve", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 +
      [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_add_diode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_int32, c_char_pp, c_double_p, ctypes.c_int32,
+                                    c_char_pp, c_double_p, ctypes.c_int32]),
     ("xg_lu_solve", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 +
      [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_lu_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -93,6 +96,10 @@ SIGNATURES = [
     ("xg_tran_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_tran_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_tran_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("xg_tran_restore", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64,
+                                       ctypes.c_int64, ctypes.c_int64]),
+    ("xg_instance_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
     ("xg_tran_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_set_eval_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_set_output_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

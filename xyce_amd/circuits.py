@@ -68,6 +68,7 @@ class Netlist:
         self.caps = []         # (name, a, b, C)
         self.res = []          # (name, a, b, R)
         self.vsrc = []         # Vsource
+        self.diodes = []       # (name, a, b, model, instance params)
         self.ic = {}           # node -> initial voltage (.IC)
         self._laid_out = False
 
@@ -94,6 +95,10 @@ class Netlist:
     def add_res(self, name, a, b, R):
         self.node(a), self.node(b)
         self.res.append((name, a, b, float(R)))
+
+    def add_diode(self, name, a, b, model, **params):
+        self.node(a), self.node(b)
+        self.diodes.append((name, a, b, model, dict(params)))
 
     def add_vsrc(self, name, p, n, dc, pulse=None):
         self.node(p), self.node(n)
@@ -151,6 +156,11 @@ class Netlist:
                 self.bsim3.append(rec)
             else:
                 raise ValueError("unknown model kind %r" % mdl.kind)
+        # one store slot per diode (li_storevd)
+        self.diode_sto = []
+        for _ in self.diodes:
+            self.diode_sto.append(sto)
+            sto += 1
         self.vsrc_branch = []
         for v in self.vsrc:
             self.vsrc_branch.append(n)
@@ -179,12 +189,17 @@ class Netlist:
 
     def linear_devices(self):
         """(kind, pos, neg, branch, value, pulse, sta) in load order:
-        resistors, capacitors, sources, each in netlist order."""
+        resistors, capacitors, diodes, sources, each in netlist order.  For
+        a diode, value = (model parameters, instance parameters) and sta is
+        its store slot."""
         out = []
         for name, a, b, r in self.res:
             out.append(("R", self.node(a), self.node(b), -1, r, None, 0))
         for (name, a, b, c), st in zip(self.caps, self.cap_sta):
             out.append(("C", self.node(a), self.node(b), -1, c, None, st))
+        for (name, a, b, model, params), st in zip(self.diodes, self.diode_sto):
+            out.append(("D", self.node(a), self.node(b), -1,
+                        (self.models[model].params, params), None, st))
         for v, br in zip(self.vsrc, self.vsrc_branch):
             out.append(("V", self.node(v.p), self.node(v.n), br, v.dc, self.pulse_params(v), 0))
         return out
@@ -196,6 +211,8 @@ class Netlist:
         [[0 0 1],[0 0 1],[1 1 0]]-shaped stamp (N_DEV_Vsrc.C)."""
         out = []
         for _, a, b, _C in self.caps + self.res:
+            out.append(([self.node(a), self.node(b)], [0, 0, 1, 1], [0, 1, 0, 1]))
+        for _, a, b, _m, _p in self.diodes:
             out.append(([self.node(a), self.node(b)], [0, 0, 1, 1], [0, 1, 0, 1]))
         for v, br in zip(self.vsrc, self.vsrc_branch):
             out.append(([self.node(v.p), self.node(v.n), br], [0, 1, 2, 2], [2, 2, 0, 1]))
@@ -461,6 +478,86 @@ def random_node_voltages(nl, seed=0, lo=-0.2, hi=2.0):
     return x
 
 
+def power_grid_bsim3(nx=500, ny=500, n_inv=100000, seed=5, vdd=1.2, period=1e-9):
+    """Resistive power-grid mesh driving BSIM3 inverter loads: an nx x ny
+    mesh (R ~ U[0.05, 0.2] ohm between neighbours, 10 fF from every node to
+    ground) fed by Vdd sources at its four corners; n_inv inverters at
+    evenly spread mesh taps (PMOS source and bulk on the tap, NMOS on
+    ground), each input a clock PULSE(0 vdd) of the given period with a
+    phase drawn from U[0, period], each output loaded with 2 fF.  Seeded."""
+    rng = np.random.default_rng(seed)
+    nl = Netlist("power grid %dx%d, %d BSIM3 inverters" % (nx, ny, n_inv))
+    nl.add_model("NB", "bsim3", 1, **BSIM3_NMOS_130)
+    nl.add_model("PB", "bsim3", -1, **BSIM3_PMOS_130)
+    g = lambda i, j: "g%d_%d" % (i, j)
+    rh = rng.uniform(0.05, 0.2, (nx, ny - 1))
+    rv = rng.uniform(0.05, 0.2, (nx - 1, ny))
+    for i in range(nx):
+        for j in range(ny):
+            if j + 1 < ny:
+                nl.add_res("RH%d_%d" % (i, j), g(i, j), g(i, j + 1), rh[i, j])
+            if i + 1 < nx:
+                nl.add_res("RV%d_%d" % (i, j), g(i, j), g(i + 1, j), rv[i, j])
+            nl.add_cap("CG%d_%d" % (i, j), g(i, j), "0", 10e-15)
+    for k, (i, j) in enumerate(((0, 0), (0, ny - 1), (nx - 1, 0), (nx - 1, ny - 1))):
+        nl.add_vsrc("VDD%d" % k, g(i, j), "0", vdd)
+    phase = rng.uniform(0.0, period, n_inv)
+    taps = np.linspace(0, nx * ny - 1, n_inv).round().astype(int)
+    wn, wp = 0.5e-6, 1.0e-6
+    for k in range(n_inv):
+        tap = g(taps[k] // ny, taps[k] % ny)
+        inp, out = "in%d" % k, "out%d" % k
+        nl.add_vsrc("VCLK%d" % k, inp, "0", 0.0,
+                    Pulse(0.0, vdd, float(phase[k]), 50e-12, 50e-12,
+                          period / 2 - 50e-12, period))
+        nl.add_mos("MN%d" % k, out, inp, "0", "0", "NB", W=wn, L=0.13e-6, **_diffusion(wn))
+        nl.add_mos("MP%d" % k, out, inp, tap, tap, "PB", W=wp, L=0.13e-6, **_diffusion(wp))
+        nl.add_cap("CL%d" % k, out, "0", 2e-15)
+    return nl
+
+
+def rc_ladder_diodes(n_nodes=10000, n_diodes=1000, seed=4, vdd=1.0, r=(10.0, 100.0),
+                     c=1e-15, model=None):
+    """RC ladder of n_nodes nodes (R ~ U[r], C from every node to ground)
+    driven at its head by a PULSE(0 vdd) source, with n_diodes junction
+    diodes (IS = 1e-14, N = 1.0 unless `model` says otherwise) from evenly
+    spread ladder nodes to ground.  Seeded."""
+    rng = np.random.default_rng(seed)
+    nl = Netlist("RC ladder, %d nodes, %d diodes" % (n_nodes, n_diodes))
+    nl.add_model("DL", "diode", 1, **(model or dict(IS=1e-14, N=1.0)))
+    nl.add_vsrc("VIN", "n0", "0", 0.0, Pulse(0.0, vdd, 0.0, 20e-12, 20e-12, 1e-9, 2e-9))
+    rs = rng.uniform(r[0], r[1], n_nodes - 1)
+    for k in range(n_nodes - 1):
+        nl.add_res("R%d" % k, "n%d" % k, "n%d" % (k + 1), rs[k])
+        nl.add_cap("C%d" % k, "n%d" % (k + 1), "0", c)
+    taps = np.linspace(1, n_nodes - 1, n_diodes).round().astype(int)
+    for j, t in enumerate(taps):
+        nl.add_diode("D%d" % j, "n%d" % t, "0", "DL")
+    return nl
+
+
+def diode_variety():
+    """Diodes in every regime the load distinguishes: forward, linear
+    reverse, breakdown (BV), depletion and diffusion charge (CJO, TT, FC),
+    AREA, OFF, IC, TNOM / TEMP, grounded and floating terminals."""
+    nl = Netlist("diode variety")
+    nl.add_model("D0", "diode", 1, IS=1e-14, N=1.0)
+    nl.add_model("D1", "diode", 1, IS=2e-15, N=1.3, CJO=2e-15, VJ=0.8, M=0.4, FC=0.6, TT=1e-11)
+    nl.add_model("D2", "diode", 1, IS=1e-13, BV=3.0, IBV=1e-4, CJO=1e-15, TNOM=40.0)
+    nl.add_model("D3", "diode", 1, IS=5e-14, N=1.1, BV=5.0, IBV=10.0, IRF=2.0, XTI=2.0, EG=1.0)
+    nl.add_vsrc("V1", "a", "0", 0.7)
+    nl.add_res("R1", "a", "b", 50.0)
+    nl.add_diode("DA", "b", "0", "D0")
+    nl.add_diode("DB", "b", "c", "D1", AREA=2.0)
+    nl.add_diode("DC", "0", "c", "D2")
+    nl.add_diode("DD", "c", "d", "D3", TEMP=60.0)
+    nl.add_diode("DE", "d", "b", "D1", OFF=1)
+    nl.add_diode("DF", "a", "d", "D0", IC=0.3)
+    nl.add_res("R2", "c", "0", 1e3)
+    nl.add_cap("C1", "d", "0", 1e-15)
+    return nl
+
+
 def merge(parts, prefixes=None):
     """One netlist holding several independent circuits (they share only
     ground): node, model and device names get a per-part prefix.  The
@@ -483,6 +580,8 @@ def merge(parts, prefixes=None):
             nl.add_res(pre + name, ren(a), ren(b), r)
         for name, a, b, c in src.caps:
             nl.add_cap(pre + name, ren(a), ren(b), c)
+        for name, a, b, model, params in src.diodes:
+            nl.add_diode(pre + name, ren(a), ren(b), pre + model, **params)
         for v in src.vsrc:
             nl.add_vsrc(pre + v.name, ren(v.p), ren(v.n), v.dc, v.pulse)
     return nl

@@ -259,6 +259,28 @@ int xg_add_vsource(xg_circuit *c, int32_t pos, int32_t neg, int32_t branch,
   return add_lin(c, xg::LIN_V, pos, neg, branch, dc, pulse, 0, "xg_add_vsource");
 }
 
+int xg_add_diode(xg_circuit *c, int32_t pos, int32_t neg, int32_t n_model,
+                 const char *const *mkeys, const double *mvals, int32_t n_inst,
+                 const char *const *ikeys, const double *ivals, int32_t sto) {
+  if (!c) return fail(XG_E_ARG, "xg_add_diode: bad argument");
+  if (sto < 0 || sto >= c->c.n_store)
+    return fail(XG_E_ARG, "xg_add_diode: store slot out of range");
+  xg::ParamSet mp, ip;
+  if (xg::parse_params(n_model, mkeys, mvals, mp) ||
+      xg::parse_params(n_inst, ikeys, ivals, ip))
+    return fail(XG_E_ARG, "xg_add_diode: bad parameter list");
+  double f[xg::DIO_N];
+  std::string err;
+  int rc = xg::diode_setup(mp, ip, c->c.opt, f, err);
+  if (rc) return fail(rc, "xg_add_diode: " + err);
+  rc = add_lin(c, xg::LIN_D, pos, neg, -1, 0.0, nullptr, 0, "xg_add_diode");
+  if (rc) return rc;
+  xg::LinDev &d = c->c.lin.back();
+  d.sta = sto;
+  for (int k = 0; k < xg::DIO_N; ++k) d.dpar[k] = f[k];
+  return XG_OK;
+}
+
 int xg_components(const xg_handle *H, int32_t *n_comp, int32_t *comp_of_node) {
   if (!H || !n_comp) return fail(XG_E_ARG, "xg_components: bad argument");
   const xg::Plan &P = H->h.plan;
@@ -473,6 +495,25 @@ void handle_sizes(const xg_handle *H, int64_t *n_state, int64_t *n_store, int64_
 
 extern "C" {
 
+int xg_instance_state(xg_handle *H, int32_t mode, double *buf, int64_t *n) {
+  if (!H || !n || mode < 0 || mode > 2 || (mode && !buf))
+    return fail(XG_E_ARG, "xg_instance_state: bad argument");
+  xg::Handle &h = H->h;
+  const int64_t m = h.plan.lin_slots;
+  if (mode == 0) {
+    *n = m;
+    return XG_OK;
+  }
+  if (*n != m) return fail(XG_E_ARG, "xg_instance_state: size mismatch");
+  if (h.plan_only) return fail(XG_E_STATE, "handle was finalized without a device");
+  if (m == 0) return XG_OK;
+  hipError_t e = hipStreamSynchronize(h.stream);
+  if (e == hipSuccess)
+    e = mode == 1 ? hipMemcpy(buf, h.lin.dvd, sizeof(double) * m, hipMemcpyDeviceToHost)
+                  : hipMemcpy(h.lin.dvd, buf, sizeof(double) * m, hipMemcpyHostToDevice);
+  return e == hipSuccess ? XG_OK : hip_fail(e, "instance state copy");
+}
+
 int xg_set_step_source(xg_handle *H, const double *dstep) {
   if (!H) return fail(XG_E_ARG, "xg_set_step_source: bad argument");
   H->h.dstep = dstep;
@@ -564,7 +605,7 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
   }
   // slots, flags
   chk(h.slots = h.alloc<double>((size_t)P.n_slot_values));
-  h.n_limited_slots = P.m1_slots + xg::bsim3_eval_slots(P);
+  h.n_limited_slots = P.m1_slots + xg::bsim3_eval_slots(P) + P.lin_slots;
   chk(h.limited_all = h.alloc<int32_t>((size_t)h.n_limited_slots));
   chk(h.count_buf = h.alloc<int64_t>(128));
   // circuit CSR and components for the Newton solve
@@ -650,8 +691,16 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
   chk(h.lin.val = h.upload(P.lin_val));
   chk(h.lin.pulse = h.upload(P.lin_pulse));
   chk(h.lin.sta = h.upload(P.lin_sta));
+  chk(h.lin.dpar = h.upload(P.lin_dpar));
+  {
+    std::vector<double> z((size_t)(P.lin_slots > 0 ? P.lin_slots : 1), 0.0);
+    chk(h.lin.dvd = h.upload(z));
+  }
+  h.lin.gmin = c->c.opt.gmin;
+  h.lin.limiter = c->c.opt.limiter;
   if (ok) {
     h.lin.slots = h.slots + P.lin_slot_base;
+    h.lin.limited = h.limited_all + P.m1_slots + xg::bsim3_eval_slots(P);
     h.m1.slots = h.slots;
     h.m1.limited = h.limited_all;
     xg::bsim3_bind(h.b3, h.slots, h.limited_all + P.m1_slots);

@@ -101,8 +101,34 @@ struct ReduceDev {
 
 // Linear devices (resistor, capacitor, independent voltage source), loaded
 // through the same statement / ordered-reduction path as the MOSFETs.
-enum LinType : int32_t { LIN_R = 0, LIN_C = 1, LIN_V = 2 };
-constexpr int kLinStatements = 8;
+enum LinType : int32_t { LIN_R = 0, LIN_C = 1, LIN_D = 2, LIN_V = 3 };
+constexpr int kLinStatements = 20;  // the diode's; the others use fewer
+
+// Per-instance diode fields (temperature- and area-scaled, N_DEV_Diode.C
+// updateTemperature :1332-1470 and Model::processParams :1772-1793).
+enum DioField : int32_t {
+  DIO_ISAT = 0,   // tSatCur * AREA
+  DIO_VTE,        // N * k T / q
+  DIO_VCRIT,      // tVcrit
+  DIO_IRF,        // tIRF
+  DIO_BRKV,       // tBrkdwnV (0: no breakdown)
+  DIO_BVGIVEN,    // 1 if BV was given
+  DIO_BV,         // BV
+  DIO_TJCAP,      // tJctCap
+  DIO_CZERO,      // tJctCap * AREA
+  DIO_TJPOT,      // tJctPot
+  DIO_TDEPCAP,    // tDepCap = FC * tJctPot
+  DIO_TF1,        // tF1
+  DIO_M,          // grading coefficient
+  DIO_F2,
+  DIO_F3,
+  DIO_TT,         // transit time
+  DIO_GSPR,       // tCOND * AREA (0: no series resistance)
+  DIO_OFF,        // OFF flag
+  DIO_HASIC,      // IC given
+  DIO_IC,         // IC value
+  DIO_N
+};
 
 struct LinArgs {
   int32_t n_slots = 0;            // padded (multiple of 64), grouped by type
@@ -110,7 +136,13 @@ struct LinArgs {
   const int32_t *lid = nullptr;   // 3 x n_slots: pos, neg, branch (ground = n_nodes)
   const double *val = nullptr;    // n_slots: G (R), C (C), DC value (V)
   const double *pulse = nullptr;  // 8 x n_slots: has_pulse, V1, V2, TD, TR, TF, PW, PER
-  const int32_t *sta = nullptr;   // n_slots: capacitor charge state slot
+  const int32_t *sta = nullptr;   // n_slots: capacitor charge state slot,
+                                  // diode store slot (Vd)
+  const double *dpar = nullptr;   // DIO_N x n_slots diode fields
+  double *dvd = nullptr;          // n_slots: each diode's last junction voltage
+  int32_t *limited = nullptr;     // n_slots: limiter / non-finite flags
+  double gmin = 1e-12;
+  int32_t limiter = 1;
   double *slots = nullptr;        // kLinStatements x n_slots (in the shared buffer)
 };
 

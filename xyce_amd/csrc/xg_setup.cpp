@@ -27,6 +27,131 @@ constexpr double kRefTemp = 300.15;
 constexpr double kKoverQ = kBoltz / kQ;
 }  // namespace
 
+// Diode level 1 with RS = 0 (the merged stamp): model defaults of
+// N_DEV_Diode.C:97-247, Model::processParams :1772-1793,
+// Instance::updateTemperature :1332-1470 (breakdown voltage by the
+// default, non-"ORIGINAL" Picard iteration).  TNOM and TEMP are Celsius.
+int diode_setup(const ParamSet &mp, const ParamSet &ip, const Options &opt,
+                double *f, std::string &err) {
+  static const char *kModel[] = {"LEVEL", "IS", "JS", "RS", "N", "TT", "CJO", "CJ",
+                                 "CJ0", "VJ", "M", "EG", "XTI", "FC", "BV", "VB",
+                                 "IBV", "IRF", "TNOM", "KF", "AF"};
+  static const char *kInst[] = {"AREA", "IC", "TEMP", "OFF", "LAMBERTW"};
+  for (auto &kv : mp.v) {
+    bool ok = false;
+    for (const char *k : kModel) ok = ok || kv.first == k;
+    if (!ok) {
+      err = "diode model parameter " + kv.first + " is not modelled here";
+      return XG_E_PARAM;
+    }
+  }
+  for (auto &kv : ip.v) {
+    bool ok = false;
+    for (const char *k : kInst) ok = ok || kv.first == k;
+    if (!ok) {
+      err = "diode instance parameter " + kv.first + " is not modelled here";
+      return XG_E_PARAM;
+    }
+  }
+  if (mp.get("LEVEL", 1.0) != 1.0 || mp.get("RS", 0.0) != 0.0 ||
+      ip.get("LAMBERTW", 0.0) != 0.0) {
+    err = "diode: only level 1, RS = 0, LAMBERTW = 0 are modelled";
+    return XG_E_PARAM;
+  }
+  const double IS = mp.given("IS") ? mp.get("IS", 0) : mp.get("JS", 1.0e-14);
+  const double N = mp.get("N", 1.0);
+  const double TT = mp.get("TT", 0.0);
+  const double CJO = mp.given("CJO") ? mp.get("CJO", 0)
+                     : mp.given("CJ") ? mp.get("CJ", 0) : mp.get("CJ0", 0.0);
+  const double VJ = mp.get("VJ", 1.0);
+  double M = mp.get("M", 0.5), EG = mp.get("EG", 1.11), FC = mp.get("FC", 0.5);
+  const double XTI = mp.get("XTI", 3.0);
+  const bool bv_given = mp.given("BV") || mp.given("VB");
+  const double BV = mp.given("BV") ? mp.get("BV", 0) : mp.get("VB", 1e99);
+  const double IBV = mp.get("IBV", 1.0e-3), IRF = mp.get("IRF", 1.0);
+  const double TNOM = mp.given("TNOM") ? mp.get("TNOM", 0) + 273.15 : opt.tnom;
+  const double Temp = ip.given("TEMP") ? ip.get("TEMP", 0) + 273.15 : opt.temp;
+  const double Area = ip.get("AREA", 1.0);
+  // Model::processParams
+  if (M > 0.9) M = 0.9;
+  if (EG < 0.1) EG = 0.1;
+  if (FC > 0.95) FC = 0.95;
+  const double xfc_m = std::log(1.0 - FC);
+  const double F2 = std::exp((1.0 + M) * xfc_m);
+  const double F3 = 1.0 - FC * (1.0 + M);
+  // Instance::updateTemperature
+  const double Eg0 = 1.16, alphaEg = 7.02e-4, betaEg = 1108.0, Eg300 = 1.1150877;
+  const double vtnom = kKoverQ * TNOM;
+  const double xfc = std::log(1.0 - FC);
+  const double vt = kKoverQ * Temp;
+  const double fact2 = Temp / kRefTemp;
+  const double egfet = Eg0 - (alphaEg * Temp * Temp) / (Temp + betaEg);
+  const double arg = -egfet / (2.0 * kBoltz * Temp) +
+                     Eg300 / (kBoltz * (kRefTemp + kRefTemp));
+  const double pbfact = -2.0 * vt * (1.5 * std::log(fact2) + kQ * arg);
+  const double egfet1 = Eg0 - (alphaEg * TNOM * TNOM) / (TNOM + betaEg);
+  const double arg1 = -egfet1 / (2.0 * kBoltz * TNOM) + Eg300 / (2.0 * kBoltz * kRefTemp);
+  const double fact1 = TNOM / kRefTemp;
+  const double pbfact1 = -2.0 * vtnom * (1.5 * std::log(fact1) + kQ * arg1);
+  const double pbo = (VJ - pbfact1) / fact1;
+  const double gmaold = (VJ - pbo) / pbo;
+  double tJctCap = CJO / (1.0 + M * (4.0e-4 * (TNOM - kRefTemp) - gmaold));
+  const double tJctPot = pbfact + fact2 * pbo;
+  const double gmanew = (tJctPot - pbo) / pbo;
+  tJctCap *= 1.0 + M * (4.0e-4 * (Temp - kRefTemp) - gmanew);
+  const double tSatCur = IS * std::exp(((Temp / TNOM) - 1.0) * EG / (N * vt) +
+                                       XTI / N * std::log(Temp / TNOM));
+  const double tF1 = tJctPot * (1.0 - std::exp((1.0 - M) * xfc)) / (1.0 - M);
+  const double tDepCap = FC * tJctPot;
+  const double vte = N * vt;
+  const double tVcrit = vte * std::log(vte / (std::sqrt(2.0) * tSatCur));
+  const double tIRF = IRF * std::pow(fact2, 1.6);
+  double tBrkdwnV = 0.0;
+  if (bv_given) {
+    const double e = std::exp(1.0);
+    const double cbv = IBV;
+    double a1 = 3.0 * vte / (e * BV);
+    a1 = a1 * a1 * a1;
+    const double lo = tSatCur * tIRF * (1 - a1);
+    const double hi = tSatCur * tIRF * (1 - 1.0 / (e * e * e)) *
+                      std::exp(-1.0 * (3.0 * vte - BV) / vte);
+    if (cbv >= hi) {
+      tBrkdwnV = 3.0 * vte;
+    } else if (cbv <= lo) {
+      tBrkdwnV = BV;
+    } else {
+      double xbv = BV;
+      for (int it = 0; it < 8; ++it) {
+        double a2 = 3.0 * vte / (e * xbv);
+        a2 = a2 * a2 * a2;
+        xbv = BV - vte * std::log(cbv / (tSatCur * tIRF)) + vte * std::log(1 - a2);
+      }
+      tBrkdwnV = xbv;
+    }
+  }
+  f[DIO_ISAT] = tSatCur * Area;
+  f[DIO_VTE] = vte;
+  f[DIO_VCRIT] = tVcrit;
+  f[DIO_IRF] = tIRF;
+  f[DIO_BRKV] = tBrkdwnV;
+  f[DIO_BVGIVEN] = bv_given ? 1.0 : 0.0;
+  f[DIO_BV] = BV;
+  f[DIO_TJCAP] = tJctCap;
+  f[DIO_CZERO] = tJctCap * Area;
+  f[DIO_TJPOT] = tJctPot;
+  f[DIO_TDEPCAP] = tDepCap;
+  f[DIO_TF1] = tF1;
+  f[DIO_M] = M;
+  f[DIO_F2] = F2;
+  f[DIO_F3] = F3;
+  f[DIO_TT] = TT;
+  f[DIO_GSPR] = 0.0;  // tCOND * AREA with COND = 0 for RS = 0
+  f[DIO_OFF] = ip.get("OFF", 0.0) != 0.0 ? 1.0 : 0.0;
+  f[DIO_HASIC] = ip.given("IC") ? 1.0 : 0.0;
+  f[DIO_IC] = ip.get("IC", 0.0);
+  return XG_OK;
+}
+
 Circuit::~Circuit() { bsim3_free(*this); }
 Plan::~Plan() { bsim3_plan_free(*this); }
 
@@ -452,7 +577,8 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
       entries.push_back({b, p});
       entries.push_back({b, q});
     } else {
-      // N_DEV_Resistor.C / N_DEV_Capacitor.C: the full 2 x 2 block
+      // N_DEV_Resistor.C / N_DEV_Capacitor.C, and N_DEV_Diode.C with RS = 0
+      // (internal node merged into the positive one): the full 2 x 2 block
       entries.push_back({p, p});
       entries.push_back({p, q});
       entries.push_back({q, p});
@@ -617,7 +743,7 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
   }
   slot_total += bsim3_slot_count(plan);
 
-  // ---- linear devices, after the MOSFETs; resistors, then capacitors,
+  // ---- linear devices, after the MOSFETs; resistors, capacitors, diodes,
   //      then sources (one Master loop per device type), each in the order
   //      they were added, each type padded to whole waves
   {
@@ -643,6 +769,7 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     plan.lin_sta.assign(SL, 0);
     plan.lin_val.assign(SL, 0.0);
     plan.lin_pulse.assign((size_t)8 * SL, 0.0);
+    plan.lin_dpar.assign((size_t)DIO_N * SL, 0.0);
     auto LS = [&](int e, int32_t s) -> int32_t {
       return (int32_t)(slot_total + (int64_t)e * SL + s);
     };
@@ -660,12 +787,26 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
       plan.lin_sta[s] = ld.sta;
       plan.lin_val[s] = ld.type == LIN_R ? 1.0 / ld.val : ld.val;
       for (int k = 0; k < 8; ++k) plan.lin_pulse[(size_t)k * SL + s] = ld.pulse[k];
+      if (ld.type == LIN_D)
+        for (int k = 0; k < DIO_N; ++k) plan.lin_dpar[(size_t)k * SL + s] = ld.dpar[k];
       if (ld.type == LIN_R) {          // N_DEV_Resistor.C:1118-1131
         vec[0].push_back({p, LS(0, s)});
         vec[0].push_back({q, LS(1, s)});
       } else if (ld.type == LIN_C) {   // N_DEV_Capacitor.C:1881-1960
         vec[1].push_back({p, LS(0, s)});
         vec[1].push_back({q, LS(1, s)});
+      } else if (ld.type == LIN_D) {   // N_DEV_Diode.C:2040-2067 (Pri = Pos)
+        vec[0].push_back({p, LS(0, s)});
+        vec[0].push_back({q, LS(1, s)});
+        vec[0].push_back({p, LS(2, s)});
+        vec[1].push_back({q, LS(3, s)});
+        vec[1].push_back({p, LS(4, s)});
+        if (c.opt.limiter) {
+          vec[2].push_back({q, LS(5, s)});
+          vec[2].push_back({p, LS(6, s)});
+          vec[3].push_back({q, LS(7, s)});
+          vec[3].push_back({p, LS(8, s)});
+        }
       } else {                         // N_DEV_Vsrc.C:1002-1013
         vec[0].push_back({p, LS(0, s)});
         vec[0].push_back({q, LS(1, s)});
@@ -678,6 +819,23 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
       if (i < 0) continue;
       const LinDev &ld = c.lin[i];
       const int32_t p = L(ld.lids[0]), q = L(ld.lids[1]), b = L(ld.lids[2]);
+      if (ld.type == LIN_D) {
+        // N_DEV_Diode.C:2094-2108 with Pri = Pos: dF/dx (P,P) +Gspr,
+        // (P,P) -Gspr, (N,N) +Gd, (N,P) -Gd, (P,P) -Gspr, (P,N) -Gd,
+        // (P,P) +Gspr+Gd; dQ/dx (N,N) +Cd, (N,P) -Cd, (P,N) -Cd, (P,P) +Cd
+        const int32_t tf[7][2] = {{p, p}, {p, p}, {q, q}, {q, p}, {p, p}, {p, q}, {p, p}};
+        const int32_t tq[4][2] = {{q, q}, {q, p}, {p, q}, {p, p}};
+        for (int k = 0; k < 11; ++k) {
+          const int32_t *e = k < 7 ? tf[k] : tq[k - 7];
+          int32_t pos;
+          if (!csr_pos(e[0], e[1], pos)) {
+            err = "supplied CSR pattern lacks a diode stamp entry";
+            return XG_E_PATTERN;
+          }
+          (k < 7 ? mat[0] : mat[1]).push_back({pos, LS(9 + k, s)});
+        }
+        continue;
+      }
       int32_t rc_[4][2];
       if (ld.type == LIN_V) {          // N_DEV_Vsrc.C:1023-1036
         const int32_t t[4][2] = {{p, b}, {q, b}, {b, p}, {b, q}};
@@ -766,6 +924,13 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
       node_of.push_back(node);
     }
     bsim3_slot_nodes(plan, G, node_of);
+    // linear-device slots (only diodes flag limiting)
+    for (int32_t s = 0; s < plan.lin_slots; ++s) {
+      const size_t S = (size_t)plan.lin_slots;
+      int32_t node = plan.lin_lid[s] != G ? plan.lin_lid[s] : plan.lin_lid[S + s];
+      if (plan.lin_type[s] < 0 || node == G) node = -1;
+      node_of.push_back(node);
+    }
     plan.slot_comp.assign(node_of.size(), -1);
     for (size_t s = 0; s < node_of.size(); ++s)
       if (node_of[s] >= 0) plan.slot_comp[s] = id[find(node_of[s])];

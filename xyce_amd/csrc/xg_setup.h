@@ -56,11 +56,12 @@ struct Mos1Inst {
 
 // A linear device as added through the C ABI.
 struct LinDev {
-  int32_t type;        // LIN_R / LIN_C / LIN_V
+  int32_t type;        // LIN_R / LIN_C / LIN_D / LIN_V
   int32_t lids[3];     // pos, neg, branch (-1 = ground)
   double val;          // R (ohm), C (farad) or DC value (volt)
   double pulse[8];     // has_pulse, V1, V2, TD, TR, TF, PW, PER
-  int32_t sta;         // capacitor charge state slot
+  int32_t sta;         // capacitor charge state slot / diode store slot
+  double dpar[DIO_N];  // diode fields
 };
 
 struct ForeignStamp {
@@ -149,7 +150,7 @@ struct Plan {
   // linear devices (eval order: resistors, capacitors, sources)
   int32_t lin_n = 0, lin_slots = 0;
   std::vector<int32_t> lin_type, lin_lid, lin_sta;
-  std::vector<double> lin_val, lin_pulse;
+  std::vector<double> lin_val, lin_pulse, lin_dpar;
   int64_t lin_slot_base = 0;
   // slot buffer and reduction programs
   int64_t n_slot_values = 0;
@@ -180,6 +181,10 @@ int mos1_model_process(Mos1Model &m, const Options &opt, std::string &err);
 int mos1_instance_setup(const Mos1Model &m, Mos1Inst &in, const Options &opt,
                         std::string &err);
 int build_plan(Circuit &c, Plan &plan, std::string &err);
+// Diode fields from .MODEL and instance parameters (level 1, no series
+// resistance); XG_E_PARAM for what the load does not model.
+int diode_setup(const ParamSet &model, const ParamSet &inst, const Options &opt,
+                double *fields, std::string &err);
 int lu_analyse(Plan &plan, std::string &err);
 
 // BSIM3 hooks (xg_bsim3_setup.cpp).

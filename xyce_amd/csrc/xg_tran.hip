@@ -420,6 +420,16 @@ int xg_tran_info(xg_tran *T, double *t, int64_t *counters) {
   return XG_OK;
 }
 
+int xg_tran_restore(xg_tran *T, double time, int64_t n_steps, int64_t newton_total,
+                    int64_t step_failures) {
+  if (!T || n_steps < 0) return XG_E_ARG;
+  T->t = time;
+  T->n_steps = n_steps;
+  T->newton_total = newton_total;
+  T->step_failures = step_failures;
+  return XG_OK;
+}
+
 int xg_tran_sync(xg_tran *T) {
   if (!T) return XG_E_ARG;
   return hipStreamSynchronize(T->stream) == hipSuccess ? XG_OK : XG_E_HIP;

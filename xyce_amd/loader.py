@@ -43,6 +43,8 @@ class DeviceLoad:
                   "xg_circuit_set_options")
             mid = {}
             for name, m in nl.models.items():
+                if m.kind == "diode":
+                    continue
                 n, k, v = keys_vals(m.params)
                 out = ctypes.c_int32()
                 fn = L.xg_add_mos1_model if m.kind == "mos1" else L.xg_add_bsim3_model
@@ -67,6 +69,10 @@ class DeviceLoad:
                     check(L.xg_add_resistor(c, p, q, val), "add resistor")
                 elif kind == "C":
                     check(L.xg_add_capacitor(c, p, q, val, st), "add capacitor")
+                elif kind == "D":
+                    nm, km, vm = keys_vals(val[0])
+                    ni, ki, vi = keys_vals(val[1])
+                    check(L.xg_add_diode(c, p, q, nm, km, vm, ni, ki, vi, st), "add diode")
                 else:
                     pp = (ctypes.c_double * 7)(*pulse) if pulse else None
                     check(L.xg_add_vsource(c, p, q, br, val, pp), "add source")

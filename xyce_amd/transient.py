@@ -333,3 +333,46 @@ class Transient:
         check(L.xg_tran_sync(self._tr), "sync")
         self._sync_counters()
         return np.array(times), out[:k].T.cpu().numpy()
+
+    # ---- checkpoint / restart (N_IO_RestartMgr's role for a transient)
+    _CKPT = ("x_curr", "xh1", "xh2", "sta_c", "sto_c", "qn0", "fprev")
+
+    def checkpoint(self, path):
+        """Writes the accepted state (solution and its history, state and
+        store vectors, charge and F history, device-held instance state,
+        time and counters) to an .npz file."""
+        if self._tr is not None:
+            check(lib().xg_tran_sync(self._tr), "sync")
+            self._sync_counters()
+        self.torch.cuda.synchronize()
+        n = ctypes.c_int64()
+        check(lib().xg_instance_state(self.dev.h, 0, None, ctypes.byref(n)), "instance state")
+        inst = np.zeros(max(n.value, 1))
+        check(lib().xg_instance_state(self.dev.h, 1, inst.ctypes.data, ctypes.byref(n)),
+              "instance state")
+        arrays = {k: getattr(self, k).cpu().numpy() for k in self._CKPT}
+        np.savez(path, instance=inst[:n.value], t=self.t, n_steps=self.n_steps,
+                 newton_total=self.newton_total, step_failures=self.step_failures,
+                 sizes=np.array([self.dev.nnz, len(self.x), len(self.sta_c), len(self.sto_c)]),
+                 **arrays)
+
+    def restore(self, path):
+        """Loads a checkpoint of the same circuit; the next step continues
+        exactly where the checkpointed run was."""
+        d = np.load(path)
+        sizes = [self.dev.nnz, len(self.x), len(self.sta_c), len(self.sto_c)]
+        if list(d["sizes"]) != sizes:
+            raise ValueError("checkpoint is of another circuit")
+        for k in self._CKPT:
+            getattr(self, k).copy_(self.torch.from_numpy(d[k]))
+        inst = np.ascontiguousarray(d["instance"], dtype=np.float64)
+        n = ctypes.c_int64(len(inst))
+        if len(inst):
+            check(lib().xg_instance_state(self.dev.h, 2, inst.ctypes.data, ctypes.byref(n)),
+                  "instance state")
+        self.t, self.n_steps = float(d["t"]), int(d["n_steps"])
+        self.newton_total, self.step_failures = int(d["newton_total"]), int(d["step_failures"])
+        self.torch.cuda.synchronize()
+        if self._tr is not None:
+            check(lib().xg_tran_restore(self._tr, self.t, self.n_steps, self.newton_total,
+                                        self.step_failures), "restore")
