@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--mc-tstop", type=float, default=0.5e-9)
     ap.add_argument("--mc-dt", type=float, default=1e-12)
     ap.add_argument("--no-mc", action="store_true", help="skip the batched-transient leg")
+    ap.add_argument("--mc-only", action="store_true",
+                    help="run only the batched-transient leg and print its result "
+                         "(for profiling that leg alone)")
     ap.add_argument("--no-grid", action="store_true",
                     help="skip the power-grid (100k inverter) load timing")
     ap.add_argument("--copy-calib", action="store_true",
@@ -116,6 +119,18 @@ def batched_transient_leg(torch, args):
     b = run_batched(parts, args.mc_tstop, args.mc_dt, probes, every=10, sync=sync)
     progress("batched transient: %.3f s; plain form on %d samples"
              % (b["seconds"], args.mc_plain))
+    res = {
+        "workload": "%d Monte-Carlo BSIM3 ring oscillators, 11-301 stages, %d FETs; "
+                    ".TRAN %.3g s at %.3g s steps after the DC operating point"
+                    % (len(parts), sum(len(q.mos) for q in parts), args.mc_tstop, args.mc_dt),
+        "solves_per_s": len(parts) / b["seconds"],
+        "batch_latency_s": b["seconds"],
+        "batch_newton_iterations": int(b["newton"]),
+        "batch_step_failures": int(b["step_failures"]),
+        "batch_setup_s": b["setup_s"],
+    }
+    if args.mc_plain <= 0:
+        return res
     pick = np.unique(np.linspace(0, len(parts) - 1, args.mc_plain).round().astype(int))
     p = run_plain([parts[k] for k in pick], args.mc_tstop, args.mc_dt, probes, every=10,
                   sync=sync)
@@ -253,6 +268,11 @@ def main():
     import torch
     from xyce_amd import circuits
     from xyce_amd.loader import DeviceLoad
+
+    if args.mc_only:
+        print(json.dumps({"batched_transient": batched_transient_leg(torch, args)}),
+              flush=True)
+        return
 
     progress("building the %d-gate netlist" % args.gates)
     nl = circuits.random_logic_bsim3(n_gates=args.gates, seed=2)
