@@ -122,7 +122,7 @@ class Transient:
                 self.sto_c, self.sto_n, self.F, self.Q, self.FL, self.QL, self.JF, self.JQ,
                 self.J, self.rhs, self.qn0, self.fprev, self.stats, self.nstats, self.lim,
                 self.active, self.conv, self.counts, self.ctl, self.stepv)])
-            torch.cuda.synchronize()
+            torch.cuda.current_stream().synchronize()
             tr = ctypes.c_void_p()
             check(lib().xg_tran_create(self.dev.h, ctypes.byref(b), bp_tol or 0.0,
                                        ctypes.byref(tr)), "transient driver")
@@ -313,7 +313,7 @@ class Transient:
         n = int(round(tstop / dt))
         out = torch.zeros(n // every + 1, len(probes), dtype=torch.float64,
                           device=self.x.device)
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
         self.dcop()
         check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes), out[0].data_ptr()),
               "probe")
