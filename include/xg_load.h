@@ -320,6 +320,17 @@ int xg_lu_export(const xg_handle *h, int64_t sizes[6], int32_t *const *arrays,
    1: copy to buf; 2: copy from buf (*n must match).  Synchronises. */
 int xg_instance_state(xg_handle *h, int32_t mode, double *buf, int64_t *n);
 
+/* Component activity for the loads: when active (device array, one flag per
+   component of xg_components) is not NULL, xg_update_state skips the
+   instances of components whose flag is 0 -- their statement values,
+   state and store keep what their last evaluation wrote, so the loads
+   reproduce that evaluation's sums.  A Newton loop over independent
+   sub-circuits passes its convergence mask, so a converged sub-circuit is
+   not re-evaluated while others iterate (re-evaluation would move a
+   junction limiter that the iteration limit let converge).  NULL: every
+   instance.  No counterpart in the reference. */
+int xg_set_active_mask(xg_handle *h, const int32_t *active);
+
 /* Step parameters from device memory (for graph-captured step loops, whose
    kernel arguments are fixed at capture): when dstep is not NULL, the
    linear sources read time = dstep[0] and breakpoint tolerance = dstep[1],
@@ -392,6 +403,13 @@ int xg_tran_create(xg_handle *h, const xg_tran_buffers *b, double bp_tol,
    history starts there.  > 0: Newton failure (1 singular Jacobian,
    2 non-finite device values, 3 no convergence). */
 int xg_tran_dcop(xg_tran *t);
+/* Start without an operating point (.TRAN ... UIC / NOOP): the solution,
+   state and store start at zero, and one load there starts the charge
+   and F history.  For circuits whose sources are all zero at t = 0 this
+   is the DC solution itself, reached without the DC Newton solve (which
+   has no continuation here, unlike the reference's gmin / source
+   stepping). */
+int xg_tran_uic(xg_tran *t);
 /* One step of dt (order 1 for the first two steps, then 2); a step whose
    Newton solve fails is replaced by two of dt/2, recursively. */
 int xg_tran_step(xg_tran *t, double dt);

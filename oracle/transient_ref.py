@@ -167,6 +167,24 @@ class RefTransient:
         self.t = 0.0
         self.n_steps = 0
 
+    def uic(self):
+        """Zero start without an operating point: one load at x = 0 with
+        operating-point flags (no junction initialisation) starts the
+        history."""
+        self.x[:] = 0.0
+        self.sta_c[:] = 0.0
+        self.sto_c[:] = 0.0
+        self._load(1, False, dcop=True, t=0.0)
+        self.x_curr = self.x.copy()
+        self.sta_c[:] = self.sta_n
+        self.sto_c[:] = self.sto_n
+        self.qn0 = self.Q.copy()
+        self.fprev = self.F.copy()
+        self.xh1[:] = 0.0
+        self.xh2[:] = 0.0
+        self.t = 0.0
+        self.n_steps = 0
+
     def step(self, dt):
         order = 2 if self.n_steps >= 2 else 1
         t_next = self.t + dt
@@ -195,9 +213,9 @@ class RefTransient:
         self.t = t_next
         self.n_steps += 1
 
-    def run(self, tstop, dt, probes, every=1):
+    def run(self, tstop, dt, probes, every=1, uic=False):
         idx = [self.nl.node(p) for p in probes]
-        self.dcop()
+        self.uic() if uic else self.dcop()
         n = int(round(tstop / dt))
         times, out = [0.0], [self.x_curr[idx].copy()]
         for s in range(1, n + 1):

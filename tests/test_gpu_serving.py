@@ -26,10 +26,13 @@ def test_mixed_stream_batches_and_matches_alone():
         if r.kind in checked:
             continue
         checked.add(r.kind)
+        from xyce_amd.serving import UIC
         alone = Transient(r._build())
-        t, w = alone.run(r.tstop, r.dt, r.probes, every=10)
+        t, w = alone.run(r.tstop, r.dt, r.probes, every=10, uic=UIC[r.kind])
         alone.close()
         tb, wb = r.result
+        print("%s: batch %s, alone %d step failures, max difference %.3g V"
+              % (r.kind, r.batch_info, alone.step_failures, np.abs(w - wb).max()), flush=True)
         assert np.array_equal(t, tb)
         vmax = 1.2 if r.kind != "ladder" else 1.0
         assert np.abs(w - wb).max() <= 1e-3 * vmax + 1e-6, r.kind

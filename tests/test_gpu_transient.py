@@ -327,3 +327,25 @@ def test_checkpoint_restart_continues_bit_for_bit(tmp_path):
     assert b.x_curr.cpu().numpy().tobytes() == full.x_curr.cpu().numpy().tobytes()
     assert b.sto_c.cpu().numpy().tobytes() == full.sto_c.cpu().numpy().tobytes()
     assert b.newton_total == full.newton_total
+
+
+def test_uic_start_logic_parity():
+    """A random-logic transient started from zero (.TRAN UIC, supply
+    ramping up) on the device against the oracle, and the two drivers bit
+    for bit."""
+    from oracle.transient_ref import RefTransient
+    from xyce_amd import circuits
+    from xyce_amd.transient import Transient
+    mk = lambda: circuits.random_logic_bsim3(n_gates=40, seed=9, ramp=50e-12)
+    g = Transient(mk())
+    tg, wg = g.run(0.1e-9, 1e-12, ["g0", "g5"], every=5, uic=True)
+    p = Transient(mk(), driver="python")
+    tp, wp = p.run(0.1e-9, 1e-12, ["g0", "g5"], every=5, uic=True)
+    r = RefTransient(mk())
+    tr_, wr = r.run(0.1e-9, 1e-12, ["g0", "g5"], every=5, uic=True)
+    err = np.abs(wg - wr).max()
+    print("logic UIC: max waveform difference %.3g V, Newton %d / %d"
+          % (err, g.newton_total, r.newton_total), flush=True)
+    assert np.allclose(tg, tr_) and np.array_equal(tg, tp)
+    assert wg.tobytes() == wp.tobytes()
+    assert err <= 1e-3 * 1.2 + 1e-6

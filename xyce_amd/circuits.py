@@ -409,18 +409,22 @@ def ring_oscillator_bsim3(stages=11, seed=3, vdd=1.2, dvth0=0.0, dtox=0.0, ramp=
     return nl
 
 
-def random_logic_bsim3(n_gates=5000, seed=2, vdd=1.2, n_inputs=64):
+def random_logic_bsim3(n_gates=5000, seed=2, vdd=1.2, n_inputs=64, ramp=0.0):
     """Seeded random CMOS logic: n_gates 2-input NAND / NOR gates (50/50),
     i.e. 4 * n_gates BSIM3 transistors.  Each gate output fans out to 1-4
     gate inputs; W and L are drawn independently from U[0.8, 1.2] x
     (0.5u, 0.13u); every net is an RC line of 1-3 segments with R from
     U[10, 200] ohm and C from U[1, 20] fF per segment.  Gate inputs that no
-    output reaches are driven by n_inputs primary-input sources."""
+    output reaches are driven by n_inputs primary-input sources.  ramp > 0:
+    the supply rises from 0 in `ramp` seconds (a trivial DC operating point
+    for circuits whose feedback paths leave the DC problem without a
+    Newton-reachable solution)."""
     rng = np.random.default_rng(seed)
     nl = Netlist("random BSIM3 logic, %d gates" % n_gates)
     nl.add_model("NB", "bsim3", 1, **BSIM3_NMOS_130)
     nl.add_model("PB", "bsim3", -1, **BSIM3_PMOS_130)
-    nl.add_vsrc("VDD", "vdd", "0", vdd)
+    sv = supply(vdd, ramp)
+    nl.add_vsrc("VDD", "vdd", "0", sv["dc"], sv.get("pulse"))
     for k in range(n_inputs):
         nl.add_vsrc("VIN%d" % k, "in%d" % k, "0", 0.0,
                     Pulse(0.0, vdd, float(rng.uniform(0, 1e-9)), 50e-12, 50e-12,

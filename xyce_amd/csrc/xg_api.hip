@@ -36,6 +36,7 @@ struct Handle {
   LinArgs lin{};
   double time = 0.0, bp_tol = 0.0;
   const double *dstep = nullptr;  // device {time, bp_tol, 1/dt, order}
+  const int32_t *active = nullptr;  // component activity (xg_set_active_mask)
   // per-component dense solve, and the static-pivot sparse LU
   DenseSolveArgs ds{};
   SparseLuArgs lu{};
@@ -514,6 +515,12 @@ int xg_instance_state(xg_handle *H, int32_t mode, double *buf, int64_t *n) {
   return e == hipSuccess ? XG_OK : hip_fail(e, "instance state copy");
 }
 
+int xg_set_active_mask(xg_handle *H, const int32_t *active) {
+  if (!H) return fail(XG_E_ARG, "xg_set_active_mask: bad argument");
+  H->h.active = active;
+  return XG_OK;
+}
+
 int xg_set_step_source(xg_handle *H, const double *dstep) {
   if (!H) return fail(XG_E_ARG, "xg_set_step_source: bad argument");
   H->h.dstep = dstep;
@@ -846,10 +853,14 @@ int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
       (h.plan.n_store > 0 && (!sto_curr || !sto_next)))
     return fail(XG_E_ARG, "xg_update_state: null pointer");
   xg::LoadPtrs p{x_next, sta_curr, sto_curr, sta_next, sto_next};
+  p.active = h.active;
+  p.comp = h.slot_comp;
   hipError_t e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
   if (e != hipSuccess) return hip_fail(e, "mos1 eval");
+  p.comp = h.slot_comp + h.plan.m1_slots;
   e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream);
   if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
+  p.comp = h.slot_comp + h.plan.m1_slots + xg::bsim3_eval_slots(h.plan);
   e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.dstep, h.stream);
   if (e != hipSuccess) return hip_fail(e, "linear eval");
   return XG_OK;

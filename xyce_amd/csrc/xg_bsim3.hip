@@ -1460,6 +1460,7 @@ __global__ void __launch_bounds__(kBlock)
   __syncthreads();
   if (slot >= S) return;
   if (a.inst_of[slot] < 0) return;
+  if (skip_slot(p, slot)) return;
   B3Inputs in;
   b3_inputs(a, ss, p, slot, in);
   b3_eval<false, kFull>(a, ss, p, slot, card_lds[wave], a.sdp + a.sdp_idx[slot],
@@ -1507,7 +1508,7 @@ __global__ void __launch_bounds__(kSplit ? 128 : 64)
   }
   if (threadIdx.x < B3C_N) card[threadIdx.x] = a.cards[a.wave_card[w] * B3C_N + threadIdx.x];
   __syncthreads();
-  if (!live) return;
+  if (!live || skip_slot(p, slot)) return;
   const double *R = rec + lane, *Gp = rec + B3R_N * 64 + lane;
   if constexpr (kSplit) {
     if (wave)

@@ -173,7 +173,19 @@ struct LoadPtrs {
   const double *sto_curr;
   double *sta_next;
   double *sto_next;
+  // optional: component activity (xg_set_active_mask) and the component of
+  // each eval slot of the kernel's device type; instances of inactive
+  // components are not evaluated (their statement values, state and store
+  // keep what the last evaluation wrote)
+  const int32_t *active = nullptr;
+  const int32_t *comp = nullptr;
 };
+
+__device__ __forceinline__ bool skip_slot(const LoadPtrs &p, int slot) {
+  if (!p.active) return false;
+  const int32_t c = p.comp[slot];
+  return c >= 0 && !p.active[c];
+}
 
 // Kernel launchers (defined in the .hip files).
 hipError_t launch_mos1_eval(const Mos1Args &a, const SolverStateDev &ss,

@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(256)
   const int S = a.n_slots;
   if (s >= S) return;
   const int32_t type = a.type[s];
-  if (type < 0) return;
+  if (type < 0 || skip_slot(p, s)) return;
   const double vp = p.x[a.lid[s]], vn = p.x[a.lid[S + s]];
   const double val = a.val[s];
   double *out = a.slots + s;

@@ -120,6 +120,7 @@ __global__ void __launch_bounds__(kBlock)
   __syncthreads();
   if (slot >= S) return;
   if (a.inst_of[slot] < 0) return;
+  if (skip_slot(p, slot)) return;
   const double *cd = card_lds[wave];
   const double dtype = cd[M1C_DTYPE];
   const double gamma = cd[M1C_GAMMA];

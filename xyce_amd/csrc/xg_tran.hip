@@ -30,8 +30,11 @@ struct xg_tran {
   double t = 0.0, bp_tol = 0.0;
   int64_t n_steps = 0, newton_total = 0, step_failures = 0;
   int32_t use_graphs = 1;
-  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};  // DC, first step, later
-  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  // graphs: 0 DC iteration, 1 first-step iteration, 2 later-step iteration,
+  // 3 / 4 a whole first / later step's first load, first iteration and
+  // (when that converged) completion
+  hipGraphExec_t exec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipGraph_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int32_t *ctl_host = nullptr;  // pinned copy of the control words
   std::string err;
 };
@@ -158,12 +161,15 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Accepted step (N_TIA_OneStep::completeStep history and state rotation).
+// With ctl given, only when the Newton solve has converged (the fused step
+// graph runs it right after the first iteration).
 __global__ void __launch_bounds__(kBlock)
     tran_complete_kernel(int64_t n1, int64_t n_state, int64_t n_store, const double *x,
                          double *x_curr, double *xh1, double *xh2, const double *Q,
                          const double *F, double *qn0, double *fprev,
                          const double *sta_n, double *sta_c, const double *sto_n,
-                         double *sto_c, int32_t dc) {
+                         double *sto_c, int32_t dc, const int32_t *ctl) {
+  if (ctl && !ctl[kDone]) return;
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n1) {
     if (dc) {
@@ -252,52 +258,79 @@ int enqueue_iteration(xg_tran *T, int variant) {
   return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
 }
 
-int launch_iteration(xg_tran *T, int variant) {
-  if (!T->use_graphs) return enqueue_iteration(T, variant);
-  if (!T->exec[variant]) {
-    if (hipStreamBeginCapture(T->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
-      return tfail(T, XG_E_HIP, "graph capture begin");
-    const int rc = enqueue_iteration(T, variant);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(T->stream, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    if (e != hipSuccess) return tfail(T, XG_E_HIP, "graph capture end");
-    T->graph[variant] = g;
-    if (hipGraphInstantiate(&T->exec[variant], g, nullptr, nullptr, 0) != hipSuccess)
-      return tfail(T, XG_E_HIP, "graph instantiate");
-  }
-  return hipGraphLaunch(T->exec[variant], T->stream) == hipSuccess ? XG_OK : XG_E_HIP;
-}
-
-// Newton loop after the first load: returns 0 converged, >0 the failure
-// word, <0 an error code.
-int newton_loop(xg_tran *T, int variant, int32_t *iters) {
-  for (;;) {
-    int rc = launch_iteration(T, variant);
-    if (rc) return rc < 0 ? rc : XG_E_HIP;
-    if (hipMemcpyAsync(T->ctl_host, T->b.ctl, sizeof(int32_t) * xg::kCtlWords,
-                       hipMemcpyDeviceToHost, T->stream) != hipSuccess ||
-        hipStreamSynchronize(T->stream) != hipSuccess)
-      return tfail(T, XG_E_HIP, "control word copy");
-    *iters = T->ctl_host[xg::kIter];
-    if (T->ctl_host[xg::kFail]) return T->ctl_host[xg::kFail];
-    if (T->ctl_host[xg::kDone]) return 0;
-  }
-}
-
-int complete(xg_tran *T, int dc) {
+int enqueue_complete(xg_tran *T, int dc, const int32_t *ctl) {
   const xg_tran_buffers &b = T->b;
   const int64_t n = std::max(T->n1, std::max(T->n_state, T->n_store));
   hipLaunchKernelGGL(xg::tran_complete_kernel, dim3(xg::grid_for(n)), dim3(xg::kBlock), 0,
                      T->stream, T->n1, T->n_state, T->n_store, (const double *)b.x,
                      b.x_curr, b.xh1, b.xh2, (const double *)b.Q, (const double *)b.F,
                      b.qn0, b.fprev, (const double *)b.sta_n, b.sta_c,
-                     (const double *)b.sto_n, b.sto_c, dc);
+                     (const double *)b.sto_n, b.sto_c, dc, ctl);
   return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
 }
+
+// Variants 0-2: one Newton iteration; 3 / 4: a first / later step's first
+// load and assembly, its first iteration, and the completion if that
+// iteration converged (one launch for the common one-iteration step).
+int enqueue_variant(xg_tran *T, int variant) {
+  if (variant < 3) return enqueue_iteration(T, variant);
+  const int first = variant == 3;
+  int rc = enqueue_first_load(T, 0, first);
+  if (rc) return rc;
+  rc = enqueue_iteration(T, first ? 1 : 2);
+  if (rc) return rc;
+  return enqueue_complete(T, 0, T->b.ctl);
+}
+
+int capture_variant(xg_tran *T, int variant) {
+  if (T->exec[variant]) return XG_OK;
+  if (hipStreamBeginCapture(T->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return tfail(T, XG_E_HIP, "graph capture begin");
+  const int rc = enqueue_variant(T, variant);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(T->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return tfail(T, XG_E_HIP, "graph capture end");
+  T->graph[variant] = g;
+  if (hipGraphInstantiate(&T->exec[variant], g, nullptr, nullptr, 0) != hipSuccess)
+    return tfail(T, XG_E_HIP, "graph instantiate");
+  return XG_OK;
+}
+
+int launch_iteration(xg_tran *T, int variant) {
+  if (!T->use_graphs) return enqueue_variant(T, variant);
+  const int rc = capture_variant(T, variant);
+  if (rc) return rc;
+  return hipGraphLaunch(T->exec[variant], T->stream) == hipSuccess ? XG_OK : XG_E_HIP;
+}
+
+int read_ctl(xg_tran *T) {
+  if (hipMemcpyAsync(T->ctl_host, T->b.ctl, sizeof(int32_t) * xg::kCtlWords,
+                     hipMemcpyDeviceToHost, T->stream) != hipSuccess ||
+      hipStreamSynchronize(T->stream) != hipSuccess)
+    return tfail(T, XG_E_HIP, "control word copy");
+  return XG_OK;
+}
+
+// Newton iterations until convergence: returns 0 converged, >0 the failure
+// word, <0 an error code.  `first_variant`, if not negative, is launched
+// first instead of a plain iteration (the fused step graph).
+int newton_loop(xg_tran *T, int variant, int32_t *iters, int first_variant = -1) {
+  for (int k = 0;; ++k) {
+    int rc = launch_iteration(T, k == 0 && first_variant >= 0 ? first_variant : variant);
+    if (rc) return rc < 0 ? rc : XG_E_HIP;
+    rc = read_ctl(T);
+    if (rc) return rc;
+    *iters = T->ctl_host[xg::kIter];
+    if (T->ctl_host[xg::kFail]) return T->ctl_host[xg::kFail];
+    if (T->ctl_host[xg::kDone]) return 0;
+  }
+}
+
+int complete(xg_tran *T, int dc) { return enqueue_complete(T, dc, nullptr); }
 
 }  // namespace
 
@@ -334,6 +367,17 @@ int xg_tran_create(xg_handle *h, const xg_tran_buffers *b, double bp_tol,
   (void)hipStreamSynchronize(T->old_stream);
   xg_set_stream(h, T->stream);
   xg_set_step_source(h, b->step);
+  xg_set_active_mask(h, b->active);
+  // every graph is captured here, so that the run itself makes no capture:
+  // a capture must not overlap another thread's device allocations or
+  // synchronous copies, and creation is where a caller can serialise those
+  for (int v = 0; v < 5; ++v) {
+    const int rc = capture_variant(T, v);
+    if (rc) {
+      xg_tran_destroy(T);
+      return rc;
+    }
+  }
   *out = T;
   return XG_OK;
 }
@@ -366,6 +410,28 @@ int xg_tran_dcop(xg_tran *T) {
   return XG_OK;
 }
 
+int xg_tran_uic(xg_tran *T) {
+  if (!T) return XG_E_ARG;
+  const xg_tran_buffers &b = T->b;
+  const int64_t n = std::max(T->n1, std::max(T->n_state, std::max(T->n_store,
+                                                                  (int64_t)T->n_comp)));
+  hipLaunchKernelGGL(xg::dc_begin_kernel, dim3(xg::grid_for(n)), dim3(xg::kBlock), 0,
+                     T->stream, T->n1, T->n_state, T->n_store, T->n_comp, b.x, b.x_curr,
+                     b.sta_c, b.sto_c, b.step, T->bp_tol, b.conv, b.active, b.ctl);
+  if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "uic begin");
+  // one load at x = 0 with the operating-point flags but no junction
+  // initialisation, to start the charge and F history there
+  set_state(T->h, 1, 0, 0, 1, 0);
+  const int rc = xg_load_all(T->h, b.x, b.sta_c, b.sto_c, b.sta_n, b.sto_n, b.F, b.Q,
+                             b.FL, b.QL, b.JF, b.JQ);
+  if (rc) return rc;
+  const int rc2 = complete(T, 1);
+  if (rc2) return rc2;
+  T->t = 0.0;
+  T->n_steps = 0;
+  return XG_OK;
+}
+
 int xg_tran_step(xg_tran *T, double dt) {
   if (!T || !(dt > 0.0)) return XG_E_ARG;
   const int order = T->n_steps >= 2 ? 2 : 1;
@@ -380,10 +446,9 @@ int xg_tran_step(xg_tran *T, double dt) {
                      (const double *)b.xh1, (const double *)b.xh2, b.xpred, b.x, b.step,
                      t_next, T->bp_tol, 1.0 / dt, b.conv, b.active, b.ctl);
   if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "step begin");
-  int rc = enqueue_first_load(T, 0, first);
-  if (rc) return rc;
+  // first load, first iteration and (if converged) completion in one launch
   int32_t iters = 0;
-  rc = newton_loop(T, first ? 1 : 2, &iters);
+  int rc = newton_loop(T, first ? 1 : 2, &iters, first ? 3 : 4);
   T->newton_total += iters;
   if (rc < 0) return rc;
   if (rc > 0) {  // no convergence: two half steps
@@ -393,8 +458,10 @@ int xg_tran_step(xg_tran *T, double dt) {
     if (rc) return rc;
     return xg_tran_step(T, dt / 2);
   }
-  rc = complete(T, 0);
-  if (rc) return rc;
+  if (iters > 1) {  // converged after the fused graph: complete now
+    rc = complete(T, 0);
+    if (rc) return rc;
+  }
   T->t = t_next;
   T->n_steps++;
   return XG_OK;
@@ -438,11 +505,12 @@ int xg_tran_sync(xg_tran *T) {
 int xg_tran_destroy(xg_tran *T) {
   if (!T) return XG_OK;
   (void)hipStreamSynchronize(T->stream);
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 5; ++k) {
     if (T->exec[k]) (void)hipGraphExecDestroy(T->exec[k]);
     if (T->graph[k]) (void)hipGraphDestroy(T->graph[k]);
   }
   xg_set_step_source(T->h, nullptr);
+  xg_set_active_mask(T->h, nullptr);
   xg_set_stream(T->h, T->old_stream);
   if (T->ctl_host) (void)hipHostFree(T->ctl_host);
   (void)hipStreamDestroy(T->stream);

@@ -60,7 +60,7 @@ class Request:
         if self.kind == "ring":
             return circuits.ring_oscillator_bsim3(self.size, seed=self.seed, ramp=50e-12)
         if self.kind == "logic":
-            return circuits.random_logic_bsim3(n_gates=self.size, seed=self.seed)
+            return circuits.random_logic_bsim3(n_gates=self.size, seed=self.seed, ramp=50e-12)
         if self.kind == "ladder":
             return circuits.rc_ladder_diodes(self.size, max(1, self.size // 10),
                                              seed=self.seed)
@@ -75,6 +75,10 @@ class Request:
 SPANS = {"ring": (0.1e-9, 1e-12, ["out0"]),
          "logic": (0.05e-9, 1e-12, ["g0"]),
          "ladder": (0.05e-9, 1e-12, ["n0", "n1"])}
+# Random logic starts from zero (.TRAN UIC) with its supply ramping up: its
+# feedback paths can leave the plain DC Newton solve without convergence,
+# and there is no gmin / source stepping here.
+UIC = {"ring": False, "logic": True, "ladder": False}
 
 
 def poisson_requests(rate=200.0, n=200, seed=4, mix=(0.6, 0.3, 0.1),
@@ -116,7 +120,7 @@ class _Batch:
 
 
 def _run_batch(batch, sync):
-    from .transient import Transient
+    from .transient import DEVICE_LOCK, Transient
     reqs = batch.reqs
     merged = circuits.merge([r.netlist() for r in reqs])
     tstop, dt = reqs[0].tstop, reqs[0].dt
@@ -125,10 +129,14 @@ def _run_batch(batch, sync):
         names += ["s%d_%s" % (k, p) for p in r.probes]
     tr = Transient(merged)
     try:
-        t, w = tr.run(tstop, dt, names, every=10)
+        t, w = tr.run(tstop, dt, names, every=10, uic=UIC[reqs[0].kind])
+        for r in reqs:
+            r.batch_info = dict(size=len(reqs), step_failures=tr.step_failures,
+                                newton=tr.newton_total)
     finally:
-        tr.close()
-        tr.dev.close()
+        with DEVICE_LOCK:
+            tr.close()
+            tr.dev.close()
     for r in reqs:
         r._nl = None
     j = 0

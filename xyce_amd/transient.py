@@ -36,6 +36,7 @@ fixed (the reference's LTE step control is not reproduced); a step whose
 Newton solve fails is retried with half the step.
 """
 import ctypes
+import threading
 
 import numpy as np
 
@@ -52,6 +53,13 @@ class NewtonFailure(RuntimeError):
     pass
 
 
+# Device allocations, synchronous copies and graph captures of concurrent
+# transients (serving runs several at once) are serialised by this lock; a
+# capture must not overlap another thread's allocation or synchronous copy.
+# The step loops themselves run concurrently.
+DEVICE_LOCK = threading.RLock()
+
+
 class _TranBuffers(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in (
         "x", "x_curr", "xh1", "xh2", "xpred", "sta_c", "sta_n", "sto_c", "sto_n",
@@ -65,6 +73,10 @@ _FAIL_WORDS = {1: "singular Jacobian", 2: "non-finite device values", 3: "no con
 class Transient:
     def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda",
                  solver="lu", driver=None, graphs=True):
+        with DEVICE_LOCK:
+            self._init(nl, gmin, limiter, bp_tol, device, solver, driver, graphs)
+
+    def _init(self, nl, gmin, limiter, bp_tol, device, solver, driver, graphs):
         """solver: "lu" (static-pivot sparse LU with pivoting fallback) or
         "dense" (dense partial-pivoting LU for every component; python
         driver only).  driver: "native" (default with "lu") or "python"."""
@@ -105,6 +117,8 @@ class Transient:
         self.nstats = z(4 * self.n_comp)
         self.lim = torch.zeros(self.n_comp, dtype=torch.int32, device=device)
         self.active = torch.ones(self.n_comp, dtype=torch.int32, device=device)
+        # loads skip the instances of converged sub-circuits
+        check(lib().xg_set_active_mask(self.dev.h, self.active.data_ptr()), "active mask")
         self.bp_tol = bp_tol
         self.t = 0.0
         self.n_steps = 0
@@ -130,9 +144,10 @@ class Transient:
             check(lib().xg_tran_set_graphs(tr, 1 if graphs else 0), "graphs")
 
     def close(self):
-        if self._tr is not None:
-            lib().xg_tran_destroy(self._tr)
-            self._tr = None
+        with DEVICE_LOCK:
+            if self._tr is not None:
+                lib().xg_tran_destroy(self._tr)
+                self._tr = None
 
     def __del__(self):
         try:
@@ -225,6 +240,27 @@ class Transient:
         return it
 
     # ---- analyses
+    def uic(self):
+        """Start from zero without an operating point (.TRAN UIC): one load
+        at x = 0 with operating-point flags starts the history."""
+        if self._tr is not None:
+            self._native(lib().xg_tran_uic(self._tr), "UIC start")
+            return
+        self.x.zero_()
+        self.sta_c.zero_()
+        self.sto_c.zero_()
+        self.active.fill_(1)
+        self._load(1, False, dcop=True, t=0.0)
+        self.x_curr.copy_(self.x)
+        self.sta_c.copy_(self.sta_n)
+        self.sto_c.copy_(self.sto_n)
+        self.qn0.copy_(self.Q)
+        self.fprev.copy_(self.F)
+        self.xh1.zero_()
+        self.xh2.zero_()
+        self.t = 0.0
+        self.n_steps = 0
+
     def dcop(self):
         """DC operating point at t = 0 (the tranop before a transient)."""
         if self._tr is not None:
@@ -283,14 +319,15 @@ class Transient:
         self.t = t_next
         self.n_steps += 1
 
-    def run(self, tstop, dt, probes, every=1):
-        """DC operating point, then steps of dt to tstop.  Returns (times,
-        waveforms[probe, time]) for the given node names."""
+    def run(self, tstop, dt, probes, every=1, uic=False):
+        """DC operating point (or, uic=True, the zero start), then steps of
+        dt to tstop.  Returns (times, waveforms[probe, time]) for the given
+        node names."""
         torch = self.torch
         if self._tr is not None:
-            return self._run_native(tstop, dt, probes, every)
+            return self._run_native(tstop, dt, probes, every, uic)
         idx = torch.tensor([self.nl.node(p) for p in probes], device=self.x.device)
-        self.dcop()
+        self.uic() if uic else self.dcop()
         n = int(round(tstop / dt))
         out = torch.zeros(len(probes), n // every + 1, dtype=torch.float64,
                           device=self.x.device)
@@ -305,16 +342,17 @@ class Transient:
                 k += 1
         return np.array(times), out[:, :k].cpu().numpy()
 
-    def _run_native(self, tstop, dt, probes, every):
+    def _run_native(self, tstop, dt, probes, every, uic=False):
         torch = self.torch
         L = lib()
-        ids = torch.tensor([self.nl.node(p) for p in probes], dtype=torch.int32,
-                           device=self.x.device)
         n = int(round(tstop / dt))
-        out = torch.zeros(n // every + 1, len(probes), dtype=torch.float64,
-                          device=self.x.device)
-        torch.cuda.current_stream().synchronize()
-        self.dcop()
+        with DEVICE_LOCK:
+            ids = torch.tensor([self.nl.node(p) for p in probes], dtype=torch.int32,
+                               device=self.x.device)
+            out = torch.zeros(n // every + 1, len(probes), dtype=torch.float64,
+                              device=self.x.device)
+            torch.cuda.current_stream().synchronize()
+        self.uic() if uic else self.dcop()
         check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes), out[0].data_ptr()),
               "probe")
         times = [0.0]
@@ -332,7 +370,9 @@ class Transient:
                 k += 1
         check(L.xg_tran_sync(self._tr), "sync")
         self._sync_counters()
-        return np.array(times), out[:k].T.cpu().numpy()
+        with DEVICE_LOCK:
+            w = out[:k].T.cpu().numpy()
+        return np.array(times), w
 
     # ---- checkpoint / restart (N_IO_RestartMgr's role for a transient)
     _CKPT = ("x_curr", "xh1", "xh2", "sta_c", "sto_c", "qn0", "fprev")
