@@ -252,6 +252,12 @@ int xg_add_diode(xg_circuit *c, int32_t pos, int32_t neg, int32_t n_model,
                  const char *const *mkeys, const double *mvals, int32_t n_inst,
                  const char *const *ikeys, const double *ivals, int32_t sto);
 
+/* The device half of xg_finalize for a handle made by xg_finalize_plan:
+   allocate and upload (host work and device work can then be done at
+   different times, e.g. setup of one request while others run).  No
+   counterpart in the reference. */
+int xg_upload(xg_handle *h);
+
 /* ---- Newton iteration linear algebra (the rows after the device load) */
 
 /* Independent sub-circuits: connected components of the circuit graph with

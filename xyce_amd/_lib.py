@@ -88,6 +88,7 @@ SIGNATURES = [
     ("xg_lu_export", ctypes.c_int, [ctypes.c_void_p] * 4),
     ("xg_set_step_source", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_set_active_mask", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_upload", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_tran_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                       ctypes.c_void_p]),
     ("xg_tran_dcop", ctypes.c_int, [ctypes.c_void_p]),

@@ -578,17 +578,13 @@ int xg_circuit_set_pattern(xg_circuit *c, const int32_t *rowptr,
   return XG_OK;
 }
 
-int xg_finalize(xg_circuit *c, xg_handle **out) {
-  if (!c || !out) return fail(XG_E_ARG, "xg_finalize: bad argument");
-  auto *H = new (std::nothrow) xg_handle;
-  if (!H) return fail(XG_E_NOMEM, "xg_finalize: out of memory");
+}  // extern "C"
+
+// Device allocation and upload of a built plan (the device half of
+// xg_finalize).  Returns false if an allocation or copy failed.
+static bool upload_handle(xg_handle *H) {
   xg::Handle &h = H->h;
-  std::string err;
-  int rc = xg::build_plan(c->c, h.plan, err);
-  if (rc) {
-    delete H;
-    return fail(rc, err);
-  }
+  int rc = 0;
   const xg::Plan &P = h.plan;
   bool ok = true;
   auto chk = [&](const void *p) { ok = ok && p != nullptr; };
@@ -703,8 +699,8 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
     std::vector<double> z((size_t)(P.lin_slots > 0 ? P.lin_slots : 1), 0.0);
     chk(h.lin.dvd = h.upload(z));
   }
-  h.lin.gmin = c->c.opt.gmin;
-  h.lin.limiter = c->c.opt.limiter;
+  h.lin.gmin = P.gmin;
+  h.lin.limiter = P.limiter;
   if (ok) {
     h.lin.slots = h.slots + P.lin_slot_base;
     h.lin.limited = h.limited_all + P.m1_slots + xg::bsim3_eval_slots(P);
@@ -735,11 +731,35 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
   chk(h.partial = h.alloc<double>((size_t)P.n_partials));
   for (int k = 0; k < 4; ++k) up_prog(P.red_vec[k], h.red_vec[k]);
   for (int k = 0; k < 2; ++k) up_prog(P.red_mat[k], h.red_mat[k]);
-  if (!ok) {
+  (void)rc;
+  return ok;
+}
+
+extern "C" {
+
+int xg_finalize(xg_circuit *c, xg_handle **out) {
+  if (!c || !out) return fail(XG_E_ARG, "xg_finalize: bad argument");
+  auto *H = new (std::nothrow) xg_handle;
+  if (!H) return fail(XG_E_NOMEM, "xg_finalize: out of memory");
+  std::string err;
+  int rc = xg::build_plan(c->c, H->h.plan, err);
+  if (rc) {
+    delete H;
+    return fail(rc, err);
+  }
+  if (!upload_handle(H)) {
     delete H;
     return fail(XG_E_NOMEM, "xg_finalize: device allocation/upload failed");
   }
   *out = H;
+  return XG_OK;
+}
+
+int xg_upload(xg_handle *H) {
+  if (!H) return fail(XG_E_ARG, "xg_upload: bad argument");
+  if (!H->h.plan_only) return fail(XG_E_STATE, "xg_upload: handle is already on the device");
+  if (!upload_handle(H)) return fail(XG_E_NOMEM, "xg_upload: device allocation/upload failed");
+  H->h.plan_only = false;
   return XG_OK;
 }
 
@@ -760,7 +780,8 @@ int xg_finalize_plan(xg_circuit *c, xg_handle **out) {
 
 void xg_destroy(xg_handle *h) {
   if (h) {
-    if (!h->h.plan_only) (void)hipDeviceSynchronize();
+    // the handle's work is on its stream (a transient driver syncs its own)
+    if (!h->h.plan_only) (void)hipStreamSynchronize(h->h.stream);
     delete h;
   }
 }

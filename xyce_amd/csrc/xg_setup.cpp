@@ -872,6 +872,7 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
       mskip[j] = (r == G || plan.colidx[j] == G) ? 1 : 0;
   for (int k = 0; k < 2; ++k) finish_program(mat[k], nnz, mskip, plan.red_mat[k]);
   plan.limiter = c.opt.limiter;
+  plan.gmin = c.opt.gmin;
   // ---- connected components without ground (union-find over the pattern)
   {
     std::vector<int32_t> par(n);

@@ -158,6 +158,7 @@ struct Plan {
   ReduceProgram red_mat[2];  // dFdx, dQdx
   int32_t n_partials = 0;    // chunk partial sums over all six programs
   int32_t limiter = 1;
+  double gmin = 1e-12;
   // connected components of the circuit graph without ground (independent
   // sub-circuits), for the per-component dense solve
   std::vector<int32_t> comp_start, comp_nodes, comp_local;

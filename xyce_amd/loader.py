@@ -13,10 +13,17 @@ Arguments are torch tensors resident on the GPU (float64, contiguous); torch
 is used only to own device memory and streams.
 """
 import ctypes
+import threading
 import numpy as np
 
 from . import _lib
 from ._lib import check, keys_vals, lib
+
+# Device allocations, uploads, synchronous copies and graph captures of
+# concurrent users (the serving loop runs several transients at once) are
+# serialised by this lock; a graph capture must not overlap another thread's
+# allocation or synchronous copy.  Kernel launches are not serialised.
+DEVICE_LOCK = threading.RLock()
 
 
 def _ptr(t, dtype="torch.float64"):
@@ -82,8 +89,15 @@ class DeviceLoad:
                     c, rp.ctypes.data_as(_lib.c_int32_p), ci.ctypes.data_as(_lib.c_int32_p)),
                     "set pattern")
             h = ctypes.c_void_p()
-            fin = L.xg_finalize_plan if plan_only else L.xg_finalize
-            check(fin(c, ctypes.byref(h)), "xg_finalize")
+            # host analysis first, then the device half under the lock that
+            # keeps allocations and uploads apart from graph captures
+            check(L.xg_finalize_plan(c, ctypes.byref(h)), "xg_finalize")
+            if not plan_only:
+                with DEVICE_LOCK:
+                    rc = L.xg_upload(h)
+                if rc:
+                    L.xg_destroy(h)
+                    check(rc, "xg_upload")
         finally:
             L.xg_circuit_free(c)
         self.h = h
@@ -97,7 +111,8 @@ class DeviceLoad:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().xg_destroy(self.h)
+            with DEVICE_LOCK:
+                lib().xg_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -127,12 +127,15 @@ def _run_batch(batch, sync):
     names = []
     for k, r in enumerate(reqs):
         names += ["s%d_%s" % (k, p) for p in r.probes]
+    t0 = time.perf_counter()
     tr = Transient(merged)
+    t1 = time.perf_counter()
     try:
         t, w = tr.run(tstop, dt, names, every=10, uic=UIC[reqs[0].kind])
         for r in reqs:
             r.batch_info = dict(size=len(reqs), step_failures=tr.step_failures,
-                                newton=tr.newton_total)
+                                newton=tr.newton_total, setup_s=t1 - t0,
+                                run_s=time.perf_counter() - t1)
     finally:
         with DEVICE_LOCK:
             tr.close()
@@ -150,6 +153,10 @@ def serve(requests, max_batch=32, workers=3, window=0.005, hbm_budget=64 << 30,
           realtime=True, log=None):
     """Runs the request stream; returns the requests with results,
     admission and completion times (seconds on the stream's clock)."""
+    import torch
+    torch.cuda.init()            # the HIP runtime starts here, not in a worker
+    from . import _lib
+    _lib.lib()
     sync = None
     pending = {}             # kind -> list of arrived requests
     lock = threading.Lock()
@@ -232,6 +239,18 @@ def summary(reqs):
     lat = np.array([r.latency for r in reqs if r.error is None])
     done = [r for r in reqs if r.error is None]
     span = max(r.t_done for r in done) - min(r.arrival for r in reqs)
-    return dict(requests=len(reqs), failed=sum(r.error is not None for r in reqs),
-                solves_per_s=len(done) / span, latency_p50_s=float(np.percentile(lat, 50)),
-                latency_p99_s=float(np.percentile(lat, 99)), span_s=span)
+    out = dict(requests=len(reqs), failed=sum(r.error is not None for r in reqs),
+               solves_per_s=len(done) / span, latency_p50_s=float(np.percentile(lat, 50)),
+               latency_p99_s=float(np.percentile(lat, 99)), span_s=span)
+    # per kind: batches, mean batch size, mean setup and run seconds per batch
+    for kind in sorted({r.kind for r in done}):
+        batches = {}
+        for r in done:
+            if r.kind == kind and getattr(r, "batch_info", None):
+                batches[r.t_admit] = r.batch_info
+        if batches:
+            b = list(batches.values())
+            out[kind] = dict(batches=len(b), mean_size=float(np.mean([x["size"] for x in b])),
+                             setup_s=float(np.mean([x["setup_s"] for x in b])),
+                             run_s=float(np.mean([x["run_s"] for x in b])))
+    return out

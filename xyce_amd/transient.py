@@ -36,13 +36,12 @@ fixed (the reference's LTE step control is not reproduced); a step whose
 Newton solve fails is retried with half the step.
 """
 import ctypes
-import threading
 
 import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .loader import DeviceLoad, _iptr, _ptr
+from .loader import DEVICE_LOCK, DeviceLoad, _iptr, _ptr
 
 DC_PARAMS = dict(reltol=1e-3, abstol=1e-12, rhstol=1e-6, dxtol=1.0, maxit=200)
 TRAN_PARAMS = dict(reltol=1e-2, abstol=1e-6, rhstol=1e-2, dxtol=0.33, maxit=20)
@@ -53,11 +52,6 @@ class NewtonFailure(RuntimeError):
     pass
 
 
-# Device allocations, synchronous copies and graph captures of concurrent
-# transients (serving runs several at once) are serialised by this lock; a
-# capture must not overlap another thread's allocation or synchronous copy.
-# The step loops themselves run concurrently.
-DEVICE_LOCK = threading.RLock()
 
 
 class _TranBuffers(ctypes.Structure):
@@ -73,10 +67,13 @@ _FAIL_WORDS = {1: "singular Jacobian", 2: "non-finite device values", 3: "no con
 class Transient:
     def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda",
                  solver="lu", driver=None, graphs=True):
+        # the device load's host analysis runs unlocked; its upload, the
+        # buffers and the driver's graph captures under DEVICE_LOCK
+        dev = DeviceLoad(nl, gmin=gmin, limiter=limiter)
         with DEVICE_LOCK:
-            self._init(nl, gmin, limiter, bp_tol, device, solver, driver, graphs)
+            self._init(nl, dev, bp_tol, device, solver, driver, graphs, limiter)
 
-    def _init(self, nl, gmin, limiter, bp_tol, device, solver, driver, graphs):
+    def _init(self, nl, dev, bp_tol, device, solver, driver, graphs, limiter):
         """solver: "lu" (static-pivot sparse LU with pivoting fallback) or
         "dense" (dense partial-pivoting LU for every component; python
         driver only).  driver: "native" (default with "lu") or "python"."""
@@ -93,7 +90,7 @@ class Transient:
         self.driver = driver
         self.torch = torch
         self.nl = nl.layout()
-        self.dev = DeviceLoad(nl, gmin=gmin, limiter=limiter)
+        self.dev = dev
         self.dev.set_output_mode(True)
         n1 = nl.n_nodes + 1
         nc = ctypes.c_int32()
