@@ -34,11 +34,21 @@ B3_SDP_FIELDS = 84
 B3_GEO_FIELDS = 15
 B3_BYTES_PER_EVAL = 32 + 40 + 80 + 400 + 152 + 8 * (B3_SDP_FIELDS + B3_GEO_FIELDS)
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per bsim3_eval_kernel launch on this workload, from the FETCH_SIZE
+# HBM bytes per BSIM3 eval launch (the staged split kernel the library picks
+# for 20000 instances) on this workload, from the FETCH_SIZE
 # and WRITE_SIZE counter passes scaled by a plain-copy calibration
 # (profiles/bsim3_device_load_random_logic_5000g_hbm_traffic.md); valid for
 # the default --gates only.
-B3_EVAL_TRAFFIC_BYTES = 38.81e6
+B3_EVAL_TRAFFIC_BYTES = 29.66e6
+# Eval waves up to which the library's automatic choice is the LDS-staged
+# split kernel (one wave per 64 instances), else the HBM-record kernel.
+B3_STAGED_MAX_WAVES = 3 * 256
+
+
+def b3_kernel_name(n_b3):
+    waves = (n_b3 + 63) // 64
+    return ("bsim3_eval_staged_kernel<true>" if waves <= B3_STAGED_MAX_WAVES
+            else "bsim3_eval_kernel")
 
 
 def progress(msg):
@@ -387,7 +397,7 @@ def main():
     torch.cuda.synchronize()
     a.record(stream)
     for _ in range(n_k):
-        dev.update_state(X, SC, TC, SN, TN)
+        dev.update_state_type("bsim3", X, SC, TC, SN, TN)
     b.record(stream)
     torch.cuda.synchronize()
     eval_ms = a.elapsed_time(b) / n_k
@@ -421,7 +431,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": B3_EVAL_TRAFFIC_BYTES if args.gates == 5000 else None,
-                     "kernel": "bsim3_eval_kernel", "kernel_ms": eval_ms,
+                     "kernel": b3_kernel_name(n_b3), "kernel_ms": eval_ms,
                      "bytes_per_eval": B3_BYTES_PER_EVAL},
         "wall_s_timed_region": wall,
         "launch": launch,

@@ -196,6 +196,19 @@ int xg_set_solver_state(xg_handle *h, const xg_solver_state *ss);
 int xg_update_state(xg_handle *h, const double *x_next, double *sta_curr,
                     const double *sto_curr, double *sta_next, double *sto_next);
 
+/* One device type's Master::updateState alone (the per-type call inside the
+   loop of DeviceMgr::updateState, N_DEV_DeviceMgr.C:2258, i.e.
+   DeviceMaster<T>::updateState, N_DEV_DeviceMaster.h:642): XG_DEV_MOS1,
+   XG_DEV_BSIM3, or XG_DEV_LINEAR (resistors, capacitors, diodes and
+   sources, evaluated by one kernel).  Calling it for every type in that
+   order equals xg_update_state. */
+#define XG_DEV_MOS1 1
+#define XG_DEV_BSIM3 2
+#define XG_DEV_LINEAR 3
+int xg_update_state_type(xg_handle *h, int32_t type, const double *x_next,
+                         double *sta_curr, const double *sto_curr,
+                         double *sta_next, double *sto_next);
+
 /* Master::loadDAEVectors (N_DEV_MOSFET1.C:4202, N_DEV_MOSFET_B3.C:9306). */
 int xg_load_vectors(xg_handle *h, double *F, double *Q, double *dFdxdVp,
                     double *dQdxdVp);

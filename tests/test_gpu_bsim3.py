@@ -286,3 +286,40 @@ def test_graph_replay_matches_eager():
             torch.cuda.synchronize()
             got = [t.cpu().numpy().tobytes() for t in outs + [SN, TN]]
             assert got == eager, (overwrite, rep)
+
+
+def test_update_state_by_type_matches_all():
+    """xg_update_state_type for MOS1, BSIM3 and the linear devices in turn
+    writes exactly what one xg_update_state writes (states, stores and
+    every load that follows)."""
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd.loader import DeviceLoad
+    nl = circuits.merge([circuits.mos1_variety(seed=3), circuits.bsim3_variety(seed=3),
+                         circuits.diode_variety()])
+    nl.layout()
+    n1 = nl.n_nodes + 1
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-0.2, 1.5, n1)
+    x[-1] = 0.0
+    sto = rng.uniform(-0.5, 1.5, nl.n_store)
+    cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    res = []
+    for by_type in (False, True):
+        # a fresh handle each: the diode keeps its last junction voltage
+        dev = DeviceLoad(nl)
+        dev.set_solver_state(transient=1, newtonIter=1)
+        X, SC, TC = cu(x), cu(np.zeros(nl.n_state)), cu(sto)
+        SN, TN = cu(np.zeros(nl.n_state)), cu(sto)
+        outs = [cu(np.zeros(n1)) for _ in range(4)] + [cu(np.zeros(dev.nnz)) for _ in range(2)]
+        if by_type:
+            for kind in ("mos1", "bsim3", "linear"):
+                dev.update_state_type(kind, X, SC, TC, SN, TN)
+        else:
+            dev.update_state(X, SC, TC, SN, TN)
+        dev.load_vectors(*outs[:4])
+        dev.load_matrices(*outs[4:])
+        dev.sync()
+        res.append([t.cpu().numpy().tobytes() for t in outs + [SN, TN, SC]])
+        dev.close()
+    assert res[0] == res[1]

@@ -67,18 +67,12 @@ def test_bsim3_ring_waveform_parity():
     assert w[0].max() > 1.0
 
 
-def test_dense_solve_against_numpy():
-    """The per-component LU (LDS and global-memory forms) solves random
-    diagonally dominant systems on the circuit pattern like LAPACK."""
-    import ctypes
-    import torch
+def _mos1_rings(stage_counts):
+    """MOS1 rings (one connected component each) in one netlist."""
     from xyce_amd import circuits
-    from xyce_amd._lib import check, lib
-    from xyce_amd.loader import DeviceLoad, _ptr
-    nl = circuits.Netlist("two rings")
-    a = circuits.ring_oscillator_mos1(21, seed=1)       # one small component
-    b = circuits.ring_oscillator_mos1(301, seed=2)      # one large (> 128 unknowns)
-    for src, pre in ((a, "a"), (b, "b")):
+    nl = circuits.Netlist("rings")
+    for k, stages in enumerate(stage_counts):
+        src, pre = circuits.ring_oscillator_mos1(stages, seed=k + 1), "r%d_" % k
         for name, m in src.models.items():
             nl.add_model(pre + name, m.kind, m.dtype, **m.params)
         for m in src.mos:
@@ -88,7 +82,55 @@ def test_dense_solve_against_numpy():
         for v in src.vsrc:
             nl.add_vsrc(pre + v.name, pre + v.p, "0", v.dc, v.pulse)
     nl.layout()
+    return nl
+
+
+def _dense_solve(nl, vals, rhs):
+    """xg_dense_solve of one Newton update from x = 0 (weights 1):
+    returns (x, per-component stats, component of each node)."""
+    import torch
+    from xyce_amd._lib import check, lib
+    from xyce_amd.loader import DeviceLoad, _ptr
     dev = DeviceLoad(nl)
+    n = nl.n_nodes
+    cu = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda")
+    J, R, X = cu(vals), cu(rhs), cu(np.zeros(n + 1))
+    W = cu(np.ones(n + 1))
+    comp = _components(dev)
+    stats = cu(np.zeros(4 * (comp.max() + 1)))
+    check(lib().xg_dense_solve(dev.h, _ptr(J), _ptr(R), _ptr(X), _ptr(W), _ptr(W), 0.0, 1.0,
+                               None, _ptr(stats)), "solve")
+    out = X.cpu().numpy(), stats.cpu().numpy().reshape(-1, 4), comp
+    dev.close()
+    return out
+
+
+def _components(dev):
+    import ctypes
+    from xyce_amd._lib import check, lib
+    nc = ctypes.c_int32()
+    comp = np.zeros(dev.n_rows - 1, np.int32)
+    check(lib().xg_components(dev.h, ctypes.byref(nc), comp.ctypes.data), "components")
+    return comp
+
+
+def _component_system(rp, ci, vals, rhs, idx):
+    n1 = len(rp) - 1
+    rows = np.repeat(np.arange(n1), np.diff(rp))
+    loc = -np.ones(n1, int)
+    loc[idx] = np.arange(len(idx))
+    A = np.zeros((len(idx), len(idx)))
+    m = (loc[rows] >= 0) & (loc[ci] >= 0)
+    A[loc[rows[m]], loc[ci[m]]] = vals[m]
+    return A, rhs[idx]
+
+
+def test_dense_solve_against_numpy():
+    """The per-component LU (LDS and global-memory forms) solves random
+    diagonally dominant systems on the circuit pattern like LAPACK."""
+    from xyce_amd.loader import DeviceLoad
+    nl = _mos1_rings([21, 301])        # one small component, one > 128 unknowns
+    dev = DeviceLoad(nl, plan_only=True)
     rp, ci = dev.pattern()
     n = nl.n_nodes
     rng = np.random.default_rng(3)
@@ -97,28 +139,52 @@ def test_dense_solve_against_numpy():
     vals[rows == ci] += 20.0
     rhs = rng.uniform(-1, 1, n + 1)
     rhs[n] = 0.0
-    cu = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda")
-    J, R, X = cu(vals), cu(rhs), cu(np.zeros(n + 1))
-    W = cu(np.ones(n + 1))
-    nc = ctypes.c_int32()
-    comp = np.zeros(n, np.int32)
-    check(lib().xg_components(dev.h, ctypes.byref(nc), comp.ctypes.data), "components")
-    assert nc.value == 2
-    stats = cu(np.zeros(4 * nc.value))
-    check(lib().xg_dense_solve(dev.h, _ptr(J), _ptr(R), _ptr(X), _ptr(W), _ptr(W), 0.0, 1.0,
-                               None, _ptr(stats)), "solve")
-    x = X.cpu().numpy()
-    for c in range(nc.value):
+    x, st, comp = _dense_solve(nl, vals, rhs)
+    assert comp.max() + 1 == 2
+    for c in range(2):
         idx = np.flatnonzero(comp == c)
-        loc = -np.ones(n + 1, int)
-        loc[idx] = np.arange(len(idx))
-        A = np.zeros((len(idx), len(idx)))
-        m = (loc[rows] >= 0) & (loc[ci] >= 0)
-        A[loc[rows[m]], loc[ci[m]]] = vals[m]
-        ref = np.linalg.solve(A, rhs[idx])
+        A, b = _component_system(rp, ci, vals, rhs, idx)
+        ref = np.linalg.solve(A, b)
         assert np.allclose(x[idx], ref, rtol=1e-12, atol=1e-14), c
-    st = stats.cpu().numpy().reshape(-1, 4)
     assert not st[:, 2].any()
+
+
+def test_dense_solve_with_row_interchanges():
+    """Random values with no dominant diagonal (partial pivoting swaps rows
+    at most steps) on components in all three size classes of the dense
+    LU: factored in LDS (<= 128 unknowns), in global memory with 16-column
+    panels (<= 1024) and with 8-column panels (<= 2048).  Each solution has
+    a normwise backward error of a few ulps, and one column of zeros makes
+    its component report a singular matrix."""
+    from xyce_amd.loader import DeviceLoad
+    nl = _mos1_rings([40, 500, 1200, 90])
+    dev = DeviceLoad(nl, plan_only=True)
+    rp, ci = dev.pattern()
+    n = nl.n_nodes
+    rng = np.random.default_rng(11)
+    vals = rng.uniform(-1, 1, len(ci))
+    rhs = rng.uniform(-1, 1, n + 1)
+    rhs[n] = 0.0
+    comp = _components(dev)
+    sizes = np.bincount(comp)
+    order = np.argsort(sizes)          # 40-, 90-, 500- and 1200-stage rings
+    assert sizes[order[1]] <= 128 < sizes[order[2]] <= 1024 < sizes[order[3]] <= 2048
+    # the 90-stage ring's component loses one column: singular
+    sing = order[1]
+    vals[np.isin(ci, np.flatnonzero(comp == sing)[5:6])] = 0.0
+    x, st, comp2 = _dense_solve(nl, vals, rhs)
+    assert np.array_equal(comp, comp2)
+    for c in range(4):
+        idx = np.flatnonzero(comp == c)
+        if c == sing:
+            assert st[c, 2] == 1.0
+            continue
+        assert st[c, 2] == 0.0
+        A, b = _component_system(rp, ci, vals, rhs, idx)
+        r = b - A @ x[idx]
+        berr = np.abs(r).max() / (np.abs(A).sum(1).max() * np.abs(x[idx]).max() + np.abs(b).max())
+        print("component %d: %d unknowns, backward error %.2e" % (c, len(idx), berr), flush=True)
+        assert berr < 1e-13, c
 
 
 def test_batched_monte_carlo_equals_each_alone():

@@ -55,6 +55,8 @@ SIGNATURES = [
     ("xg_get_bsim3_fields", ctypes.c_int, [ctypes.c_void_p, c_double_p, c_int32_p]),
     ("xg_set_solver_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverState)]),
     ("xg_update_state", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
+    ("xg_update_state_type", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 5),
     ("xg_load_vectors", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     ("xg_load_matrices", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 2),
     ("xg_load_all", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 11),
@@ -123,6 +125,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise XgError("libxg_load.so is not built: run `python -m xyce_amd.build`")
+        # torch first: its HIP runtime (soname libamdhip64.so.7) then serves
+        # this library too.  Loaded the other way round, the process holds
+        # two HIP runtimes and torch's own fails to find the GPU.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             f = getattr(L, name)

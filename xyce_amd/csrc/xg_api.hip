@@ -862,12 +862,15 @@ int xg_set_stream(xg_handle *H, void *stream) {
   return XG_OK;
 }
 
-int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
-                    const double *sto_curr, double *sta_next,
-                    double *sto_next) {
+// The eval launches of one device type (0 = every type, in load order).
+static int update_state_types(xg_handle *H, int32_t type, const double *x_next,
+                              double *sta_curr, const double *sto_curr,
+                              double *sta_next, double *sto_next) {
   if (H && H->h.plan_only)
     return fail(XG_E_STATE, "handle was finalized without a device");
   if (!H || !x_next) return fail(XG_E_ARG, "xg_update_state: null pointer");
+  if (type < 0 || type > XG_DEV_LINEAR)
+    return fail(XG_E_ARG, "xg_update_state_type: unknown device type");
   xg::Handle &h = H->h;
   // state / store pointers may be null when the circuit has none
   if ((h.plan.n_state > 0 && (!sta_curr || !sta_next)) ||
@@ -875,16 +878,36 @@ int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
     return fail(XG_E_ARG, "xg_update_state: null pointer");
   xg::LoadPtrs p{x_next, sta_curr, sto_curr, sta_next, sto_next};
   p.active = h.active;
-  p.comp = h.slot_comp;
-  hipError_t e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
-  if (e != hipSuccess) return hip_fail(e, "mos1 eval");
-  p.comp = h.slot_comp + h.plan.m1_slots;
-  e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream);
-  if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
-  p.comp = h.slot_comp + h.plan.m1_slots + xg::bsim3_eval_slots(h.plan);
-  e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.dstep, h.stream);
-  if (e != hipSuccess) return hip_fail(e, "linear eval");
+  hipError_t e = hipSuccess;
+  if (type == 0 || type == XG_DEV_MOS1) {
+    p.comp = h.slot_comp;
+    e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
+    if (e != hipSuccess) return hip_fail(e, "mos1 eval");
+  }
+  if (type == 0 || type == XG_DEV_BSIM3) {
+    p.comp = h.slot_comp + h.plan.m1_slots;
+    e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream);
+    if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
+  }
+  if (type == 0 || type == XG_DEV_LINEAR) {
+    p.comp = h.slot_comp + h.plan.m1_slots + xg::bsim3_eval_slots(h.plan);
+    e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.dstep, h.stream);
+    if (e != hipSuccess) return hip_fail(e, "linear eval");
+  }
   return XG_OK;
+}
+
+int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
+                    const double *sto_curr, double *sta_next,
+                    double *sto_next) {
+  return update_state_types(H, 0, x_next, sta_curr, sto_curr, sta_next, sto_next);
+}
+
+int xg_update_state_type(xg_handle *H, int32_t type, const double *x_next,
+                         double *sta_curr, const double *sto_curr,
+                         double *sta_next, double *sto_next) {
+  if (type == 0) return fail(XG_E_ARG, "xg_update_state_type: unknown device type");
+  return update_state_types(H, type, x_next, sta_curr, sto_curr, sta_next, sto_next);
 }
 
 int xg_load_vectors(xg_handle *H, double *F, double *Q, double *dFdxdVp,

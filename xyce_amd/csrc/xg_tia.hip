@@ -9,7 +9,9 @@
 // J = dF/dx).  The solve replaces N_LAS_AmesosSolver (KLU): LU with
 // partial pivoting (the largest magnitude in the column, first on ties, as
 // LAPACK getrf), one workgroup per component; components of at most
-// kLdsN unknowns factor in LDS, larger ones in a global-memory workspace.
+// kLdsN unknowns factor in LDS, larger ones in a global-memory workspace,
+// both as a blocked elimination (column panels factored in LDS, rank-16
+// updates below them) and blocked triangular solves.
 #include "xg_internal.h"
 
 namespace xg {
@@ -76,63 +78,266 @@ __global__ void __launch_bounds__(kBlock)
   out[i] = v;
 }
 
-// Block-wide max of (|v|, index) with the smallest index on ties.
-__device__ __forceinline__ void argmax_abs(double v, int idx, double *sv, int *si,
-                                           double &best, int &bidx) {
-  const int t = threadIdx.x;
-  sv[t] = v;
-  si[t] = idx;
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      const double a = sv[t], b = sv[t + s];
-      const int ia = si[t], ib = si[t + s];
-      if (b > a || (b == a && ib < ia)) {
-        sv[t] = b;
-        si[t] = ib;
-      }
+// Sum over the 64 lanes of a wave (every lane gets the same value).
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide max of (|v|, index) with the smallest index on ties; callers
+// synchronise before the next call (sv / si are reused).
+__device__ __forceinline__ void block_argmax(double v, int idx, double *sv, int *si,
+                                             double &best, int &bidx) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o, 64);
+    const int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
     }
-    __syncthreads();
   }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sv[w] = v;
+    si[w] = idx;
+  }
+  __syncthreads();
   best = sv[0];
   bidx = si[0];
-  __syncthreads();
+  for (int k = 1; k < kBlock / 64; ++k)
+    if (sv[k] > best || (sv[k] == best && si[k] < bidx)) {
+      best = sv[k];
+      bidx = si[k];
+    }
 }
 
 __device__ __forceinline__ double block_sum(double v, double *sv) {
-  const int t = threadIdx.x;
-  sv[t] = v;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (t < s) sv[t] = sv[t] + sv[t + s];
-    __syncthreads();
-  }
-  const double r = sv[0];
+  double r = sv[0];
+  for (int k = 1; k < kBlock / 64; ++k) r = r + sv[k];
   __syncthreads();
   return r;
 }
 
 __device__ __forceinline__ double block_max(double v, double *sv) {
-  const int t = threadIdx.x;
-  sv[t] = v;
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (t < s) sv[t] = fmax(sv[t], sv[t + s]);
-    __syncthreads();
-  }
-  const double r = sv[0];
+  double r = sv[0];
+  for (int k = 1; k < kBlock / 64; ++k) r = fmax(r, sv[k]);
   __syncthreads();
   return r;
 }
+
+// Blocked right-looking LU with partial pivoting of the n x n row-major A
+// (LDS or global), row swaps applied to b as they happen.  Panels of NB
+// columns factor in the LDS buffer P (rows kb..n-1); the rows below the
+// panel then take the panel's NB rank-1 updates in one pass per element,
+// in the order of the unblocked elimination (each element's updates in
+// ascending pivot order, so the factors are those of the column-by-column
+// algorithm).  Returns false when a pivot column has no nonzero magnitude.
+template <int NB>
+__device__ __forceinline__ bool dense_factor(double *A, int n, double *b, double *P, double *sv,
+                             int *si, int *spiv) {
+  const int t = threadIdx.x;
+  for (int kb = 0; kb < n; kb += NB) {
+    const int nb = min(NB, n - kb);
+    const int m = n - kb;
+    for (int e = t; e < m * NB; e += kBlock) {
+      const int r = e / NB, c = e % NB;
+      P[e] = c < nb ? A[(int64_t)(kb + r) * n + kb + c] : 0.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+      double v = -1.0;
+      int idx = 0x7fffffff;
+      for (int r = c + t; r < m; r += kBlock) {
+        const double a = fabs(P[r * NB + c]);
+        if (a > v) {
+          v = a;
+          idx = r;
+        }
+      }
+      double best;
+      int br;
+      block_argmax(v, idx, sv, si, best, br);
+      if (!(best > 0.0)) return false;  // the same on every thread
+      if (t == 0) spiv[c] = kb + br;
+      if (br != c) {
+        for (int cc = t; cc < NB; cc += kBlock) {
+          const double tmp = P[c * NB + cc];
+          P[c * NB + cc] = P[br * NB + cc];
+          P[br * NB + cc] = tmp;
+        }
+        if (t == 0) {
+          const double tmp = b[kb + c];
+          b[kb + c] = b[kb + br];
+          b[kb + br] = tmp;
+        }
+      }
+      __syncthreads();
+      const double piv = P[c * NB + c];
+      for (int r = c + 1 + t; r < m; r += kBlock) {
+        const double l = P[r * NB + c] / piv;
+        P[r * NB + c] = l;
+        for (int cc = c + 1; cc < nb; ++cc) P[r * NB + cc] -= l * P[c * NB + cc];
+      }
+      __syncthreads();
+    }
+    // the factored panel back into A; the panel's row swaps on the other
+    // columns (left of the panel too, so L ends in the final row order)
+    for (int e = t; e < m * NB; e += kBlock) {
+      const int r = e / NB, c = e % NB;
+      if (c < nb) A[(int64_t)(kb + r) * n + kb + c] = P[e];
+    }
+    for (int j = t; j < n; j += kBlock) {
+      if (j >= kb && j < kb + nb) continue;
+      for (int c = 0; c < nb; ++c) {
+        const int pr = spiv[c];
+        if (pr != kb + c) {
+          const double tmp = A[(int64_t)(kb + c) * n + j];
+          A[(int64_t)(kb + c) * n + j] = A[(int64_t)pr * n + j];
+          A[(int64_t)pr * n + j] = tmp;
+        }
+      }
+    }
+    __syncthreads();
+    // U12 (rows kb..kb+NB-1 right of the panel) and the trailing update;
+    // a column is solved in registers, then swept down its rows.  Columns
+    // remain only when the panel is full (nb == NB).
+    const int j0 = kb + nb, ncols = n - j0;
+    // solve column j's U12 part into u (rows kb..kb+NB-1)
+    auto solve_u = [&](int j, double *u) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) u[c] = A[(int64_t)(kb + c) * n + j];
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int i = c + 1; i < NB; ++i) u[i] -= P[i * NB + c] * u[c];
+    };
+    // rows j0+g, j0+g+G, ... of column j take the panel's updates
+    auto sweep = [&](int j, int g, int G, const double *u) {
+      for (int i = j0 + g; i < n; i += G) {
+        double a = A[(int64_t)i * n + j];
+        const double *l = P + (i - kb) * NB;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) a -= l[c] * u[c];
+        A[(int64_t)i * n + j] = a;
+      }
+    };
+    if (ncols >= kBlock) {
+      // a column per thread
+      for (int j = j0 + t; j < n; j += kBlock) {
+        double u[NB];
+        solve_u(j, u);
+#pragma unroll
+        for (int c = 0; c < NB; ++c) A[(int64_t)(kb + c) * n + j] = u[c];
+        sweep(j, 0, 1, u);
+      }
+    } else if (ncols > 0) {
+      // G threads per column, each sweeping every G-th row; all of them
+      // read the column's U12 rows before the first one writes them
+      const int G = min(kBlock / ncols, 16);
+      const bool mine = t < ncols * G;
+      const int j = j0 + t % ncols, g = t / ncols;
+      double u[NB];
+      if (mine) solve_u(j, u);
+      __syncthreads();
+      if (mine) {
+        if (g == 0)
+#pragma unroll
+          for (int c = 0; c < NB; ++c) A[(int64_t)(kb + c) * n + j] = u[c];
+        sweep(j, g, G, u);
+      }
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
+// L y = b (unit lower) then U x = y on the factored A, b in LDS, blocks of
+// 64 rows: the block's products with the solved part are wave dot
+// products, the 64 x 64 diagonal block is solved by one wave.  D is an
+// LDS buffer of 64 x 64 doubles for the diagonal block (unused when A is
+// itself in LDS).
+template <bool kLds>
+__device__ __forceinline__ void dense_solve_tri(const double *A, int n, double *b, double *D,
+                                double *ss) {
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  constexpr int kWaves = kBlock / 64;
+  auto diag = [&](int r0, int l, int i) -> double {
+    return kLds ? A[(int64_t)(r0 + l) * n + r0 + i] : D[l * 64 + i];
+  };
+  for (int r0 = 0; r0 < n; r0 += 64) {
+    const int R = min(64, n - r0);
+    for (int rr = w; rr < R; rr += kWaves) {
+      const double *row = A + (int64_t)(r0 + rr) * n;
+      double s = 0.0;
+      for (int j = lane; j < r0; j += 64) s = s + row[j] * b[j];
+      s = wave_sum(s);
+      if (lane == 0) ss[rr] = s;
+    }
+    if (!kLds)
+      for (int e = t; e < R * 64; e += kBlock) {
+        const int l = e >> 6, i = e & 63;
+        if (i < R) D[e] = A[(int64_t)(r0 + l) * n + r0 + i];
+      }
+    __syncthreads();
+    if (w == 0) {
+      double bl = lane < R ? b[r0 + lane] - ss[lane] : 0.0;
+      for (int i = 0; i < R; ++i) {
+        const double yi = __shfl(bl, i, 64);
+        if (lane > i && lane < R) bl -= diag(r0, lane, i) * yi;
+      }
+      if (lane < R) b[r0 + lane] = bl;
+    }
+    __syncthreads();
+  }
+  for (int r0 = ((n - 1) / 64) * 64; r0 >= 0; r0 -= 64) {
+    const int R = min(64, n - r0);
+    for (int rr = w; rr < R; rr += kWaves) {
+      const double *row = A + (int64_t)(r0 + rr) * n;
+      double s = 0.0;
+      for (int j = r0 + R + lane; j < n; j += 64) s = s + row[j] * b[j];
+      s = wave_sum(s);
+      if (lane == 0) ss[rr] = s;
+    }
+    if (!kLds)
+      for (int e = t; e < R * 64; e += kBlock) {
+        const int l = e >> 6, i = e & 63;
+        if (i < R) D[e] = A[(int64_t)(r0 + l) * n + r0 + i];
+      }
+    __syncthreads();
+    if (w == 0) {
+      double bl = lane < R ? b[r0 + lane] - ss[lane] : 0.0;
+      for (int i = R - 1; i >= 0; --i) {
+        if (lane == i) bl = bl / diag(r0, i, i);
+        const double xi = __shfl(bl, i, 64);
+        if (lane < i) bl -= diag(r0, lane, i) * xi;
+      }
+      if (lane < R) b[r0 + lane] = bl;
+    }
+    __syncthreads();
+  }
+}
+
+// Panel buffer of the global-memory form: n x 16 doubles up to n = 1024,
+// n x 8 beyond (kDenseMax = 2048).
+constexpr int kPanelDoubles = 16384;
 
 // One component per workgroup.  A is n x n row-major (LDS or workspace).
 template <bool kLds>
 __global__ void __launch_bounds__(kBlock)
     dense_solve_kernel(DenseSolveArgs a) {
   __shared__ double lds_a[kLds ? kLdsN * kLdsN : 1];
-  __shared__ double sv[kBlock];
-  __shared__ int si[kBlock];
-  __shared__ double sb[kLds ? kLdsN : 1];
+  __shared__ double panel[kLds ? kLdsN * 16 : kPanelDoubles];
+  __shared__ double sb[kLds ? kLdsN : kDenseMax];
+  __shared__ double ss[64];
+  __shared__ double sv[kBlock / 64];
+  __shared__ int si[kBlock / 64];
+  __shared__ int spiv[16];
   const int c = blockIdx.x;
   const int t = threadIdx.x;
   const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
@@ -158,7 +363,7 @@ __global__ void __launch_bounds__(kBlock)
     return;
   }
   double *A = kLds ? lds_a : a.work + (a.aoff[c] < 0 ? 0 : a.aoff[c]);
-  double *b = kLds ? sb : a.bwork + beg;
+  double *b = sb;
   const int32_t *nodes = a.nodes + beg;
   // gather the component's rows of J (and its RHS) into dense form
   if (!a.norm_only)
@@ -181,47 +386,10 @@ __global__ void __launch_bounds__(kBlock)
     if (t == 0) stats[0] = max_rhs;
     return;
   }
-  // LU with partial pivoting, row swaps applied to b as they happen
-  int singular = 0;
-  for (int k = 0; k < n; ++k) {
-    double best = -1.0;
-    int bidx = k;
-    for (int i = k + t; i < n; i += kBlock) {
-      const double v = fabs(A[(int64_t)i * n + k]);
-      if (v > best) {
-        best = v;
-        bidx = i;
-      }
-    }
-    argmax_abs(best, bidx, sv, si, best, bidx);
-    if (!(best > 0.0)) {
-      singular = 1;
-      break;
-    }
-    if (bidx != k) {
-      for (int j = t; j < n; j += kBlock) {
-        const double tmp = A[(int64_t)k * n + j];
-        A[(int64_t)k * n + j] = A[(int64_t)bidx * n + j];
-        A[(int64_t)bidx * n + j] = tmp;
-      }
-      if (t == 0) {
-        const double tmp = b[k];
-        b[k] = b[bidx];
-        b[bidx] = tmp;
-      }
-    }
-    __syncthreads();
-    const double piv = A[(int64_t)k * n + k];
-    for (int i = k + 1 + t; i < n; i += kBlock) A[(int64_t)i * n + k] /= piv;
-    __syncthreads();
-    const int m = n - k - 1;
-    for (int64_t e = t; e < (int64_t)m * m; e += kBlock) {
-      const int i = k + 1 + (int)(e / m), j = k + 1 + (int)(e % m);
-      A[(int64_t)i * n + j] -= A[(int64_t)i * n + k] * A[(int64_t)k * n + j];
-    }
-    __syncthreads();
-  }
-  if (singular) {
+  const bool ok = (kLds || n <= kPanelDoubles / 16)
+                      ? dense_factor<16>(A, n, b, panel, sv, si, spiv)
+                      : dense_factor<8>(A, n, b, panel, sv, si, spiv);
+  if (!ok) {
     if (t == 0) {
       stats[0] = max_rhs;
       stats[1] = 0.0;
@@ -230,22 +398,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     return;
   }
-  // L y = P b (unit lower), then U x = y; one row at a time, the inner
-  // product spread over the workgroup
-  for (int r = 0; r < n; ++r) {
-    double s = 0.0;
-    for (int j = t; j < r; j += kBlock) s = s + A[(int64_t)r * n + j] * b[j];
-    s = block_sum(s, sv);
-    if (t == 0) b[r] = b[r] - s;
-    __syncthreads();
-  }
-  for (int r = n - 1; r >= 0; --r) {
-    double s = 0.0;
-    for (int j = r + 1 + t; j < n; j += kBlock) s = s + A[(int64_t)r * n + j] * b[j];
-    s = block_sum(s, sv);
-    if (t == 0) b[r] = (b[r] - s) / A[(int64_t)r * n + r];
-    __syncthreads();
-  }
+  dense_solve_tri<kLds>(A, n, b, panel, ss);
   // Newton update x += dx and the weighted RMS norm of dx
   // (N_NLS_DampedNewton.C:311-335 weights, wRMSNorm)
   double acc = 0.0;
@@ -257,10 +410,10 @@ __global__ void __launch_bounds__(kBlock)
     acc = acc + q * q;
     a.x[g] = a.x[g] + dx;
   }
-  const double ss = block_sum(acc, sv);
+  const double ss2 = block_sum(acc, sv);
   if (t == 0) {
     stats[0] = max_rhs;
-    stats[1] = sqrt(ss / n);
+    stats[1] = sqrt(ss2 / n);
     stats[2] = 0.0;
     stats[3] = (double)n;
   }

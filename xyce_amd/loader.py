@@ -212,6 +212,14 @@ class DeviceLoad:
         check(lib().xg_update_state(self.h, _ptr(x), _ptr(sta_curr), _ptr(sto_curr),
                                     _ptr(sta_next), _ptr(sto_next)), "xg_update_state")
 
+    DEVICE_TYPES = {"mos1": 1, "bsim3": 2, "linear": 3}
+
+    def update_state_type(self, kind, x, sta_curr, sto_curr, sta_next, sto_next):
+        """One device type's updateState alone ("mos1", "bsim3" or "linear")."""
+        check(lib().xg_update_state_type(self.h, self.DEVICE_TYPES[kind], _ptr(x),
+                                         _ptr(sta_curr), _ptr(sto_curr), _ptr(sta_next),
+                                         _ptr(sto_next)), "xg_update_state_type")
+
     def load_vectors(self, F, Q, dFdxdVp, dQdxdVp):
         check(lib().xg_load_vectors(self.h, _ptr(F), _ptr(Q), _ptr(dFdxdVp),
                                     _ptr(dQdxdVp)), "xg_load_vectors")
