@@ -144,10 +144,18 @@ def check(rc, what=""):
         raise XgError("%s failed: %s (%s)" % (what, ERRORS.get(rc, rc), msg))
 
 
+_KEY_ARRAYS = {}
+
+
 def keys_vals(params):
-    """dict -> (n, char**, double*) for the name/value parameter lists."""
-    items = list(params.items())
-    n = len(items)
-    keys = (ctypes.c_char_p * max(n, 1))(*[k.encode() for k, _ in items])
-    vals = (ctypes.c_double * max(n, 1))(*[float(v) for _, v in items])
+    """dict -> (n, char**, double*) for the name/value parameter lists (the
+    name arrays are built once per distinct list of names)."""
+    names = tuple(params)
+    n = len(names)
+    keys = _KEY_ARRAYS.get(names)
+    if keys is None:
+        keys = (ctypes.c_char_p * max(n, 1))(*[k.encode() for k in names])
+        if len(_KEY_ARRAYS) < 4096:
+            _KEY_ARRAYS[names] = keys
+    vals = (ctypes.c_double * max(n, 1))(*[float(v) for v in params.values()])
     return n, keys, vals

@@ -884,12 +884,24 @@ static int update_state_types(xg_handle *H, int32_t type, const double *x_next,
     e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
     if (e != hipSuccess) return hip_fail(e, "mos1 eval");
   }
+  // every type: the linear devices ride in trailing workgroups of the
+  // BSIM3 launch when there are both
+  const bool fused = type == 0 && h.b3.n_slots > 0 && h.lin.n_slots > 0;
   if (type == 0 || type == XG_DEV_BSIM3) {
     p.comp = h.slot_comp + h.plan.m1_slots;
-    e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream);
+    xg::LinTail tail;
+    if (fused) {
+      tail.a = h.lin;
+      tail.p = p;
+      tail.p.comp = h.slot_comp + h.plan.m1_slots + xg::bsim3_eval_slots(h.plan);
+      tail.time = h.time;
+      tail.bp_tol = h.bp_tol;
+      tail.dstep = h.dstep;
+    }
+    e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream, fused ? &tail : nullptr);
     if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
   }
-  if (type == 0 || type == XG_DEV_LINEAR) {
+  if ((type == 0 && !fused) || type == XG_DEV_LINEAR) {
     p.comp = h.slot_comp + h.plan.m1_slots + xg::bsim3_eval_slots(h.plan);
     e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.dstep, h.stream);
     if (e != hipSuccess) return hip_fail(e, "linear eval");

@@ -22,6 +22,7 @@
 #include "xg_bsim3.h"
 #include "xg_bsim3_layout.h"
 #include "xg_devmath.h"
+#include "xg_linear_dev.h"
 
 namespace xg {
 
@@ -1443,8 +1444,14 @@ __device__ __forceinline__ void b3_eval(const Bsim3Dev &a,
 // Many waves per CU: four waves per block, records read from HBM where the
 // evaluation uses them (other waves hide the latency).
 __global__ void __launch_bounds__(kBlock)
-    bsim3_eval_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p) {
+    bsim3_eval_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
+                      int32_t b3_blocks) {
   __shared__ double card_lds[kWavesPerBlock][B3C_N];
+  if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
+    lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
+                         ((int)blockIdx.x - b3_blocks) * kBlock + threadIdx.x);
+    return;
+  }
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int slot = blockIdx.x * kBlock + tid;
@@ -1482,9 +1489,16 @@ constexpr int kRecN = B3R_N + B3I_N;
 
 template <bool kSplit>
 __global__ void __launch_bounds__(kSplit ? 128 : 64)
-    bsim3_eval_staged_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p) {
+    bsim3_eval_staged_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
+                             int32_t b3_blocks) {
   __shared__ double rec[kRecN * 64];
   __shared__ double card[B3C_N];
+  if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
+    lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
+                         ((int)blockIdx.x - b3_blocks) * (kSplit ? 128 : 64) +
+                             threadIdx.x);
+    return;
+  }
   const int lane = threadIdx.x & 63;
   const int wave = kSplit ? (int)(threadIdx.x >> 6) : 0;
   constexpr int kWaves = kSplit ? 2 : 1;
@@ -1529,22 +1543,30 @@ __global__ void __launch_bounds__(kSplit ? 128 : 64)
 #endif
 
 hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
-                             const LoadPtrs &p, hipStream_t s) {
+                             const LoadPtrs &p, hipStream_t s, const LinTail *tail) {
   if (d.n_slots == 0) return hipSuccess;
   const int waves = d.n_slots / 64;
   int v = d.variant;
   if (v == XG_EVAL_AUTO)
     v = waves <= XG_B3_STAGED_MAX_WAVES ? XG_EVAL_STAGED_SPLIT : XG_EVAL_HBM;
+  const int block = v == XG_EVAL_STAGED_SPLIT ? 128 : (v == XG_EVAL_STAGED ? 64 : kBlock);
+  const int b3_blocks = v == XG_EVAL_HBM ? (d.n_slots + kBlock - 1) / kBlock : waves;
+  LinTail lt{};
+  int lin_blocks = 0;
+  if (tail && tail->a.n_slots > 0) {
+    lt = *tail;
+    lin_blocks = (tail->a.n_slots + block - 1) / block;
+  }
+  const dim3 grid(b3_blocks + lin_blocks);
   if (v == XG_EVAL_STAGED_SPLIT) {
-    hipLaunchKernelGGL(bsim3_eval_staged_kernel<true>, dim3(waves), dim3(128), 0,
-                       s, d, ss, p);
+    hipLaunchKernelGGL(bsim3_eval_staged_kernel<true>, grid, dim3(128), 0, s, d, ss, p,
+                       lt, b3_blocks);
   } else if (v == XG_EVAL_STAGED) {
-    hipLaunchKernelGGL(bsim3_eval_staged_kernel<false>, dim3(waves), dim3(64),
-                       0, s, d, ss, p);
+    hipLaunchKernelGGL(bsim3_eval_staged_kernel<false>, grid, dim3(64), 0, s, d, ss, p,
+                       lt, b3_blocks);
   } else {
-    const int grid = (d.n_slots + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(bsim3_eval_kernel, dim3(grid), dim3(kBlock), 0, s, d,
-                       ss, p);
+    hipLaunchKernelGGL(bsim3_eval_kernel, grid, dim3(kBlock), 0, s, d, ss, p, lt,
+                       b3_blocks);
   }
   return hipGetLastError();
 }

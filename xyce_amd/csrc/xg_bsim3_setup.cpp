@@ -16,6 +16,8 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <string>
+#include <unordered_map>
 
 #include "xg_bsim3.h"
 #include "xg_bsim3_layout.h"
@@ -173,10 +175,16 @@ int bsim3_add_model(Circuit &c, int dtype, const ParamSet &p, int32_t *id,
     m->v[k] = kDefaults[k];
     m->g[k] = false;
   }
+  // name -> parameter index, built once (thread-safe static init)
+  static const std::unordered_map<std::string, int> index = [] {
+    std::unordered_map<std::string, int> m;
+    for (int k = 0; k < P_COUNT; ++k) m.emplace(kNames[k], k);
+    return m;
+  }();
   for (auto &kv : p.v) {
     if (kv.first == "LEVEL") continue;
-    int k = 0;
-    while (k < P_COUNT && kv.first != kNames[k]) ++k;
+    const auto it = index.find(kv.first);
+    const int k = it == index.end() ? P_COUNT : it->second;
     if (k == P_COUNT) {
       err = "unknown BSIM3 model parameter " + kv.first;
       delete m;

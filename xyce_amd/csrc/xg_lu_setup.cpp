@@ -29,9 +29,11 @@
 // Everything here uses component-local indices only, so a sub-circuit gets
 // the same factorisation alone and inside a batch.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "xg_internal.h"
@@ -213,6 +215,295 @@ bool match_rows(const Graph &rows, int32_t n, const std::vector<int32_t> &pair_r
   return true;
 }
 
+// One component's static factorisation, with task indices local to the
+// component (lu_analyse appends the components in order).  The component's
+// entries of prow / qcol / qpos / lu_pos (disjoint from every other
+// component's) are written in place.
+struct CompLu {
+  int rc = XG_OK;
+  std::string err;
+  bool ok = false, in_lds = true;
+  int64_t fn = 0, factor_ops = 0;
+  int nlev[3] = {0, 0, 0};
+  std::vector<int32_t> t_dst, t_div, ops;
+  std::vector<int32_t> t_end;                    // per task: end of its ops (pairs)
+  std::vector<int32_t> lv_end, lv_mid, lv_long;  // per level, task indices
+};
+
+void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
+                       const std::vector<std::pair<int32_t, int32_t>> &pairs, CompLu &out) {
+  const int32_t G = plan.n_nodes;
+  const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
+  const int32_t *nodes = plan.comp_nodes.data() + beg;
+  auto loc = [&](int32_t g) { return plan.comp_local[g]; };
+  // local structure of the component's rows
+  Graph rows(n);
+  for (int32_t r = 0; r < n; ++r) {
+    const int32_t g = nodes[r];
+    for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
+      const int32_t col = plan.colidx[k];
+      if (col < G) rows[r].push_back(loc(col));
+    }
+    std::sort(rows[r].begin(), rows[r].end());
+    rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
+  }
+  std::vector<int32_t> pr, pc;
+  for (auto &p : pairs) {
+    pr.push_back(loc(p.first));
+    pc.push_back(loc(p.second));
+  }
+  std::vector<int32_t> col_of_row, row_of_col;
+  if (!match_rows(rows, n, pr, pc, col_of_row, row_of_col)) {
+    for (int32_t i = 0; i < n; ++i) {
+      L.prow[beg + i] = nodes[i];
+      L.qcol[beg + i] = nodes[i];
+      L.qpos[nodes[i]] = i;
+    }
+    return;
+  }
+  // symmetrised graph of the matched matrix M(c, j) = A(row_of_col[c], j)
+  Graph sym(n);
+  for (int32_t cc = 0; cc < n; ++cc)
+    for (int32_t j : rows[row_of_col[cc]])
+      if (j != cc) {
+        sym[cc].push_back(j);
+        sym[j].push_back(cc);
+      }
+  size_t edges = 0;
+  for (auto &a : sym) {
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+    edges += a.size();
+  }
+  // ordering: nested dissection; rows of very high degree last
+  const double avg = n ? (double)edges / n : 0.0;
+  const size_t hub = (size_t)std::max(32.0, std::max(8.0 * avg, 4.0 * std::sqrt((double)n)));
+  std::vector<int32_t> order;
+  order.reserve(n);
+  {
+    Dissector d(sym, order);
+    std::vector<int32_t> rest, hubs;
+    for (int32_t v = 0; v < n; ++v) (sym[v].size() > hub ? hubs : rest).push_back(v);
+    for (int32_t v : rest) d.in[v] = 0;
+    d.next_tag = 1;
+    d.run(rest, 0);
+    std::stable_sort(hubs.begin(), hubs.end(), [&](int32_t a, int32_t b) {
+      return sym[a].size() < sym[b].size();
+    });
+    for (int32_t v : hubs) order.push_back(v);
+  }
+  std::vector<int32_t> pinv(n);
+  for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
+  for (int32_t i = 0; i < n; ++i) {
+    L.qcol[beg + i] = nodes[order[i]];
+    L.qpos[nodes[order[i]]] = i;
+    L.prow[beg + i] = nodes[row_of_col[order[i]]];
+  }
+  // elimination tree and row structure of L (Liu's row-subtree walk),
+  // abandoned when the fill or the factorisation work exceeds a budget
+  // (the component then has no static factorisation)
+  std::vector<std::vector<int32_t>> lrow(n), ucol(n);
+  std::vector<int32_t> parent(n, -1), mark(n, -1);
+  int64_t fill = 0;
+  bool too_big = false;
+  for (int32_t i = 0; i < n && !too_big; ++i) {
+    mark[i] = i;
+    for (int32_t u : sym[order[i]]) {
+      int32_t j = pinv[u];
+      if (j >= i) continue;
+      while (mark[j] != i) {
+        lrow[i].push_back(j);
+        mark[j] = i;
+        if (parent[j] < 0) {
+          parent[j] = i;
+          break;
+        }
+        j = parent[j];
+      }
+    }
+    std::sort(lrow[i].begin(), lrow[i].end());
+    for (int32_t j : lrow[i]) ucol[j].push_back(i);
+    fill += (int64_t)lrow[i].size();
+    too_big = fill > kLuMaxFill;
+  }
+  if (!too_big) {
+    int64_t work = 0;  // multiply-adds: sum over pivots of (column count)^2
+    for (int32_t k = 0; k < n; ++k) work += (int64_t)ucol[k].size() * (int64_t)ucol[k].size();
+    too_big = work > kLuMaxOps;
+  }
+  if (too_big) {
+    out.in_lds = false;
+    return;
+  }
+  // filled row-major pattern: row i = lrow[i], i, ucol[i] (all sorted)
+  std::vector<int64_t> rp(n + 1, 0);
+  for (int32_t i = 0; i < n; ++i)
+    rp[i + 1] = rp[i] + (int64_t)lrow[i].size() + 1 + (int64_t)ucol[i].size();
+  const int64_t fn = rp[n];
+  if (fn > INT32_MAX) {
+    out.rc = XG_E_ARG;
+    out.err = "sparse LU: filled pattern of a component exceeds 2^31 entries";
+    return;
+  }
+  auto pos = [&](int32_t i, int32_t j) -> int32_t {
+    if (j < i) {
+      auto it = std::lower_bound(lrow[i].begin(), lrow[i].end(), j);
+      return (int32_t)(rp[i] + (it - lrow[i].begin()));
+    }
+    if (j == i) return (int32_t)(rp[i] + (int64_t)lrow[i].size());
+    auto it = std::lower_bound(ucol[i].begin(), ucol[i].end(), j);
+    return (int32_t)(rp[i] + (int64_t)lrow[i].size() + 1 + (it - ucol[i].begin()));
+  };
+  // U(m, j) exactly where L(j, m) is: uof[lo[j] + y] is the position of
+  // U(lrow[j][y], j) (ucol[m] lists its rows in ascending order)
+  std::vector<int64_t> lo(n + 1, 0);
+  for (int32_t i = 0; i < n; ++i) lo[i + 1] = lo[i] + (int64_t)lrow[i].size();
+  std::vector<int32_t> uof(lo[n]);
+  {
+    std::vector<int32_t> next(n, 0);
+    for (int32_t j = 0; j < n; ++j)
+      for (size_t y = 0; y < lrow[j].size(); ++y) {
+        const int32_t m = lrow[j][y];
+        uof[lo[j] + y] = (int32_t)(rp[m] + (int64_t)lrow[m].size() + 1 + next[m]++);
+      }
+  }
+  // CSR entry -> position: entry (g, col) of row g lands at
+  // (position of g's equation, position of col's unknown)
+  std::vector<int32_t> rpos(n);
+  for (int32_t i = 0; i < n; ++i) rpos[row_of_col[order[i]]] = i;
+  for (int32_t r = 0; r < n; ++r) {
+    const int32_t g = nodes[r];
+    for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
+      const int32_t col = plan.colidx[k];
+      if (col < G) L.lu_pos[k] = pos(rpos[r], pinv[loc(col)]);
+    }
+  }
+  out.fn = fn;
+  out.ok = true;
+  // ---- factor tasks: values in the order diag, column of L, row of U
+  std::vector<int32_t> lev(fn, -1);
+  struct Task {
+    int32_t dst, div, lev;
+    std::vector<int32_t> ops;
+  };
+  std::vector<Task> tasks;
+  // terms L(i, m) U(m, j) over m in lrow[i] and lrow[j] (m < min(i, j))
+  auto intersect = [&](int32_t i, int32_t j, std::vector<int32_t> &ops, int &lv) {
+    const auto &a = lrow[i], &b = lrow[j];
+    size_t x = 0, y = 0;
+    while (x < a.size() && y < b.size()) {
+      if (a[x] < b[y]) {
+        ++x;
+      } else if (b[y] < a[x]) {
+        ++y;
+      } else {
+        const int32_t pa = (int32_t)(rp[i] + (int64_t)x), pb = uof[lo[j] + y];
+        ops.push_back(pa);
+        ops.push_back(pb);
+        lv = std::max(lv, std::max(lev[pa], lev[pb]) + 1);
+        ++x;
+        ++y;
+      }
+    }
+  };
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t dp = (int32_t)(rp[p] + (int64_t)lrow[p].size());
+    {
+      Task t{dp, -2, 0, {}};
+      intersect(p, p, t.ops, t.lev);
+      lev[t.dst] = t.lev;
+      tasks.push_back(std::move(t));
+    }
+    for (int32_t i : ucol[p]) {  // L(i, p), i > p
+      Task t{pos(i, p), dp, lev[dp] + 1, {}};
+      intersect(i, p, t.ops, t.lev);
+      lev[t.dst] = t.lev;
+      tasks.push_back(std::move(t));
+    }
+    for (size_t z = 0; z < ucol[p].size(); ++z) {  // U(p, j), j > p
+      const int32_t j = ucol[p][z];
+      Task t{dp + 1 + (int32_t)z, -1, 0, {}};
+      intersect(p, j, t.ops, t.lev);
+      if (t.ops.empty()) continue;  // the scattered value is final
+      lev[t.dst] = t.lev;
+      tasks.push_back(std::move(t));
+    }
+  }
+  // ---- forward solve (L y = b, unit diagonal) and backward (U x = y)
+  std::vector<Task> fwd, bwd;
+  {
+    std::vector<int32_t> lf(n, -1), lb(n, -1);
+    for (int32_t i = 0; i < n; ++i) {
+      if (lrow[i].empty()) continue;
+      Task t{i, -1, 0, {}};
+      for (size_t x = 0; x < lrow[i].size(); ++x) {
+        const int32_t k = lrow[i][x];
+        t.ops.push_back((int32_t)(rp[i] + (int64_t)x));
+        t.ops.push_back(k);
+        t.lev = std::max(t.lev, lf[k] + 1);
+      }
+      lf[i] = t.lev;
+      fwd.push_back(std::move(t));
+    }
+    for (int32_t i = n - 1; i >= 0; --i) {
+      const int32_t di = (int32_t)(rp[i] + (int64_t)lrow[i].size());
+      Task t{i, di, 0, {}};
+      for (size_t z = 0; z < ucol[i].size(); ++z) {
+        const int32_t j = ucol[i][z];
+        t.ops.push_back(di + 1 + (int32_t)z);
+        t.ops.push_back(j);
+        t.lev = std::max(t.lev, lb[j] + 1);
+      }
+      lb[i] = t.lev;
+      bwd.push_back(std::move(t));
+    }
+  }
+  // ---- the three phases, tasks bucketed by level (short tasks first, then
+  // mid, then long within a level)
+  auto emit = [&](std::vector<Task> &ts, int ph) {
+    int nlev = 0;
+    for (auto &t : ts) nlev = std::max(nlev, t.lev + 1);
+    std::vector<std::vector<int32_t>> by(nlev);
+    for (size_t k = 0; k < ts.size(); ++k) by[ts[k].lev].push_back((int32_t)k);
+    for (auto &lv : by) {
+      auto cls = [&](int32_t k) {
+        const size_t m = ts[k].ops.size() / 2;
+        return m <= (size_t)kLuShortOps ? 0 : m <= (size_t)kLuLongOps ? 1 : 2;
+      };
+      std::stable_sort(lv.begin(), lv.end(),
+                       [&](int32_t x, int32_t y) { return cls(x) < cls(y); });
+      int32_t first_mid = -1, first_long = -1;
+      for (int32_t k : lv) {
+        if (first_mid < 0 && cls(k) >= 1) first_mid = (int32_t)out.t_dst.size();
+        if (first_long < 0 && cls(k) == 2) first_long = (int32_t)out.t_dst.size();
+        const Task &t = ts[k];
+        out.t_dst.push_back(t.dst);
+        out.t_div.push_back(t.div);
+        out.ops.insert(out.ops.end(), t.ops.begin(), t.ops.end());
+        out.t_end.push_back((int32_t)(out.ops.size() / 2));
+        if (ph == 0) out.factor_ops += (int64_t)t.ops.size() / 2;
+      }
+      const int32_t end = (int32_t)out.t_dst.size();
+      out.lv_long.push_back(first_long < 0 ? end : first_long);
+      out.lv_mid.push_back(first_mid < 0 ? out.lv_long.back() : first_mid);
+      out.lv_end.push_back(end);
+    }
+    out.nlev[ph] = nlev;
+    std::vector<Task>().swap(ts);
+  };
+  emit(tasks, 0);
+  emit(fwd, 1);
+  emit(bwd, 2);
+  out.in_lds = sparse_lu_lds_bytes(n, fn, (int64_t)out.t_dst.size(),
+                                   (int64_t)out.ops.size() / 2,
+                                   out.nlev[0] + out.nlev[1] + out.nlev[2]) <=
+               sparse_lu_lds_budget();
+  if (out.ops.size() / 2 > (size_t)INT32_MAX) {
+    out.rc = XG_E_ARG;
+    out.err = "sparse LU: task lists exceed 2^31 entries";
+  }
+}
+
 }  // namespace
 
 int lu_analyse(Plan &plan, std::string &err) {
@@ -233,7 +524,6 @@ int lu_analyse(Plan &plan, std::string &err) {
   L.lv_ptr.assign(1, 0);
   L.t_op.assign(1, 0);
   // voltage-source pairings (branch row, node column), global ids
-  std::vector<int32_t> vs_of_node(n_all + 1, -1);
   std::vector<std::pair<int32_t, int32_t>> vpairs;
   for (int32_t s = 0; s < plan.lin_slots; ++s) {
     if (plan.lin_type[s] != LIN_V) continue;
@@ -250,271 +540,71 @@ int lu_analyse(Plan &plan, std::string &err) {
       comp_of[plan.comp_nodes[j]] = c;
   for (auto &pr : vpairs) comp_pairs[comp_of[pr.first]].push_back(pr);
 
+  // components analysed on a pool of host threads (largest first), then
+  // appended in component order, so the plan does not depend on the pool
+  std::vector<CompLu> res(nc);
+  std::vector<int32_t> by_size(nc);
+  for (int32_t c = 0; c < nc; ++c) by_size[c] = c;
+  std::stable_sort(by_size.begin(), by_size.end(), [&](int32_t a, int32_t b) {
+    return plan.comp_start[a + 1] - plan.comp_start[a] >
+           plan.comp_start[b + 1] - plan.comp_start[b];
+  });
+  std::atomic<int32_t> next{0};
+  auto work = [&]() {
+    for (int32_t k; (k = next.fetch_add(1)) < nc;) {
+      const int32_t c = by_size[k];
+      analyse_component(plan, L, c, comp_pairs[c], res[c]);
+    }
+  };
+  const int32_t n_threads =
+      std::max(1, std::min<int32_t>({nc, 16, (int32_t)std::thread::hardware_concurrency()}));
+  {
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < n_threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+  }
   for (int32_t c = 0; c < nc; ++c) {
-    const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
-    const int32_t *nodes = plan.comp_nodes.data() + beg;
-    auto loc = [&](int32_t g) { return plan.comp_local[g]; };
-    // local structure of the component's rows
-    Graph rows(n);
-    for (int32_t r = 0; r < n; ++r) {
-      const int32_t g = nodes[r];
-      for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
-        const int32_t col = plan.colidx[k];
-        if (col < G) rows[r].push_back(loc(col));
-      }
-      std::sort(rows[r].begin(), rows[r].end());
-      rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
+    CompLu &r = res[c];
+    if (r.rc != XG_OK) {
+      err = r.err;
+      return r.rc;
     }
-    std::vector<int32_t> pr, pc;
-    for (auto &p : comp_pairs[c]) {
-      pr.push_back(loc(p.first));
-      pc.push_back(loc(p.second));
-    }
-    std::vector<int32_t> col_of_row, row_of_col;
-    const bool matched = match_rows(rows, n, pr, pc, col_of_row, row_of_col);
-    // L.voff stays flat for a component without a static factorisation
-    if (!matched) {
+    if (!r.ok) {
       L.n_static_fail++;
       for (int ph = 0; ph < 3; ++ph)
         L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
       L.voff[c + 1] = L.voff[c];
-      for (int32_t i = 0; i < n; ++i) {
-        L.prow[beg + i] = nodes[i];
-        L.qcol[beg + i] = nodes[i];
-        L.qpos[nodes[i]] = i;
-      }
+      L.in_lds[c] = r.in_lds ? 1 : 0;
       continue;
     }
-    // symmetrised graph of the matched matrix M(c, j) = A(row_of_col[c], j)
-    Graph sym(n);
-    for (int32_t cc = 0; cc < n; ++cc)
-      for (int32_t j : rows[row_of_col[cc]])
-        if (j != cc) {
-          sym[cc].push_back(j);
-          sym[j].push_back(cc);
-        }
-    size_t edges = 0;
-    for (auto &a : sym) {
-      std::sort(a.begin(), a.end());
-      a.erase(std::unique(a.begin(), a.end()), a.end());
-      edges += a.size();
-    }
-    // ordering: nested dissection; rows of very high degree last
-    const double avg = n ? (double)edges / n : 0.0;
-    const size_t hub = (size_t)std::max(32.0, std::max(8.0 * avg, 4.0 * std::sqrt((double)n)));
-    std::vector<int32_t> order;
-    order.reserve(n);
-    {
-      Dissector d(sym, order);
-      std::vector<int32_t> rest, hubs;
-      for (int32_t v = 0; v < n; ++v) (sym[v].size() > hub ? hubs : rest).push_back(v);
-      for (int32_t v : rest) d.in[v] = 0;
-      d.next_tag = 1;
-      d.run(rest, 0);
-      std::stable_sort(hubs.begin(), hubs.end(), [&](int32_t a, int32_t b) {
-        return sym[a].size() < sym[b].size();
-      });
-      for (int32_t v : hubs) order.push_back(v);
-    }
-    std::vector<int32_t> pinv(n);
-    for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
-    for (int32_t i = 0; i < n; ++i) {
-      L.qcol[beg + i] = nodes[order[i]];
-      L.qpos[nodes[order[i]]] = i;
-      L.prow[beg + i] = nodes[row_of_col[order[i]]];
-    }
-    // elimination tree and row structure of L (Liu's row-subtree walk),
-    // abandoned when the fill or the factorisation work exceeds a budget
-    // (the component then has no static factorisation)
-    std::vector<std::vector<int32_t>> lrow(n), ucol(n);
-    std::vector<int32_t> parent(n, -1), mark(n, -1);
-    int64_t fill = 0;
-    bool too_big = false;
-    for (int32_t i = 0; i < n && !too_big; ++i) {
-      mark[i] = i;
-      for (int32_t u : sym[order[i]]) {
-        int32_t j = pinv[u];
-        if (j >= i) continue;
-        while (mark[j] != i) {
-          lrow[i].push_back(j);
-          mark[j] = i;
-          if (parent[j] < 0) {
-            parent[j] = i;
-            break;
-          }
-          j = parent[j];
-        }
-      }
-      std::sort(lrow[i].begin(), lrow[i].end());
-      for (int32_t j : lrow[i]) ucol[j].push_back(i);
-      fill += (int64_t)lrow[i].size();
-      too_big = fill > kLuMaxFill;
-    }
-    if (!too_big) {
-      int64_t work = 0;  // multiply-adds: sum over pivots of (column count)^2
-      for (int32_t k = 0; k < n; ++k) work += (int64_t)ucol[k].size() * (int64_t)ucol[k].size();
-      too_big = work > kLuMaxOps;
-    }
-    if (too_big) {
-      L.n_static_fail++;
-      for (int ph = 0; ph < 3; ++ph)
-        L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
-      L.voff[c + 1] = L.voff[c];
-      L.in_lds[c] = 0;
-      continue;
-    }
-    // filled row-major pattern: row i = lrow[i], i, ucol[i] (all sorted)
-    std::vector<int64_t> rp(n + 1, 0);
-    for (int32_t i = 0; i < n; ++i)
-      rp[i + 1] = rp[i] + (int64_t)lrow[i].size() + 1 + (int64_t)ucol[i].size();
-    const int64_t fn = rp[n];
-    if (fn > INT32_MAX) {
-      err = "sparse LU: filled pattern of a component exceeds 2^31 entries";
-      return XG_E_ARG;
-    }
-    auto pos = [&](int32_t i, int32_t j) -> int32_t {
-      if (j < i) {
-        auto it = std::lower_bound(lrow[i].begin(), lrow[i].end(), j);
-        return (int32_t)(rp[i] + (it - lrow[i].begin()));
-      }
-      if (j == i) return (int32_t)(rp[i] + (int64_t)lrow[i].size());
-      auto it = std::lower_bound(ucol[i].begin(), ucol[i].end(), j);
-      return (int32_t)(rp[i] + (int64_t)lrow[i].size() + 1 + (it - ucol[i].begin()));
-    };
-    // CSR entry -> position: entry (g, col) of row g lands at
-    // (position of g's equation, position of col's unknown)
-    std::vector<int32_t> rpos(n);
-    for (int32_t i = 0; i < n; ++i) rpos[row_of_col[order[i]]] = i;
-    for (int32_t r = 0; r < n; ++r) {
-      const int32_t g = nodes[r];
-      for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
-        const int32_t col = plan.colidx[k];
-        if (col < G) L.lu_pos[k] = pos(rpos[r], pinv[loc(col)]);
-      }
-    }
-    L.voff[c + 1] = L.voff[c] + fn;
-    L.fnnz += fn;
+    L.voff[c + 1] = L.voff[c] + r.fn;
+    L.fnnz += r.fn;
     L.ok[c] = 1;
-    // ---- factor tasks: values in the order diag, column of L, row of U
-    std::vector<int32_t> lev(fn, -1);
-    struct Task {
-      int32_t dst, div, lev;
-      std::vector<int32_t> ops;
-    };
-    std::vector<Task> tasks;
-    auto intersect = [&](int32_t i, int32_t j, std::vector<int32_t> &ops, int &lv) {
-      const auto &a = lrow[i], &b = lrow[j];
-      size_t x = 0, y = 0;
-      while (x < a.size() && y < b.size()) {
-        if (a[x] < b[y]) {
-          ++x;
-        } else if (b[y] < a[x]) {
-          ++y;
-        } else {
-          const int32_t m = a[x];
-          const int32_t pa = pos(i, m), pb = pos(m, j);
-          ops.push_back(pa);
-          ops.push_back(pb);
-          lv = std::max(lv, std::max(lev[pa], lev[pb]) + 1);
-          ++x;
-          ++y;
-        }
-      }
-    };
-    for (int32_t p = 0; p < n; ++p) {
-      {
-        Task t{pos(p, p), -2, 0, {}};
-        intersect(p, p, t.ops, t.lev);
-        lev[t.dst] = t.lev;
-        tasks.push_back(std::move(t));
-      }
-      const int32_t dp = pos(p, p);
-      for (int32_t i : ucol[p]) {  // L(i, p), i > p
-        Task t{pos(i, p), dp, lev[dp] + 1, {}};
-        intersect(i, p, t.ops, t.lev);
-        lev[t.dst] = t.lev;
-        tasks.push_back(std::move(t));
-      }
-      for (int32_t j : ucol[p]) {  // U(p, j), j > p
-        Task t{pos(p, j), -1, 0, {}};
-        intersect(p, j, t.ops, t.lev);
-        if (t.ops.empty()) continue;  // the scattered value is final
-        lev[t.dst] = t.lev;
-        tasks.push_back(std::move(t));
-      }
-    }
-    // ---- forward solve (L y = b, unit diagonal) and backward (U x = y)
-    std::vector<Task> fwd, bwd;
-    {
-      std::vector<int32_t> lf(n, -1), lb(n, -1);
-      for (int32_t i = 0; i < n; ++i) {
-        if (lrow[i].empty()) continue;
-        Task t{i, -1, 0, {}};
-        for (int32_t k : lrow[i]) {
-          t.ops.push_back(pos(i, k));
-          t.ops.push_back(k);
-          t.lev = std::max(t.lev, lf[k] + 1);
-        }
-        lf[i] = t.lev;
-        fwd.push_back(std::move(t));
-      }
-      for (int32_t i = n - 1; i >= 0; --i) {
-        Task t{i, pos(i, i), 0, {}};
-        for (int32_t j : ucol[i]) {
-          t.ops.push_back(pos(i, j));
-          t.ops.push_back(j);
-          t.lev = std::max(t.lev, lb[j] + 1);
-        }
-        lb[i] = t.lev;
-        bwd.push_back(std::move(t));
-      }
-    }
-    // ---- emit the three phases, tasks bucketed by level
-    auto emit = [&](std::vector<Task> &ts, int ph) {
-      int nlev = 0;
-      for (auto &t : ts) nlev = std::max(nlev, t.lev + 1);
-      std::vector<std::vector<int32_t>> by(nlev);
-      for (size_t k = 0; k < ts.size(); ++k) by[ts[k].lev].push_back((int32_t)k);
-      for (auto &lv : by) {
-        // short tasks first, then mid, then long
-        auto cls = [&](int32_t k) {
-          const size_t m = ts[k].ops.size() / 2;
-          return m <= (size_t)kLuShortOps ? 0 : m <= (size_t)kLuLongOps ? 1 : 2;
-        };
-        std::stable_sort(lv.begin(), lv.end(),
-                         [&](int32_t x, int32_t y) { return cls(x) < cls(y); });
-        int32_t first_mid = -1, first_long = -1;
-        for (int32_t k : lv) {
-          if (first_mid < 0 && cls(k) >= 1) first_mid = (int32_t)L.t_dst.size();
-          if (first_long < 0 && cls(k) == 2) first_long = (int32_t)L.t_dst.size();
-          const Task &t = ts[k];
-          L.t_dst.push_back(t.dst);
-          L.t_div.push_back(t.div);
-          L.ops.insert(L.ops.end(), t.ops.begin(), t.ops.end());
-          L.t_op.push_back((int32_t)(L.ops.size() / 2));
-          if (ph == 0) L.factor_ops += (int64_t)t.ops.size() / 2;
-        }
-        const int32_t end = (int32_t)L.t_dst.size();
-        L.lv_long.push_back(first_long < 0 ? end : first_long);
-        L.lv_mid.push_back(first_mid < 0 ? L.lv_long.back() : first_mid);
-        L.lv_ptr.push_back((int32_t)L.t_dst.size());
-      }
-      L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
-      return nlev;
-    };
-    const int64_t t0 = (int64_t)L.t_dst.size(), o0 = (int64_t)L.ops.size() / 2;
-    const int f = emit(tasks, 0);
-    const int a = emit(fwd, 1);
-    const int b = emit(bwd, 2);
-    L.in_lds[c] = sparse_lu_lds_bytes(n, fn, (int64_t)L.t_dst.size() - t0,
-                                      (int64_t)L.ops.size() / 2 - o0, f + a + b) <=
-                  sparse_lu_lds_budget();
-    L.max_factor_levels = std::max(L.max_factor_levels, f);
-    L.max_solve_levels = std::max(L.max_solve_levels, a + b);
-    if (L.ops.size() / 2 > (size_t)INT32_MAX || L.t_dst.size() > (size_t)INT32_MAX) {
+    L.in_lds[c] = r.in_lds ? 1 : 0;
+    const int64_t tb = (int64_t)L.t_dst.size(), ob = (int64_t)L.ops.size() / 2;
+    if (tb + (int64_t)r.t_dst.size() > INT32_MAX ||
+        ob + (int64_t)r.ops.size() / 2 > INT32_MAX) {
       err = "sparse LU: task lists exceed 2^31 entries";
       return XG_E_ARG;
     }
+    L.t_dst.insert(L.t_dst.end(), r.t_dst.begin(), r.t_dst.end());
+    L.t_div.insert(L.t_div.end(), r.t_div.begin(), r.t_div.end());
+    L.ops.insert(L.ops.end(), r.ops.begin(), r.ops.end());
+    for (int32_t e : r.t_end) L.t_op.push_back((int32_t)(ob + e));
+    size_t lvl = 0;
+    for (int ph = 0; ph < 3; ++ph) {
+      for (int k = 0; k < r.nlev[ph]; ++k, ++lvl) {
+        L.lv_long.push_back((int32_t)(tb + r.lv_long[lvl]));
+        L.lv_mid.push_back((int32_t)(tb + r.lv_mid[lvl]));
+        L.lv_ptr.push_back((int32_t)(tb + r.lv_end[lvl]));
+      }
+      L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
+    }
+    L.factor_ops += r.factor_ops;
+    L.max_factor_levels = std::max(L.max_factor_levels, r.nlev[0]);
+    L.max_solve_levels = std::max(L.max_solve_levels, r.nlev[1] + r.nlev[2]);
+    r = CompLu();  // release the component's lists
   }
   // entry lists (every component, with or without a static factorisation)
   for (int32_t c = 0; c < nc; ++c) {
