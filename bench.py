@@ -41,14 +41,16 @@ HBM_PEAK_GBS = 8000.0
 # the default --gates only.
 B3_EVAL_TRAFFIC_BYTES = 29.66e6
 # Eval waves up to which the library's automatic choice is the LDS-staged
-# split kernel (one wave per 64 instances), else the HBM-record kernel.
-B3_STAGED_MAX_WAVES = 3 * 256
+# split kernel (one eval wave per 64 instances; two eval waves per workgroup
+# above 256), else the HBM-record kernel.
+B3_STAGED_MAX_WAVES = 2 * 256
 
 
 def b3_kernel_name(n_b3):
     waves = (n_b3 + 63) // 64
-    return ("bsim3_eval_staged_kernel<true>" if waves <= B3_STAGED_MAX_WAVES
-            else "bsim3_eval_kernel")
+    if waves > B3_STAGED_MAX_WAVES:
+        return "bsim3_eval_kernel"
+    return "bsim3_eval_staged_kernel<true, %d>" % (2 if waves > 256 else 1)
 
 
 def progress(msg):

@@ -103,6 +103,29 @@ def test_bsim3_random_logic_full_size_parity():
     check(*run_pair(nl, SOLVER_STATES[5], 11, torch, vhi=1.5))
 
 
+def test_bsim3_two_eval_waves_per_workgroup_odd_count():
+    """Above 256 eval waves the split kernel packs two eval waves into each
+    workgroup; with an odd count the last workgroup's second group has no
+    instances.  4,200 gates give 2 x 132 waves; one extra card of ten
+    NMOS adds the 265th."""
+    import torch
+    from xyce_amd import circuits
+    logic = circuits.random_logic_bsim3(n_gates=4200, seed=5)
+    extra = circuits.Netlist("one card")
+    extra.add_model("NX", "bsim3", 1, **dict(circuits.BSIM3_NMOS_130, MOBMOD=2, TNOM=40.0))
+    for i in range(10):
+        extra.add_mos("MX%d" % i, "x%d" % i, "xg", "0", "0", "NX", W=1e-6, L=0.13e-6)
+    nl = circuits.merge([logic, extra])
+    counts = {}
+    for m in nl.mos:
+        if nl.models[m.model].kind == "bsim3":
+            counts[m.model] = counts.get(m.model, 0) + 1
+    waves = sum((c + 63) // 64 for c in counts.values())
+    assert waves > 256 and waves % 2 == 1, waves
+    print("case %d eval waves" % waves, flush=True)
+    check(*run_pair(nl, SOLVER_STATES[5], 13, torch, vhi=1.5))
+
+
 def test_bsim3_monte_carlo_cards_parity():
     """Per-device model cards (VTH0 / TOX perturbed) share one LDS card per
     polarity; results still match the oracle's per-model evaluation."""

@@ -1479,33 +1479,42 @@ __global__ void __launch_bounds__(kBlock)
 // dependent and junction fields, kRecN x 64 doubles, 50.7 KB) are copied to
 // LDS with every load in flight at once, so the evaluation never waits on
 // HBM for a parameter.
-// kSplit: one workgroup of two waves per eval wave, wave 0 the kDC part and
-// wave 1 the kCV part of the same 64 instances, sharing one LDS copy of the
-// records (each wave stages half of the fields).  Both waves read their
-// inputs -- including the store the previous Newton iteration wrote, which
-// the kDC wave overwrites -- before the workgroup barrier, and write only
-// after it, so the in-place store update is race-free.
+// kSplit: two waves per eval wave, wave 0 the kDC part and wave 1 the kCV
+// part of the same 64 instances, sharing one LDS copy of the records (each
+// wave stages half of the fields).  Both waves read their inputs --
+// including the store the previous Newton iteration wrote, which the kDC
+// wave overwrites -- before the workgroup barrier, and write only after it,
+// so the in-place store update is race-free.
+// kGroups: eval waves per workgroup, each with its own LDS copy.  Two keep
+// the workgroup count within the 256 CUs up to 512 eval waves, so every wave
+// of the launch has a SIMD to itself (a second workgroup on a CU put a
+// kDC wave on a busy SIMD and made it half as slow again).
 constexpr int kRecN = B3R_N + B3I_N;
 
-template <bool kSplit>
-__global__ void __launch_bounds__(kSplit ? 128 : 64)
+template <bool kSplit, int kGroups>
+__global__ void __launch_bounds__((kSplit ? 128 : 64) * kGroups)
     bsim3_eval_staged_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
                              int32_t b3_blocks) {
-  __shared__ double rec[kRecN * 64];
-  __shared__ double card[B3C_N];
+  constexpr int kWaves = kSplit ? 2 : 1;     // waves per eval wave
+  constexpr int kThreads = 64 * kWaves * kGroups;
+  __shared__ double rec[kGroups][kRecN * 64];
+  __shared__ double card[kGroups][B3C_N];
   if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
     lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
-                         ((int)blockIdx.x - b3_blocks) * (kSplit ? 128 : 64) +
-                             threadIdx.x);
+                         ((int)blockIdx.x - b3_blocks) * kThreads + threadIdx.x);
     return;
   }
   const int lane = threadIdx.x & 63;
-  const int wave = kSplit ? (int)(threadIdx.x >> 6) : 0;
-  constexpr int kWaves = kSplit ? 2 : 1;
+  const int wave = kSplit ? (int)((threadIdx.x >> 6) % kWaves) : 0;
+  const int grp = (int)threadIdx.x / (64 * kWaves);
   constexpr int kPer = (kRecN + kWaves - 1) / kWaves;
-  const int w = blockIdx.x;
-  const int slot = w * 64 + lane;
-  const double *src = a.rec + (size_t)w * kRecN * 64 + lane;
+  const int w = (int)blockIdx.x * kGroups + grp;
+  // the last workgroup's second group may have no eval wave: it loads
+  // wave 0's fields (always valid) and evaluates nothing
+  const bool valid = w < (a.n_slots >> 6);
+  const int wl = valid ? w : 0;
+  const int slot = wl * 64 + lane;
+  const double *src = a.rec + (size_t)wl * kRecN * 64 + lane;
   double v[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
@@ -1514,32 +1523,34 @@ __global__ void __launch_bounds__(kSplit ? 128 : 64)
   }
   B3Inputs in;
   b3_inputs(a, ss, p, slot, in);
-  const bool live = a.inst_of[slot] >= 0;
+  const bool live = valid && a.inst_of[slot] >= 0;
+  double *R0 = rec[grp];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int f = j * kWaves + wave;
-    if (f < kRecN) rec[f * 64 + lane] = v[j];
+    if (f < kRecN) R0[f * 64 + lane] = v[j];
   }
-  if (threadIdx.x < B3C_N) card[threadIdx.x] = a.cards[a.wave_card[w] * B3C_N + threadIdx.x];
+  const int gt = (int)threadIdx.x % (64 * kWaves);
+  if (gt < B3C_N) card[grp][gt] = a.cards[a.wave_card[wl] * B3C_N + gt];
   __syncthreads();
   if (!live || skip_slot(p, slot)) return;
-  const double *R = rec + lane, *Gp = rec + B3R_N * 64 + lane;
+  const double *R = R0 + lane, *Gp = R0 + B3R_N * 64 + lane;
   if constexpr (kSplit) {
     if (wave)
-      b3_eval<true, kCV>(a, ss, p, slot, card, R, Gp, in);
+      b3_eval<true, kCV>(a, ss, p, slot, card[grp], R, Gp, in);
     else
-      b3_eval<true, kDC>(a, ss, p, slot, card, R, Gp, in);
+      b3_eval<true, kDC>(a, ss, p, slot, card[grp], R, Gp, in);
   } else {
-    b3_eval<true, kFull>(a, ss, p, slot, card, R, Gp, in);
+    b3_eval<true, kFull>(a, ss, p, slot, card[grp], R, Gp, in);
   }
 }
 
 }  // namespace
 
-// Eval waves up to which the LDS-staged kernels are used: their 50.7 KB of
-// LDS per workgroup allows three workgroups per CU on the 256 CUs.
+// Eval waves up to which the LDS-staged kernels are used: with two eval
+// waves (101 KB of LDS) per workgroup, 512 waves fill the 256 CUs once.
 #ifndef XG_B3_STAGED_MAX_WAVES
-#define XG_B3_STAGED_MAX_WAVES (3 * 256)
+#define XG_B3_STAGED_MAX_WAVES (2 * 256)
 #endif
 
 hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
@@ -1549,8 +1560,13 @@ hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
   int v = d.variant;
   if (v == XG_EVAL_AUTO)
     v = waves <= XG_B3_STAGED_MAX_WAVES ? XG_EVAL_STAGED_SPLIT : XG_EVAL_HBM;
-  const int block = v == XG_EVAL_STAGED_SPLIT ? 128 : (v == XG_EVAL_STAGED ? 64 : kBlock);
-  const int b3_blocks = v == XG_EVAL_HBM ? (d.n_slots + kBlock - 1) / kBlock : waves;
+  // split form: two eval waves per workgroup once one per workgroup would
+  // need more workgroups than CUs
+  const int groups = v == XG_EVAL_STAGED_SPLIT && waves > 256 ? 2 : 1;
+  const int block = v == XG_EVAL_STAGED_SPLIT ? 128 * groups
+                                              : (v == XG_EVAL_STAGED ? 64 : kBlock);
+  const int b3_blocks = v == XG_EVAL_HBM ? (d.n_slots + kBlock - 1) / kBlock
+                                         : (waves + groups - 1) / groups;
   LinTail lt{};
   int lin_blocks = 0;
   if (tail && tail->a.n_slots > 0) {
@@ -1558,12 +1574,15 @@ hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
     lin_blocks = (tail->a.n_slots + block - 1) / block;
   }
   const dim3 grid(b3_blocks + lin_blocks);
-  if (v == XG_EVAL_STAGED_SPLIT) {
-    hipLaunchKernelGGL(bsim3_eval_staged_kernel<true>, grid, dim3(128), 0, s, d, ss, p,
-                       lt, b3_blocks);
+  if (v == XG_EVAL_STAGED_SPLIT && groups == 2) {
+    hipLaunchKernelGGL((bsim3_eval_staged_kernel<true, 2>), grid, dim3(256), 0, s, d, ss,
+                       p, lt, b3_blocks);
+  } else if (v == XG_EVAL_STAGED_SPLIT) {
+    hipLaunchKernelGGL((bsim3_eval_staged_kernel<true, 1>), grid, dim3(128), 0, s, d, ss,
+                       p, lt, b3_blocks);
   } else if (v == XG_EVAL_STAGED) {
-    hipLaunchKernelGGL(bsim3_eval_staged_kernel<false>, grid, dim3(64), 0, s, d, ss, p,
-                       lt, b3_blocks);
+    hipLaunchKernelGGL((bsim3_eval_staged_kernel<false, 1>), grid, dim3(64), 0, s, d, ss,
+                       p, lt, b3_blocks);
   } else {
     hipLaunchKernelGGL(bsim3_eval_kernel, grid, dim3(kBlock), 0, s, d, ss, p, lt,
                        b3_blocks);
