@@ -15,6 +15,7 @@ which is not part of it).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -51,6 +52,18 @@ def b3_kernel_name(n_b3):
     if waves > B3_STAGED_MAX_WAVES:
         return "bsim3_eval_kernel"
     return "bsim3_eval_staged_kernel<true, %d>" % (2 if waves > 256 else 1)
+
+
+def _finite(o):
+    """Non-finite floats (an untimed mode's inf) as null, so the line is
+    strict JSON."""
+    if isinstance(o, dict):
+        return {k: _finite(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [_finite(v) for v in o]
+    if isinstance(o, float) and not math.isfinite(o):
+        return None
+    return o
 
 
 def progress(msg):
@@ -454,7 +467,7 @@ def main():
         progress("CPU baseline (%.0f s sample)" % args.cpu_seconds)
         sta_h, sto_h = sta.copy(), sto.copy()
         res["cpu_baseline"] = cpu_baseline(nl, x, sta_h, sto_h, args.cpu_seconds)
-    print(json.dumps(res), flush=True)
+    print(json.dumps(_finite(res)), flush=True)
 
 
 if __name__ == "__main__":

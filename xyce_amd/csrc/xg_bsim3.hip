@@ -1445,22 +1445,23 @@ __device__ __forceinline__ void b3_eval(const Bsim3Dev &a,
 // evaluation uses them (other waves hide the latency).
 __global__ void __launch_bounds__(kBlock)
     bsim3_eval_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
-                      int32_t b3_blocks) {
+                      int32_t lin_blocks) {
   __shared__ double card_lds[kWavesPerBlock][B3C_N];
-  if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
+  if ((int)blockIdx.x < lin_blocks) {  // a linear-device workgroup
     lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
-                         ((int)blockIdx.x - b3_blocks) * kBlock + threadIdx.x);
+                         (int)blockIdx.x * kBlock + threadIdx.x);
     return;
   }
+  const int bid = (int)blockIdx.x - lin_blocks;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
-  const int slot = blockIdx.x * kBlock + tid;
+  const int slot = bid * kBlock + tid;
   const int S = a.n_slots;
   {
     const int nwaves = S >> 6;
     for (int k = tid; k < kWavesPerBlock * B3C_N; k += kBlock) {
       const int w = k / B3C_N, f = k % B3C_N;
-      const int gw = blockIdx.x * kWavesPerBlock + w;
+      const int gw = bid * kWavesPerBlock + w;
       card_lds[w][f] = gw < nwaves ? a.cards[a.wave_card[gw] * B3C_N + f] : 0.0;
     }
   }
@@ -1494,21 +1495,21 @@ constexpr int kRecN = B3R_N + B3I_N;
 template <bool kSplit, int kGroups>
 __global__ void __launch_bounds__((kSplit ? 128 : 64) * kGroups)
     bsim3_eval_staged_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
-                             int32_t b3_blocks) {
+                             int32_t lin_blocks) {
   constexpr int kWaves = kSplit ? 2 : 1;     // waves per eval wave
   constexpr int kThreads = 64 * kWaves * kGroups;
   __shared__ double rec[kGroups][kRecN * 64];
   __shared__ double card[kGroups][B3C_N];
-  if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
+  if ((int)blockIdx.x < lin_blocks) {  // a linear-device workgroup
     lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
-                         ((int)blockIdx.x - b3_blocks) * kThreads + threadIdx.x);
+                         (int)blockIdx.x * kThreads + threadIdx.x);
     return;
   }
   const int lane = threadIdx.x & 63;
   const int wave = kSplit ? (int)((threadIdx.x >> 6) % kWaves) : 0;
   const int grp = (int)threadIdx.x / (64 * kWaves);
   constexpr int kPer = (kRecN + kWaves - 1) / kWaves;
-  const int w = (int)blockIdx.x * kGroups + grp;
+  const int w = ((int)blockIdx.x - lin_blocks) * kGroups + grp;
   // the last workgroup's second group may have no eval wave: it loads
   // wave 0's fields (always valid) and evaluates nothing
   const bool valid = w < (a.n_slots >> 6);
@@ -1563,10 +1564,10 @@ hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
   // split form: two eval waves per workgroup once one per workgroup would
   // need more workgroups than CUs
   const int groups = v == XG_EVAL_STAGED_SPLIT && waves > 256 ? 2 : 1;
-  const int block = v == XG_EVAL_STAGED_SPLIT ? 128 * groups
-                                              : (v == XG_EVAL_STAGED ? 64 : kBlock);
   const int b3_blocks = v == XG_EVAL_HBM ? (d.n_slots + kBlock - 1) / kBlock
                                          : (waves + groups - 1) / groups;
+  const int block = v == XG_EVAL_STAGED_SPLIT ? 128 * groups
+                                              : (v == XG_EVAL_STAGED ? 64 : kBlock);
   LinTail lt{};
   int lin_blocks = 0;
   if (tail && tail->a.n_slots > 0) {
@@ -1576,16 +1577,16 @@ hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
   const dim3 grid(b3_blocks + lin_blocks);
   if (v == XG_EVAL_STAGED_SPLIT && groups == 2) {
     hipLaunchKernelGGL((bsim3_eval_staged_kernel<true, 2>), grid, dim3(256), 0, s, d, ss,
-                       p, lt, b3_blocks);
+                       p, lt, lin_blocks);
   } else if (v == XG_EVAL_STAGED_SPLIT) {
     hipLaunchKernelGGL((bsim3_eval_staged_kernel<true, 1>), grid, dim3(128), 0, s, d, ss,
-                       p, lt, b3_blocks);
+                       p, lt, lin_blocks);
   } else if (v == XG_EVAL_STAGED) {
     hipLaunchKernelGGL((bsim3_eval_staged_kernel<false, 1>), grid, dim3(64), 0, s, d, ss,
-                       p, lt, b3_blocks);
+                       p, lt, lin_blocks);
   } else {
     hipLaunchKernelGGL(bsim3_eval_kernel, grid, dim3(kBlock), 0, s, d, ss, p, lt,
-                       b3_blocks);
+                       lin_blocks);
   }
   return hipGetLastError();
 }
