@@ -398,7 +398,9 @@ int xg_sizes(const xg_handle *h, int32_t *n_nodes, int32_t *n_state,
 
    The caller owns every vector (device pointers, as N_TIA_DataStore owns
    the reference's): lengths n1 = n_nodes + 1, nnz CSR values, n_comp
-   components, n_state / n_store. */
+   components, n_state / n_store.  nstats (max |rhs| at 4 c) and lim are
+   per-iteration accumulators: the driver clears them at xg_tran_create and
+   after every Newton iteration. */
 typedef struct xg_tran xg_tran;
 typedef struct xg_tran_buffers {
   double *x, *x_curr, *xh1, *xh2, *xpred;  /* n1: iterate, accepted, history */

@@ -42,6 +42,7 @@ struct Handle {
   SparseLuArgs lu{};
   int32_t *d_rowptr = nullptr, *d_colidx = nullptr;
   int32_t *slot_comp = nullptr;
+  int32_t *node_comp = nullptr;  // node -> component (ground excluded)
   // reduction programs
   ReduceDev red_vec[4]{}, red_mat[2]{};
   double *slots = nullptr;
@@ -293,11 +294,12 @@ int xg_components(const xg_handle *H, int32_t *n_comp, int32_t *comp_of_node) {
   return XG_OK;
 }
 
-int xg_tia_assemble(xg_handle *H, int32_t transient, int32_t order, double dt,
-                    const double *qn0, const double *fprev, const double *F,
-                    const double *Q, const double *dFdxdVp,
-                    const double *dQdxdVp, const double *dFdx,
-                    const double *dQdx, double *rhs, double *J) {
+// Assembly with the per-component max |rhs| folded in when norms is given.
+static int tia_assemble(xg_handle *H, int32_t transient, int32_t order, double dt,
+                        const double *qn0, const double *fprev, const double *F,
+                        const double *Q, const double *dFdxdVp, const double *dQdxdVp,
+                        const double *dFdx, const double *dQdx, double *rhs, double *J,
+                        double *norms) {
   if (H && H->h.plan_only)
     return fail(XG_E_STATE, "handle was finalized without a device");
   if (!H || !F || !dFdxdVp || !dFdx || !rhs || !J ||
@@ -325,8 +327,19 @@ int xg_tia_assemble(xg_handle *H, int32_t transient, int32_t order, double dt,
   a.dQdx = dQdx ? dQdx : dFdx;
   a.rhs = rhs;
   a.J = J;
+  a.node_comp = h.node_comp;
+  a.norms = norms;
   hipError_t e = xg::launch_tia_assemble(a, h.stream);
   return e == hipSuccess ? XG_OK : hip_fail(e, "assemble");
+}
+
+int xg_tia_assemble(xg_handle *H, int32_t transient, int32_t order, double dt,
+                    const double *qn0, const double *fprev, const double *F,
+                    const double *Q, const double *dFdxdVp,
+                    const double *dQdxdVp, const double *dFdx,
+                    const double *dQdx, double *rhs, double *J) {
+  return tia_assemble(H, transient, order, dt, qn0, fprev, F, Q, dFdxdVp, dQdxdVp, dFdx,
+                      dQdx, rhs, J, nullptr);
 }
 
 int xg_dense_solve(xg_handle *H, const double *J, const double *rhs, double *x,
@@ -482,6 +495,18 @@ hipError_t handle_count_flags(xg_handle *H, int64_t *out128) {
   Handle &h = H->h;
   return launch_count_flags(h.limited_all, h.n_limited_slots, out128, h.stream);
 }
+int handle_assemble_norms(xg_handle *H, int32_t transient, const double *qn0,
+                          const double *fprev, const double *F, const double *Q,
+                          const double *FL, const double *QL, const double *JF,
+                          const double *JQ, double *rhs, double *J, double *norms) {
+  return tia_assemble(H, transient, 1, 0.0, qn0, fprev, F, Q, FL, QL, JF, JQ, rhs, J,
+                      norms);
+}
+hipError_t handle_iteration_counts(xg_handle *H, int32_t *comp_cnt, int64_t *out128) {
+  Handle &h = H->h;
+  return launch_iteration_counts(h.limited_all, h.slot_comp, h.n_limited_slots, comp_cnt,
+                                 out128, h.stream);
+}
 hipStream_t handle_stream(const xg_handle *H) { return H->h.stream; }
 int32_t handle_n_comp(const xg_handle *H) {
   return (int32_t)H->h.plan.comp_start.size() - 1;
@@ -614,6 +639,13 @@ static bool upload_handle(xg_handle *H) {
   // circuit CSR and components for the Newton solve
   chk(h.d_rowptr = h.upload(P.rowptr));
   chk(h.slot_comp = h.upload(P.slot_comp));
+  {
+    std::vector<int32_t> nc((size_t)P.n_nodes, -1);
+    for (size_t k = 0; k + 1 < P.comp_start.size(); ++k)
+      for (int32_t j = P.comp_start[k]; j < P.comp_start[k + 1]; ++j)
+        nc[P.comp_nodes[j]] = (int32_t)k;
+    chk(h.node_comp = h.upload(nc));
+  }
   chk(h.d_colidx = h.upload(P.colidx));
   {
     auto &d = h.ds;

@@ -204,6 +204,11 @@ struct TiaAssembleArgs {
   const double *dstep = nullptr;  // if set: 1/dt = dstep[2], order = dstep[3]
   const double *qn0, *fprev, *F, *Q, *FL, *QL, *dFdx, *dQdx;
   double *rhs, *J;
+  // optional (the native Newton loop): max |rhs| per component folded into
+  // norms[4 c] (which must hold +0.0) by an exact atomic max; NaN entries
+  // are skipped, as fmax skips them
+  const int32_t *node_comp = nullptr;
+  double *norms = nullptr;
 };
 
 // Dense LU solve per connected component (ground excluded).
@@ -272,11 +277,24 @@ hipError_t launch_linear_eval(const LinArgs &a, const SolverStateDev &ss,
 hipError_t launch_component_limited(const int32_t *flags, const int32_t *slot_comp,
                                     int32_t n, int32_t *cnt, int32_t n_comp,
                                     hipStream_t s);
+hipError_t launch_iteration_counts(const int32_t *flags, const int32_t *slot_comp,
+                                   int32_t n, int32_t *comp_cnt, int64_t *out,
+                                   hipStream_t s);
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s);
 
 // Handle internals for the transient driver (xg_tran.hip).
 hipError_t handle_count_flags(xg_handle *h, int64_t *out128);
+// The native Newton loop's bookkeeping of one iteration in two launches:
+// assembly with the per-component max |rhs| (norms, zeroed beforehand),
+// then one pass over the instance flags for the per-component limiting
+// counts (comp_cnt, zeroed beforehand) and the 64-block limited /
+// non-finite counts (out128).
+int handle_assemble_norms(xg_handle *h, int32_t transient, const double *qn0,
+                          const double *fprev, const double *F, const double *Q,
+                          const double *FL, const double *QL, const double *JF,
+                          const double *JQ, double *rhs, double *J, double *norms);
+hipError_t handle_iteration_counts(xg_handle *h, int32_t *comp_cnt, int64_t *out128);
 hipStream_t handle_stream(const xg_handle *h);
 int32_t handle_n_comp(const xg_handle *h);
 int64_t handle_n_nodes(const xg_handle *h);

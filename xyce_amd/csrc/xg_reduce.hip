@@ -183,6 +183,39 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// count_kernel and comp_count_kernel in one pass (the native Newton loop;
+// cnt zeroed beforehand).  Grid: 64 blocks.
+__global__ void __launch_bounds__(256)
+    iteration_count_kernel(const int32_t *flags, const int32_t *slot_comp, int32_t n,
+                           int32_t *cnt, int64_t *out) {
+  __shared__ int64_t c2[2][256];
+  int64_t c0 = 0, c1 = 0;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const int32_t f = flags[i];
+    c0 += f & 1;
+    c1 += (f & 0x202) != 0;
+    if (f & 1) {
+      const int32_t c = slot_comp[i];
+      if (c >= 0) atomicAdd(cnt + c, 1);
+    }
+  }
+  c2[0][threadIdx.x] = c0;
+  c2[1][threadIdx.x] = c1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      c2[0][threadIdx.x] += c2[0][threadIdx.x + s];
+      c2[1][threadIdx.x] += c2[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = c2[0][0];
+    out[64 + blockIdx.x] = c2[1][0];
+  }
+}
+
 __global__ void zero_i32_kernel(int32_t *p, int32_t n) {
   const int32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -252,6 +285,14 @@ hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
 
 // Writes two partial counts per block (64 blocks: limited in out[0..63],
 // non-finite in out[64..127]); the caller sums them.
+hipError_t launch_iteration_counts(const int32_t *flags, const int32_t *slot_comp,
+                                   int32_t n, int32_t *comp_cnt, int64_t *out,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL(iteration_count_kernel, dim3(64), dim3(256), 0, s, flags, slot_comp,
+                     n, comp_cnt, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s) {
   hipLaunchKernelGGL(count_kernel, dim3(64), dim3(256), 0, s, flags, n, out);

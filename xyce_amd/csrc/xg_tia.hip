@@ -51,6 +51,14 @@ __global__ void __launch_bounds__(kBlock)
       if (a.limiter) r = r + 1.0 * a.FL[i];
     }
     a.rhs[i] = i == a.n1 - 1 ? 0.0 : r;  // the ground slot carries nothing
+    if (a.norms && i < a.n1 - 1) {
+      // non-negative doubles order as their bit patterns: an exact max
+      const int32_t c = a.node_comp[i];
+      const double m = fabs(r);
+      if (c >= 0 && m == m)
+        atomicMax(reinterpret_cast<unsigned long long *>(a.norms + 4 * (size_t)c),
+                  (unsigned long long)__double_as_longlong(m));
+    }
   }
   if (i < a.nnz) {
     if (a.transient) {

@@ -117,9 +117,9 @@ __global__ void __launch_bounds__(kBlock)
 //   (max|rhs| <= RHSTol and wRMS(dx) <= deltaXTol, or wRMS(dx) <= 1e-6);
 // a converged component stays converged and leaves the active mask.
 __global__ void __launch_bounds__(kBlock)
-    newton_ctl_kernel(int32_t n_comp, const double *stats, const double *nstats,
-                      const int32_t *lim, const int64_t *counts, NewtonParams prm,
-                      int32_t *conv, int32_t *active, int32_t *ctl) {
+    newton_ctl_kernel(int32_t n_comp, const double *stats, double *nstats, int32_t *lim,
+                      const int64_t *counts, NewtonParams prm, int32_t *conv,
+                      int32_t *active, int32_t *ctl) {
   __shared__ int32_t s_left, s_sing;
   const int t = threadIdx.x;
   if (t == 0) {
@@ -134,6 +134,8 @@ __global__ void __launch_bounds__(kBlock)
     sing |= stats[4 * (size_t)c + 2] != 0.0;
     const double max_rhs = nstats[4 * (size_t)c];
     const bool dev_ok = lim[c] == 0 || it >= prm.maxit;
+    nstats[4 * (size_t)c] = 0.0;  // the accumulators of the next iteration
+    lim[c] = 0;
     const bool now = dev_ok && ((max_rhs <= prm.rhstol && upd <= prm.dxtol) ||
                                 upd <= prm.small_update);
     const int32_t cv = conv[c] | (now ? 1 : 0);
@@ -242,19 +244,18 @@ int enqueue_iteration(xg_tran *T, int variant) {
   rc = xg_load_all(T->h, b.x, b.sta_c, b.sto_c, b.sta_n, b.sto_n, b.F, b.Q, b.FL, b.QL,
                    b.JF, b.JQ);
   if (rc) return rc;
-  rc = xg_tia_assemble(T->h, !dc, 1, 0.0, b.qn0, b.fprev, b.F, b.Q, b.FL, b.QL, b.JF,
-                       b.JQ, b.rhs, b.J);
+  // assembly with the per-component max |rhs| (xg_rhs_norms), then one
+  // pass for the per-component limiting counts (xg_component_limited) and
+  // the non-finite count; newton_ctl_kernel clears both accumulators for
+  // the next iteration
+  rc = xg::handle_assemble_norms(T->h, !dc, b.qn0, b.fprev, b.F, b.Q, b.FL, b.QL, b.JF,
+                                 b.JQ, b.rhs, b.J, b.nstats);
   if (rc) return rc;
-  rc = xg_rhs_norms(T->h, b.rhs, b.nstats);
-  if (rc) return rc;
-  rc = xg_component_limited(T->h, b.lim);
-  if (rc) return rc;
-  hipError_t e = xg::handle_count_flags(T->h, b.counts);
+  hipError_t e = xg::handle_iteration_counts(T->h, b.lim, b.counts);
   if (e != hipSuccess) return XG_E_HIP;
   hipLaunchKernelGGL(xg::newton_ctl_kernel, dim3(1), dim3(xg::kBlock), 0, T->stream,
-                     T->n_comp, (const double *)b.stats, (const double *)b.nstats,
-                     (const int32_t *)b.lim, (const int64_t *)b.counts, prm, b.conv,
-                     b.active, b.ctl);
+                     T->n_comp, (const double *)b.stats, b.nstats, b.lim,
+                     (const int64_t *)b.counts, prm, b.conv, b.active, b.ctl);
   return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
 }
 
@@ -368,6 +369,16 @@ int xg_tran_create(xg_handle *h, const xg_tran_buffers *b, double bp_tol,
   xg_set_stream(h, T->stream);
   xg_set_step_source(h, b->step);
   xg_set_active_mask(h, b->active);
+  // the per-iteration accumulators start at zero (newton_ctl_kernel clears
+  // them after every iteration)
+  if (hipMemsetAsync(b->lim, 0, sizeof(int32_t) * (size_t)T->n_comp, T->stream) !=
+          hipSuccess ||
+      hipMemsetAsync(b->nstats, 0, sizeof(double) * 4 * (size_t)T->n_comp, T->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(T->stream) != hipSuccess) {
+    xg_tran_destroy(T);
+    return XG_E_HIP;
+  }
   // every graph is captured here, so that the run itself makes no capture:
   // a capture must not overlap another thread's device allocations or
   // synchronous copies, and creation is where a caller can serialise those
