@@ -14,6 +14,9 @@ from xyce_amd import circuits
 from xyce_amd.loader import DeviceLoad
 
 
+FWD = -3   # t_div of a forward-substitution row scheduled in the factor phase
+
+
 def _comp_starts(dev):
     import ctypes
     from xyce_amd._lib import check, lib
@@ -54,28 +57,35 @@ def run_plan(dev, vals, rhs):
         bad = False
         for ph in range(3):
             for L in range(lv[ph], lv[ph + 1]):
-                new = {}
+                # values written in this level, factors and vector apart: no
+                # task of a level may read what another writes in it
+                new = {"val": {}, "vec": {}}
                 for k in range(P["lv_ptr"][L], P["lv_ptr"][L + 1]):
                     d, dv = P["t_dst"][k], P["t_div"][k]
-                    src, lim = (val, nv) if ph == 0 else (vec, m)
-                    assert 0 <= d < lim and d not in new
+                    # a forward-substitution row (t_div -3) runs among the
+                    # factor tasks and works on the vector
+                    on_vec = ph > 0 or dv == FWD
+                    src, lim, key = (vec, m, "vec") if on_vec else (val, nv, "val")
+                    assert 0 <= d < lim and d not in new[key]
                     s = src[d]
                     for o in range(P["t_op"][k], P["t_op"][k + 1]):
                         a, b = P["ops"][o]
                         assert 0 <= a < nv and 0 <= b < lim
-                        assert b not in new and (ph > 0 or a not in new)
+                        assert b not in new[key] and a not in new["val"]
                         s = s - val[a] * src[b]
                     if ph == 0 and dv >= 0:
-                        assert dv < nv and dv not in new
+                        assert dv < nv and dv not in new["val"]
                         s = s / val[dv]
                         bad |= not np.isfinite(s)
                     elif ph == 0 and dv == -2:
                         bad |= not (abs(s) > 0.0 and np.isfinite(s))
                     elif ph == 2:
                         s = s / val[dv]
-                    new[d] = s
-                for d, s in new.items():
-                    (val if ph == 0 else vec)[d] = s
+                    new[key][d] = s
+                for d, s in new["val"].items():
+                    val[d] = s
+                for d, s in new["vec"].items():
+                    vec[d] = s
         # backward error with the unfactored rows
         qpos = {int(g): i for i, g in enumerate(qcol)}
         rm = am = 0.0

@@ -231,6 +231,11 @@ struct DenseSolveArgs {
   double *stats;                  // n_comp x 4: max|rhs|, wRMS(dx), singular, n
 };
 
+// t_div of a forward-substitution row (y(i) = b(i) - sum L(i, k) y(k))
+// scheduled among the factor tasks: its value goes to the vector, its terms
+// read L from the factors and y from the vector.
+constexpr int32_t kLuFwdTask = -3;
+
 // Static-pivot sparse LU per component (task lists of lu_analyse).
 struct SparseLuArgs {
   int32_t n_comp = 0;

@@ -73,6 +73,10 @@ template <int kPhase>
 __device__ __forceinline__ void finish_task(const int4 &tk, double s, double *val,
                                             double *vec, int &bad) {
   if (kPhase == 0) {
+    if (tk.y == kLuFwdTask) {  // a forward-substitution row (y in vec)
+      vec[tk.x] = s;
+      return;
+    }
     if (tk.y >= 0) {
       s = s / val[tk.y];
       bad |= !(fabs(s) <= DBL_MAX);  // infinite or NaN
@@ -95,9 +99,10 @@ template <int kPhase>
 __device__ __forceinline__ void run_level(const Lists &ls, int L, double *val,
                                           double *vec, int &bad) {
   const int k0 = ls.lvp[L], km = ls.lvm[L], kl = ls.lvl[L], k1 = ls.lvp[L + 1];
-  const double *src = kPhase == 0 ? val : vec;
+  const double *src0 = kPhase == 0 ? val : vec;
   for (int k = k0 + (int)threadIdx.x; k < km; k += kBlock) {
     const int4 tk = ls.tasks[k];
+    const double *src = kPhase == 0 && tk.y == kLuFwdTask ? vec : src0;
     const int o0 = tk.z - ls.obase, m = tk.w - tk.z;
     double a[4], b[4];
 #pragma unroll
@@ -118,6 +123,7 @@ __device__ __forceinline__ void run_level(const Lists &ls, int L, double *val,
     const int grp = threadIdx.x >> 3, sub = threadIdx.x & 7;
     for (int k = km + grp; k < kl; k += kBlock / 8) {
       const int4 tk = ls.tasks[k];
+      const double *src = kPhase == 0 && tk.y == kLuFwdTask ? vec : src0;
       const int o0 = tk.z - ls.obase, o1 = tk.w - ls.obase;
       double p = 0.0;
       for (int o = o0 + sub; o < o1; o += 8) {
@@ -131,6 +137,7 @@ __device__ __forceinline__ void run_level(const Lists &ls, int L, double *val,
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int k = kl + wave; k < k1; k += kBlock / 64) {
     const int4 tk = ls.tasks[k];
+    const double *src = kPhase == 0 && tk.y == kLuFwdTask ? vec : src0;
     const int o0 = tk.z - ls.obase, o1 = tk.w - ls.obase;
     double p = 0.0;
     for (int o = o0 + lane; o < o1; o += 64) {

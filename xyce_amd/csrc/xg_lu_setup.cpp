@@ -429,21 +429,25 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
       tasks.push_back(std::move(t));
     }
   }
-  // ---- forward solve (L y = b, unit diagonal) and backward (U x = y)
+  // ---- forward solve (L y = b, unit diagonal) and backward (U x = y).
+  // A forward row needs its L entries and the earlier y it multiplies, not
+  // the whole factorisation: it joins the factor phase at the level after
+  // both, so the forward solve adds no levels of its own.
   std::vector<Task> fwd, bwd;
   {
     std::vector<int32_t> lf(n, -1), lb(n, -1);
     for (int32_t i = 0; i < n; ++i) {
       if (lrow[i].empty()) continue;
-      Task t{i, -1, 0, {}};
+      Task t{i, kLuFwdTask, 0, {}};
       for (size_t x = 0; x < lrow[i].size(); ++x) {
         const int32_t k = lrow[i][x];
-        t.ops.push_back((int32_t)(rp[i] + (int64_t)x));
+        const int32_t pl = (int32_t)(rp[i] + (int64_t)x);
+        t.ops.push_back(pl);
         t.ops.push_back(k);
-        t.lev = std::max(t.lev, lf[k] + 1);
+        t.lev = std::max(t.lev, std::max(lev[pl], lf[k]) + 1);
       }
       lf[i] = t.lev;
-      fwd.push_back(std::move(t));
+      tasks.push_back(std::move(t));
     }
     for (int32_t i = n - 1; i >= 0; --i) {
       const int32_t di = (int32_t)(rp[i] + (int64_t)lrow[i].size());

@@ -126,7 +126,8 @@ struct LuPlan {
                                       // short tasks (one thread each), then
                                       // mid (8 lanes each), then long (one
                                       // wavefront each)
-  std::vector<int32_t> t_dst, t_div;  // t_div: -1 none, -2 pivot (check only)
+  std::vector<int32_t> t_dst, t_div;  // t_div: -1 none, -2 pivot (check only),
+                                      // kLuFwdTask forward row
   std::vector<int32_t> t_op;          // task -> first op (n_tasks + 1)
   std::vector<int32_t> ops;           // pairs (a, b)
   int64_t fnnz = 0, factor_ops = 0;
