@@ -916,7 +916,7 @@ static int update_state_types(xg_handle *H, int32_t type, const double *x_next,
     e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
     if (e != hipSuccess) return hip_fail(e, "mos1 eval");
   }
-  // every type: the linear devices ride in the first workgroups of the
+  // every type: the linear devices ride in trailing workgroups of the
   // BSIM3 launch when there are both
   const bool fused = type == 0 && h.b3.n_slots > 0 && h.lin.n_slots > 0;
   if (type == 0 || type == XG_DEV_BSIM3) {

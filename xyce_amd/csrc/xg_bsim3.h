@@ -42,7 +42,7 @@ int32_t bsim3_eval_slots(const Plan &P);
 int32_t bsim3_count(const Plan &P);
 int bsim3_get_offsets(const Plan &P, int32_t *off);
 int bsim3_get_fields(const Plan &P, double *out, int32_t *n_fields);
-// Linear devices evaluated by the first workgroups of the BSIM3 eval launch
+// Linear devices evaluated by trailing workgroups of the BSIM3 eval launch
 // (the two device types have no data in common, so their evaluations run
 // side by side on the GPU instead of one after the other).
 struct LinTail {

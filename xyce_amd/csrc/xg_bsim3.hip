@@ -1445,14 +1445,14 @@ __device__ __forceinline__ void b3_eval(const Bsim3Dev &a,
 // evaluation uses them (other waves hide the latency).
 __global__ void __launch_bounds__(kBlock)
     bsim3_eval_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
-                      int32_t lin_blocks) {
+                      int32_t b3_blocks) {
   __shared__ double card_lds[kWavesPerBlock][B3C_N];
-  if ((int)blockIdx.x < lin_blocks) {  // a linear-device workgroup
+  if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
     lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
-                         (int)blockIdx.x * kBlock + threadIdx.x);
+                         ((int)blockIdx.x - b3_blocks) * kBlock + threadIdx.x);
     return;
   }
-  const int bid = (int)blockIdx.x - lin_blocks;
+  const int bid = (int)blockIdx.x;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int slot = bid * kBlock + tid;
@@ -1495,21 +1495,21 @@ constexpr int kRecN = B3R_N + B3I_N;
 template <bool kSplit, int kGroups>
 __global__ void __launch_bounds__((kSplit ? 128 : 64) * kGroups)
     bsim3_eval_staged_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
-                             int32_t lin_blocks) {
+                             int32_t b3_blocks) {
   constexpr int kWaves = kSplit ? 2 : 1;     // waves per eval wave
   constexpr int kThreads = 64 * kWaves * kGroups;
   __shared__ double rec[kGroups][kRecN * 64];
   __shared__ double card[kGroups][B3C_N];
-  if ((int)blockIdx.x < lin_blocks) {  // a linear-device workgroup
+  if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
     lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
-                         (int)blockIdx.x * kThreads + threadIdx.x);
+                         ((int)blockIdx.x - b3_blocks) * kThreads + threadIdx.x);
     return;
   }
   const int lane = threadIdx.x & 63;
   const int wave = kSplit ? (int)((threadIdx.x >> 6) % kWaves) : 0;
   const int grp = (int)threadIdx.x / (64 * kWaves);
   constexpr int kPer = (kRecN + kWaves - 1) / kWaves;
-  const int w = ((int)blockIdx.x - lin_blocks) * kGroups + grp;
+  const int w = (int)blockIdx.x * kGroups + grp;
   // the last workgroup's second group may have no eval wave: it loads
   // wave 0's fields (always valid) and evaluates nothing
   const bool valid = w < (a.n_slots >> 6);
@@ -1577,16 +1577,16 @@ hipError_t launch_bsim3_eval(const Bsim3Dev &d, const SolverStateDev &ss,
   const dim3 grid(b3_blocks + lin_blocks);
   if (v == XG_EVAL_STAGED_SPLIT && groups == 2) {
     hipLaunchKernelGGL((bsim3_eval_staged_kernel<true, 2>), grid, dim3(256), 0, s, d, ss,
-                       p, lt, lin_blocks);
+                       p, lt, b3_blocks);
   } else if (v == XG_EVAL_STAGED_SPLIT) {
     hipLaunchKernelGGL((bsim3_eval_staged_kernel<true, 1>), grid, dim3(128), 0, s, d, ss,
-                       p, lt, lin_blocks);
+                       p, lt, b3_blocks);
   } else if (v == XG_EVAL_STAGED) {
     hipLaunchKernelGGL((bsim3_eval_staged_kernel<false, 1>), grid, dim3(64), 0, s, d, ss,
-                       p, lt, lin_blocks);
+                       p, lt, b3_blocks);
   } else {
     hipLaunchKernelGGL(bsim3_eval_kernel, grid, dim3(kBlock), 0, s, d, ss, p, lt,
-                       lin_blocks);
+                       b3_blocks);
   }
   return hipGetLastError();
 }

@@ -5,7 +5,7 @@
 // buffer, where the ordered reduction folds them after the MOSFETs' (device
 // types load in the order MOSFET1, BSIM3, resistor, capacitor, diode,
 // source).  Device code shared by the standalone linear eval kernel and the
-// BSIM3 eval kernels, which can run the linear devices in the first
+// BSIM3 eval kernels, which can run the linear devices in trailing
 // workgroups of their own launch.
 //
 //   resistor   N_DEV_Resistor.C:1073-1174   i0 = G (v+ - v-)
