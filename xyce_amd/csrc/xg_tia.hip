@@ -29,6 +29,8 @@ __global__ void __launch_bounds__(kBlock)
     inv_dt = a.dstep[2];
     a.order = (int32_t)a.dstep[3];
   }
+  double nrm = 0.0;     // |rhs| of this unknown (NaN: 0, as fmax skips it)
+  int32_t comp = -1;    // its component (-1: none, ground, out of range)
   if (i < a.n1) {
     double r;
     if (a.transient) {
@@ -52,13 +54,34 @@ __global__ void __launch_bounds__(kBlock)
     }
     a.rhs[i] = i == a.n1 - 1 ? 0.0 : r;  // the ground slot carries nothing
     if (a.norms && i < a.n1 - 1) {
-      // non-negative doubles order as their bit patterns: an exact max
-      const int32_t c = a.node_comp[i];
-      const double m = fabs(r);
-      if (c >= 0 && m == m)
-        atomicMax(reinterpret_cast<unsigned long long *>(a.norms + 4 * (size_t)c),
-                  (unsigned long long)__double_as_longlong(m));
+      comp = a.node_comp[i];
+      nrm = r == r ? fabs(r) : 0.0;
     }
+  }
+  if (a.norms) {
+    // per-component max |rhs|: non-negative doubles order as their bit
+    // patterns, so an integer atomic max is exact.  A wave whose unknowns
+    // all lie in one component (components are contiguous runs of nodes)
+    // folds its max first and makes one atomic.
+    int32_t lo = comp >= 0 ? comp : 0x7fffffff, hi = comp;
+    double wm = nrm;
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = min(lo, __shfl_xor(lo, o, 64));
+      hi = max(hi, __shfl_xor(hi, o, 64));
+      wm = fmax(wm, __shfl_xor(wm, o, 64));
+    }
+    unsigned long long *dst = nullptr;
+    double v = 0.0;
+    if (lo == hi) {  // every unknown of the wave in one component
+      if ((threadIdx.x & 63) == 0) {
+        dst = reinterpret_cast<unsigned long long *>(a.norms + 4 * (size_t)lo);
+        v = wm;
+      }
+    } else if (comp >= 0) {
+      dst = reinterpret_cast<unsigned long long *>(a.norms + 4 * (size_t)comp);
+      v = nrm;
+    }
+    if (dst && v > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(v));
   }
   if (i < a.nnz) {
     if (a.transient) {
