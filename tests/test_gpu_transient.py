@@ -151,13 +151,14 @@ def test_dense_solve_against_numpy():
 
 def test_dense_solve_with_row_interchanges():
     """Random values with no dominant diagonal (partial pivoting swaps rows
-    at most steps) on components in all three size classes of the dense
+    at most steps) on components in all four size classes of the dense
     LU: factored in LDS (<= 128 unknowns), in global memory with 16-column
-    panels (<= 1024) and with 8-column panels (<= 2048).  Each solution has
+    panels (<= 1024), 8-column panels (<= 1934) and 4-column panels
+    (<= 2048, the largest with a dense form).  Each solution has
     a normwise backward error of a few ulps, and one column of zeros makes
     its component report a singular matrix."""
     from xyce_amd.loader import DeviceLoad
-    nl = _mos1_rings([40, 500, 1200, 90])
+    nl = _mos1_rings([40, 500, 1200, 90, 1990])
     dev = DeviceLoad(nl, plan_only=True)
     rp, ci = dev.pattern()
     n = nl.n_nodes
@@ -167,14 +168,15 @@ def test_dense_solve_with_row_interchanges():
     rhs[n] = 0.0
     comp = _components(dev)
     sizes = np.bincount(comp)
-    order = np.argsort(sizes)          # 40-, 90-, 500- and 1200-stage rings
-    assert sizes[order[1]] <= 128 < sizes[order[2]] <= 1024 < sizes[order[3]] <= 2048
+    order = np.argsort(sizes)          # 40-, 90-, 500-, 1200- and 1990-stage rings
+    assert sizes[order[1]] <= 128 < sizes[order[2]] <= 1024 < sizes[order[3]] <= 1934
+    assert 1934 < sizes[order[4]] <= 2048
     # the 90-stage ring's component loses one column: singular
     sing = order[1]
     vals[np.isin(ci, np.flatnonzero(comp == sing)[5:6])] = 0.0
     x, st, comp2 = _dense_solve(nl, vals, rhs)
     assert np.array_equal(comp, comp2)
-    for c in range(4):
+    for c in range(5):
         idx = np.flatnonzero(comp == c)
         if c == sing:
             assert st[c, 2] == 1.0

@@ -19,6 +19,11 @@ namespace xg {
 namespace {
 
 constexpr int kBlock = 256;
+// The dense LU's workgroup (one per component and CU for its LDS): four
+// waves, each with the whole register file of its SIMD (the unrolled panel
+// solve keeps a column's sixteen U values and the panel rows in registers;
+// more waves per SIMD would spill them).
+constexpr int kDenseBlock = 256;
 constexpr int kLdsN = 128;  // 128 x 128 doubles = 128 KB of the 160 KB LDS
 
 __global__ void __launch_bounds__(kBlock)
@@ -135,7 +140,7 @@ __device__ __forceinline__ void block_argmax(double v, int idx, double *sv, int 
   __syncthreads();
   best = sv[0];
   bidx = si[0];
-  for (int k = 1; k < kBlock / 64; ++k)
+  for (int k = 1; k < kDenseBlock / 64; ++k)
     if (sv[k] > best || (sv[k] == best && si[k] < bidx)) {
       best = sv[k];
       bidx = si[k];
@@ -147,7 +152,7 @@ __device__ __forceinline__ double block_sum(double v, double *sv) {
   if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = sv[0];
-  for (int k = 1; k < kBlock / 64; ++k) r = r + sv[k];
+  for (int k = 1; k < kDenseBlock / 64; ++k) r = r + sv[k];
   __syncthreads();
   return r;
 }
@@ -157,14 +162,15 @@ __device__ __forceinline__ double block_max(double v, double *sv) {
   if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = sv[0];
-  for (int k = 1; k < kBlock / 64; ++k) r = fmax(r, sv[k]);
+  for (int k = 1; k < kDenseBlock / 64; ++k) r = fmax(r, sv[k]);
   __syncthreads();
   return r;
 }
 
 // Blocked right-looking LU with partial pivoting of the n x n row-major A
 // (LDS or global), row swaps applied to b as they happen.  Panels of NB
-// columns factor in the LDS buffer P (rows kb..n-1); the rows below the
+// columns factor in the LDS buffer P (rows kb..n-1, stride NB + 1); the
+// rows below the
 // panel then take the panel's NB rank-1 updates in one pass per element,
 // in the order of the unblocked elimination (each element's updates in
 // ascending pivot order, so the factors are those of the column-by-column
@@ -173,19 +179,22 @@ template <int NB>
 __device__ __forceinline__ bool dense_factor(double *A, int n, double *b, double *P, double *sv,
                              int *si, int *spiv) {
   const int t = threadIdx.x;
+  // panel row stride NB + 1 doubles: the lanes of a column access (pivot
+  // search, scaling) fall in different LDS banks
+  constexpr int PS = NB + 1;
   for (int kb = 0; kb < n; kb += NB) {
     const int nb = min(NB, n - kb);
     const int m = n - kb;
-    for (int e = t; e < m * NB; e += kBlock) {
+    for (int e = t; e < m * NB; e += kDenseBlock) {
       const int r = e / NB, c = e % NB;
-      P[e] = c < nb ? A[(int64_t)(kb + r) * n + kb + c] : 0.0;
+      P[r * PS + c] = c < nb ? A[(int64_t)(kb + r) * n + kb + c] : 0.0;
     }
     __syncthreads();
     for (int c = 0; c < nb; ++c) {
       double v = -1.0;
       int idx = 0x7fffffff;
-      for (int r = c + t; r < m; r += kBlock) {
-        const double a = fabs(P[r * NB + c]);
+      for (int r = c + t; r < m; r += kDenseBlock) {
+        const double a = fabs(P[r * PS + c]);
         if (a > v) {
           v = a;
           idx = r;
@@ -197,10 +206,10 @@ __device__ __forceinline__ bool dense_factor(double *A, int n, double *b, double
       if (!(best > 0.0)) return false;  // the same on every thread
       if (t == 0) spiv[c] = kb + br;
       if (br != c) {
-        for (int cc = t; cc < NB; cc += kBlock) {
-          const double tmp = P[c * NB + cc];
-          P[c * NB + cc] = P[br * NB + cc];
-          P[br * NB + cc] = tmp;
+        for (int cc = t; cc < NB; cc += kDenseBlock) {
+          const double tmp = P[c * PS + cc];
+          P[c * PS + cc] = P[br * PS + cc];
+          P[br * PS + cc] = tmp;
         }
         if (t == 0) {
           const double tmp = b[kb + c];
@@ -209,21 +218,21 @@ __device__ __forceinline__ bool dense_factor(double *A, int n, double *b, double
         }
       }
       __syncthreads();
-      const double piv = P[c * NB + c];
-      for (int r = c + 1 + t; r < m; r += kBlock) {
-        const double l = P[r * NB + c] / piv;
-        P[r * NB + c] = l;
-        for (int cc = c + 1; cc < nb; ++cc) P[r * NB + cc] -= l * P[c * NB + cc];
+      const double piv = P[c * PS + c];
+      for (int r = c + 1 + t; r < m; r += kDenseBlock) {
+        const double l = P[r * PS + c] / piv;
+        P[r * PS + c] = l;
+        for (int cc = c + 1; cc < nb; ++cc) P[r * PS + cc] -= l * P[c * PS + cc];
       }
       __syncthreads();
     }
     // the factored panel back into A; the panel's row swaps on the other
     // columns (left of the panel too, so L ends in the final row order)
-    for (int e = t; e < m * NB; e += kBlock) {
+    for (int e = t; e < m * NB; e += kDenseBlock) {
       const int r = e / NB, c = e % NB;
-      if (c < nb) A[(int64_t)(kb + r) * n + kb + c] = P[e];
+      if (c < nb) A[(int64_t)(kb + r) * n + kb + c] = P[r * PS + c];
     }
-    for (int j = t; j < n; j += kBlock) {
+    for (int j = t; j < n; j += kDenseBlock) {
       if (j >= kb && j < kb + nb) continue;
       for (int c = 0; c < nb; ++c) {
         const int pr = spiv[c];
@@ -246,21 +255,34 @@ __device__ __forceinline__ bool dense_factor(double *A, int n, double *b, double
 #pragma unroll
       for (int c = 0; c < NB; ++c)
 #pragma unroll
-        for (int i = c + 1; i < NB; ++i) u[i] -= P[i * NB + c] * u[c];
+        for (int i = c + 1; i < NB; ++i) u[i] -= P[i * PS + c] * u[c];
     };
-    // rows j0+g, j0+g+G, ... of column j take the panel's updates
+    // rows j0+g, j0+g+G, ... of column j take the panel's updates, four
+    // rows' loads issued before their stores
     auto sweep = [&](int j, int g, int G, const double *u) {
-      for (int i = j0 + g; i < n; i += G) {
-        double a = A[(int64_t)i * n + j];
-        const double *l = P + (i - kb) * NB;
+      for (int i = j0 + g; i < n; i += 4 * G) {
+        double x[4];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) a -= l[c] * u[c];
-        A[(int64_t)i * n + j] = a;
+        for (int q = 0; q < 4; ++q) {
+          const int ii = i + q * G;
+          x[q] = ii < n ? A[(int64_t)ii * n + j] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ii = i + q * G;
+          if (ii < n) {
+            const double *l = P + (ii - kb) * PS;
+            double v = x[q];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) v -= l[c] * u[c];
+            A[(int64_t)ii * n + j] = v;
+          }
+        }
       }
     };
-    if (ncols >= kBlock) {
+    if (ncols >= kDenseBlock) {
       // a column per thread
-      for (int j = j0 + t; j < n; j += kBlock) {
+      for (int j = j0 + t; j < n; j += kDenseBlock) {
         double u[NB];
         solve_u(j, u);
 #pragma unroll
@@ -270,7 +292,7 @@ __device__ __forceinline__ bool dense_factor(double *A, int n, double *b, double
     } else if (ncols > 0) {
       // G threads per column, each sweeping every G-th row; all of them
       // read the column's U12 rows before the first one writes them
-      const int G = min(kBlock / ncols, 16);
+      const int G = min(kDenseBlock / ncols, 16);
       const bool mine = t < ncols * G;
       const int j = j0 + t % ncols, g = t / ncols;
       double u[NB];
@@ -291,15 +313,16 @@ __device__ __forceinline__ bool dense_factor(double *A, int n, double *b, double
 // L y = b (unit lower) then U x = y on the factored A, b in LDS, blocks of
 // 64 rows: the block's products with the solved part are wave dot
 // products, the 64 x 64 diagonal block is solved by one wave.  D is an
-// LDS buffer of 64 x 64 doubles for the diagonal block (unused when A is
+// LDS buffer of 64 x 65 doubles for the diagonal block (rows padded by one
+// so a wave's column reads fall in different banks; unused when A is
 // itself in LDS).
 template <bool kLds>
 __device__ __forceinline__ void dense_solve_tri(const double *A, int n, double *b, double *D,
                                 double *ss) {
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  constexpr int kWaves = kBlock / 64;
+  constexpr int kWaves = kDenseBlock / 64;
   auto diag = [&](int r0, int l, int i) -> double {
-    return kLds ? A[(int64_t)(r0 + l) * n + r0 + i] : D[l * 64 + i];
+    return kLds ? A[(int64_t)(r0 + l) * n + r0 + i] : D[l * 65 + i];
   };
   for (int r0 = 0; r0 < n; r0 += 64) {
     const int R = min(64, n - r0);
@@ -311,9 +334,9 @@ __device__ __forceinline__ void dense_solve_tri(const double *A, int n, double *
       if (lane == 0) ss[rr] = s;
     }
     if (!kLds)
-      for (int e = t; e < R * 64; e += kBlock) {
+      for (int e = t; e < R * 64; e += kDenseBlock) {
         const int l = e >> 6, i = e & 63;
-        if (i < R) D[e] = A[(int64_t)(r0 + l) * n + r0 + i];
+        if (i < R) D[l * 65 + i] = A[(int64_t)(r0 + l) * n + r0 + i];
       }
     __syncthreads();
     if (w == 0) {
@@ -336,9 +359,9 @@ __device__ __forceinline__ void dense_solve_tri(const double *A, int n, double *
       if (lane == 0) ss[rr] = s;
     }
     if (!kLds)
-      for (int e = t; e < R * 64; e += kBlock) {
+      for (int e = t; e < R * 64; e += kDenseBlock) {
         const int l = e >> 6, i = e & 63;
-        if (i < R) D[e] = A[(int64_t)(r0 + l) * n + r0 + i];
+        if (i < R) D[l * 65 + i] = A[(int64_t)(r0 + l) * n + r0 + i];
       }
     __syncthreads();
     if (w == 0) {
@@ -354,20 +377,20 @@ __device__ __forceinline__ void dense_solve_tri(const double *A, int n, double *
   }
 }
 
-// Panel buffer of the global-memory form: n x 16 doubles up to n = 1024,
-// n x 8 beyond (kDenseMax = 2048).
-constexpr int kPanelDoubles = 16384;
+// Panel buffer of the global-memory form: n x 17 doubles (16-column panels)
+// up to n = 1024, n x 9 up to 1934, n x 5 beyond (kDenseMax = 2048).
+constexpr int kPanelDoubles = 17408;
 
 // One component per workgroup.  A is n x n row-major (LDS or workspace).
 template <bool kLds>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kDenseBlock)
     dense_solve_kernel(DenseSolveArgs a) {
   __shared__ double lds_a[kLds ? kLdsN * kLdsN : 1];
-  __shared__ double panel[kLds ? kLdsN * 16 : kPanelDoubles];
+  __shared__ double panel[kLds ? kLdsN * 17 : kPanelDoubles];
   __shared__ double sb[kLds ? kLdsN : kDenseMax];
   __shared__ double ss[64];
-  __shared__ double sv[kBlock / 64];
-  __shared__ int si[kBlock / 64];
+  __shared__ double sv[kDenseBlock / 64];
+  __shared__ int si[kDenseBlock / 64];
   __shared__ int spiv[16];
   const int c = blockIdx.x;
   const int t = threadIdx.x;
@@ -398,10 +421,10 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t *nodes = a.nodes + beg;
   // gather the component's rows of J (and its RHS) into dense form
   if (!a.norm_only)
-    for (int64_t e = t; e < (int64_t)n * n; e += kBlock) A[e] = 0.0;
+    for (int64_t e = t; e < (int64_t)n * n; e += kDenseBlock) A[e] = 0.0;
   __syncthreads();
   double rmax = 0.0;
-  for (int r = t; r < n; r += kBlock) {
+  for (int r = t; r < n; r += kDenseBlock) {
     const int g = nodes[r];
     if (!a.norm_only) {
       for (int k = a.rowptr[g]; k < a.rowptr[g + 1]; ++k) {
@@ -417,9 +440,10 @@ __global__ void __launch_bounds__(kBlock)
     if (t == 0) stats[0] = max_rhs;
     return;
   }
-  const bool ok = (kLds || n <= kPanelDoubles / 16)
+  const bool ok = (kLds || n * 17 <= kPanelDoubles)
                       ? dense_factor<16>(A, n, b, panel, sv, si, spiv)
-                      : dense_factor<8>(A, n, b, panel, sv, si, spiv);
+                      : n * 9 <= kPanelDoubles ? dense_factor<8>(A, n, b, panel, sv, si, spiv)
+                                               : dense_factor<4>(A, n, b, panel, sv, si, spiv);
   if (!ok) {
     if (t == 0) {
       stats[0] = max_rhs;
@@ -433,7 +457,7 @@ __global__ void __launch_bounds__(kBlock)
   // Newton update x += dx and the weighted RMS norm of dx
   // (N_NLS_DampedNewton.C:311-335 weights, wRMSNorm)
   double acc = 0.0;
-  for (int r = t; r < n; r += kBlock) {
+  for (int r = t; r < n; r += kDenseBlock) {
     const int g = nodes[r];
     const double dx = b[r];
     const double w = a.reltol * fmax(fabs(a.wa[g]), fabs(a.wb[g])) + a.abstol;
@@ -462,11 +486,11 @@ hipError_t launch_tia_assemble(const TiaAssembleArgs &a, hipStream_t s) {
 
 hipError_t launch_dense_solve(const DenseSolveArgs &a, hipStream_t s) {
   if (a.n_comp == 0) return hipSuccess;
-  hipLaunchKernelGGL(dense_solve_kernel<true>, dim3(a.n_comp), dim3(kBlock), 0,
+  hipLaunchKernelGGL(dense_solve_kernel<true>, dim3(a.n_comp), dim3(kDenseBlock), 0,
                      s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !a.any_large) return e;
-  hipLaunchKernelGGL(dense_solve_kernel<false>, dim3(a.n_comp), dim3(kBlock), 0,
+  hipLaunchKernelGGL(dense_solve_kernel<false>, dim3(a.n_comp), dim3(kDenseBlock), 0,
                      s, a);
   return hipGetLastError();
 }
