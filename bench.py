@@ -98,8 +98,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests in the serving leg (Poisson at --serve-rate)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-plain", type=int, default=40,
-                    help="requests of the same stream served one circuit per launch")
+    ap.add_argument("--serve-plain", type=int, default=200,
+                    help="first requests of the same stream served one circuit per "
+                         "launch (default: all of them)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-grid", action="store_true",
                     help="skip the power-grid (100k inverter) load timing")
@@ -271,14 +272,14 @@ def serving_leg(args):
     BSIM3 rings of 11-101 stages (0.1 ns), 30% random BSIM3 logic of
     125-400 gates (0.05 ns), 10% RC ladders of 10,000 nodes with 1,000
     diodes (0.05 ns), all at 1 ps steps.  Batched: continuous batching
-    (micro-batches of one kind every 5 ms, up to 32 requests, 3 batches in
-    flight).  Plain: the first requests of the same stream, one circuit per
-    launch, one at a time.  Latency = completion - arrival, including the
-    library setup and upload of each batch."""
+    (micro-batches of one kind every 5 ms, up to 64 requests, 3 batches in
+    flight).  Plain: the same stream (its first --serve-plain requests), one
+    circuit per launch, one at a time.  Latency = completion - arrival,
+    including the library setup and upload of each batch."""
     from xyce_amd.serving import poisson_requests, serve, summary
     progress("serving: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
     reqs = poisson_requests(rate=args.serve_rate, n=args.serve_requests, seed=4)
-    b = summary(serve(reqs, max_batch=32, workers=3, window=0.005))
+    b = summary(serve(reqs, max_batch=64, workers=3, window=0.005))
     progress("serving: batched %.1f solves/s, p99 %.3f s" % (b["solves_per_s"],
                                                             b["latency_p99_s"]))
     res = {"workload": "Poisson %.0f req/s, %d requests: 60%% BSIM3 rings 11-101 stages, "
@@ -286,7 +287,8 @@ def serving_leg(args):
                        "1k diodes (seed 4)" % (args.serve_rate, args.serve_requests),
            "batched": b}
     if args.serve_plain > 0:
-        plain = poisson_requests(rate=args.serve_rate, n=args.serve_plain, seed=4)
+        plain = poisson_requests(rate=args.serve_rate, n=args.serve_requests,
+                                 seed=4)[:args.serve_plain]
         p = summary(serve(plain, max_batch=1, workers=1, window=0.001))
         progress("serving: plain %.1f solves/s, p99 %.3f s" % (p["solves_per_s"],
                                                               p["latency_p99_s"]))
