@@ -746,11 +746,14 @@ static bool upload_handle(xg_handle *H) {
       ok = false;
   }
   auto up_prog = [&](const xg::ReduceProgram &rp, xg::ReduceDev &rd) {
+    rd.n_short4 = (int32_t)rp.sdst4.size();
     rd.n_short = (int32_t)rp.sdst.size();
     rd.n_chunk = (int32_t)rp.cdst.size();
     rd.n_multi = (int32_t)rp.mtgt.size();
     rd.n_zero = (int32_t)rp.zidx.size();
     rd.pbase = rp.pbase;
+    chk(rd.ssrc4 = h.upload(rp.ssrc4));
+    chk(rd.sdst4 = h.upload(rp.sdst4));
     chk(rd.ssrc = h.upload(rp.ssrc));
     chk(rd.sdst = h.upload(rp.sdst));
     chk(rd.zidx = h.upload(rp.zidx));

@@ -60,8 +60,9 @@ enum Mos1Slot : int {
 // is to turn -0.0 into +0.0) and kPad means "+ -0.0", the exact identity,
 // used to pad fixed-width records.
 //
-//  * short entries (at most kShort sources): record i is the fixed-width
-//    list ssrc[i * kShort .. +kShort) for entry sdst[i];
+//  * short entries: at most four sources, record i is the fixed-width list
+//    ssrc4[i * 4 .. +4) for entry sdst4[i]; five to kShort sources, the list
+//    ssrc[i * kShort .. +kShort) for entry sdst[i];
 //  * the other entries' sources are stored contiguously in src and cut into
 //    chunks of at most kChunk.  Chunk c covers src[cptr[c] .. cptr[c+1]); if
 //    it is the whole segment of entry e then cdst[c] = e, else -1 and its
@@ -74,6 +75,7 @@ constexpr int32_t kChunk = 32;
 constexpr int32_t kPad = -2;
 
 struct ReduceProgram {
+  std::vector<int32_t> ssrc4, sdst4;
   std::vector<int32_t> ssrc, sdst;
   std::vector<int32_t> src;
   std::vector<int32_t> cptr, cdst;
@@ -84,11 +86,14 @@ struct ReduceProgram {
 
 // Device-side view of a reduction program.
 struct ReduceDev {
+  int32_t n_short4 = 0;
   int32_t n_short = 0;
   int32_t n_chunk = 0;
   int32_t n_multi = 0;
   int32_t n_zero = 0;
   int32_t pbase = 0;
+  int32_t *ssrc4 = nullptr;
+  int32_t *sdst4 = nullptr;
   int32_t *ssrc = nullptr;
   int32_t *sdst = nullptr;
   int32_t *src = nullptr;

@@ -5,8 +5,9 @@
 // Every output entry k owns a segment of sources (slot indices) listed in
 // the reference's accumulation order (device type, instance, statement).
 //
-//  * A segment of at most kShort sources is a fixed-width record; one thread
-//    loads the record (two 16-byte loads), gathers its values and folds
+//  * A segment of at most kShort sources is a fixed-width record of four
+//    (one 16-byte load) or eight sources (two); one thread loads the
+//    record, gathers its values and folds
 //      acc = out[k]; acc = acc + s0; acc = acc + s1; ...; out[k] = acc
 //    which is exactly the sequence of "*ptr += v" / "vec[li] += v"
 //    statements the reference executes (N_DEV_MOSFET1.C:4441-4722,
@@ -37,6 +38,7 @@ struct ReduceBatch {
   int overwrite;
   // work-item prefixes over the programs: short records, chunks, zero
   // entries, multi-chunk segments
+  int32_t s4start[kMaxProgs + 1];
   int32_t sstart[kMaxProgs + 1];
   int32_t cstart[kMaxProgs + 1];
   int32_t zstart[kMaxProgs + 1];
@@ -84,7 +86,26 @@ __device__ __forceinline__ double fold(double acc, const int32_t *__restrict__ s
 __global__ void __launch_bounds__(kBlock)
     fold_kernel(ReduceBatch b, const double *__restrict__ slots) {
   int w = blockIdx.x * kBlock + threadIdx.x;
+  const int n_short4 = b.s4start[b.n];
   const int n_short = b.sstart[b.n], n_chunk = b.cstart[b.n];
+  if (w < n_short4) {
+    const int k = find_prog(b.s4start, b.n, w);
+    const ReduceDev &p = b.prog[k];
+    const int i = w - b.s4start[k];
+    const int4 r0 = reinterpret_cast<const int4 *>(p.ssrc4)[i];
+    const int32_t dst = p.sdst4[i];
+    double *out = b.out[k];
+    const double v0 = source_value(r0.x, slots), v1 = source_value(r0.y, slots),
+                 v2 = source_value(r0.z, slots), v3 = source_value(r0.w, slots);
+    double acc = b.overwrite ? 0.0 : out[dst];
+    acc = acc + v0;
+    acc = acc + v1;
+    acc = acc + v2;
+    acc = acc + v3;
+    out[dst] = acc;
+    return;
+  }
+  w -= n_short4;
   if (w < n_short) {
     const int k = find_prog(b.sstart, b.n, w);
     const ReduceDev &p = b.prog[k];
@@ -256,21 +277,23 @@ hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
   for (int k = 0; k < n_out; ++k) {
     b.prog[k] = progs[k];
     b.out[k] = outs[k];
-    const int32_t items = progs[k].n_short + progs[k].n_chunk +
+    const int32_t items = progs[k].n_short4 + progs[k].n_short + progs[k].n_chunk +
                           (overwrite ? progs[k].n_zero : 0);
     if (!outs[k] && items) return hipErrorInvalidValue;
+    b.s4start[k + 1] = b.s4start[k] + progs[k].n_short4;
     b.sstart[k + 1] = b.sstart[k] + progs[k].n_short;
     b.cstart[k + 1] = b.cstart[k] + progs[k].n_chunk;
     b.zstart[k + 1] = b.zstart[k] + progs[k].n_zero;
     b.mstart[k + 1] = b.mstart[k] + progs[k].n_multi;
   }
   for (int k = n_out; k < kMaxProgs; ++k) {
+    b.s4start[k + 1] = b.s4start[n_out];
     b.sstart[k + 1] = b.sstart[n_out];
     b.cstart[k + 1] = b.cstart[n_out];
     b.zstart[k + 1] = b.zstart[n_out];
     b.mstart[k + 1] = b.mstart[n_out];
   }
-  const int64_t items = (int64_t)b.sstart[n_out] + b.cstart[n_out] +
+  const int64_t items = (int64_t)b.s4start[n_out] + b.sstart[n_out] + b.cstart[n_out] +
                         (overwrite ? b.zstart[n_out] : 0);
   const int n_multi = b.mstart[n_out];
   if (items == 0) return hipSuccess;

@@ -484,6 +484,8 @@ static void finish_program(const std::vector<Contrib> &cs, int32_t n_out,
   std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
   std::vector<int32_t> srcs(cs.size());
   for (auto &c : cs) srcs[pos[c.out]++] = c.src;
+  prog.ssrc4.clear();
+  prog.sdst4.clear();
   prog.ssrc.clear();
   prog.sdst.clear();
   prog.src.clear();
@@ -496,6 +498,12 @@ static void finish_program(const std::vector<Contrib> &cs, int32_t n_out,
     const int32_t len = cnt[k + 1] - cnt[k];
     if (len == 0 || skip[k]) {
       prog.zidx.push_back(k);
+      continue;
+    }
+    if (len <= 4) {
+      for (int32_t q = 0; q < 4; ++q)
+        prog.ssrc4.push_back(q < len ? srcs[cnt[k] + q] : kPad);
+      prog.sdst4.push_back(k);
       continue;
     }
     if (len <= kShort) {
