@@ -30,6 +30,9 @@
 // the same factorisation alone and inside a batch.
 #include <algorithm>
 #include <atomic>
+#include <functional>
+#include <iterator>
+#include <queue>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -215,6 +218,89 @@ bool match_rows(const Graph &rows, int32_t n, const std::vector<int32_t> &pair_r
   return true;
 }
 
+// Minimum-degree order of the symmetric graph g (sorted, duplicate-free
+// neighbour lists): repeatedly eliminate a vertex of least current degree
+// (smallest index on ties), joining its neighbours into a clique of the
+// explicit elimination graph.  False when that graph would exceed `budget`
+// (directed) edges or the list merges `work` element steps (bounding the
+// setup time where the cliques grow large).
+bool min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t budget,
+                      int64_t work) {
+  const int32_t n = (int32_t)g.size();
+  Graph e(g);
+  int64_t edges = 0;
+  for (auto &a : e) edges += (int64_t)a.size();
+  std::vector<char> gone(n, 0);
+  using Item = std::pair<int32_t, int32_t>;  // (degree, vertex)
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+  for (int32_t v = 0; v < n; ++v) pq.push({(int32_t)e[v].size(), v});
+  order.clear();
+  order.reserve(n);
+  std::vector<int32_t> merged;
+  while (!pq.empty()) {
+    const Item it = pq.top();
+    pq.pop();
+    const int32_t v = it.second;
+    if (gone[v] || it.first != (int32_t)e[v].size()) continue;  // stale entry
+    gone[v] = 1;
+    order.push_back(v);
+    std::vector<int32_t> nb;
+    nb.swap(e[v]);
+    edges -= (int64_t)nb.size();
+    for (int32_t a : nb) {
+      // a's neighbours become (theirs and v's) minus a and v
+      merged.clear();
+      std::set_union(e[a].begin(), e[a].end(), nb.begin(), nb.end(),
+                     std::back_inserter(merged));
+      merged.erase(std::remove_if(merged.begin(), merged.end(),
+                                  [&](int32_t x) { return x == a || x == v; }),
+                   merged.end());
+      edges += (int64_t)merged.size() - (int64_t)e[a].size();
+      work -= (int64_t)merged.size();
+      e[a].swap(merged);
+      if (edges > budget || work < 0) return false;
+      pq.push({(int32_t)e[a].size(), a});
+    }
+  }
+  return true;
+}
+
+// Strictly-lower fill and elimination depth (the longest chain of pivots
+// that depend on each other through L) of an order, by the row-subtree
+// walk of the factorisation below; false when the fill exceeds the budget.
+bool order_cost(const Graph &sym, const std::vector<int32_t> &order, int64_t &fill,
+                int32_t &depth) {
+  const int32_t n = (int32_t)sym.size();
+  std::vector<int32_t> pinv(n), parent(n, -1), mark(n, -1), lev(n, 0);
+  for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
+  fill = 0;
+  depth = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    mark[i] = i;
+    for (int32_t u : sym[order[i]]) {
+      int32_t j = pinv[u];
+      if (j >= i) continue;
+      while (mark[j] != i) {
+        ++fill;
+        lev[i] = std::max(lev[i], lev[j] + 1);
+        mark[j] = i;
+        if (parent[j] < 0) {
+          parent[j] = i;
+          break;
+        }
+        j = parent[j];
+      }
+    }
+    depth = std::max(depth, lev[i]);
+    if (fill > kLuMaxFill) return false;
+  }
+  return true;
+}
+
+// Components up to this size also get a minimum-degree order (kept when it
+// factors in fewer dependent steps than the nested dissection).
+constexpr int32_t kMinDegreeMaxN = 100000;
+
 // One component's static factorisation, with task indices local to the
 // component (lu_analyse appends the components in order).  The component's
 // entries of prow / qcol / qpos / lu_pos (disjoint from every other
@@ -291,6 +377,19 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
       return sym[a].size() < sym[b].size();
     });
     for (int32_t v : hubs) order.push_back(v);
+  }
+  // minimum degree instead where its elimination chain is shorter (random
+  // logic: a quarter of the fill and a third of the depth; rings and
+  // ladders keep the dissection's logarithmic depth)
+  if (n <= kMinDegreeMaxN) {
+    std::vector<int32_t> md;
+    if (min_degree_order(sym, md, 2 * kLuMaxFill, kLuMaxOps)) {
+      int64_t f_nd = 0, f_md = 0;
+      int32_t d_nd = 0, d_md = 0;
+      const bool ok_nd = order_cost(sym, order, f_nd, d_nd);
+      const bool ok_md = order_cost(sym, md, f_md, d_md);
+      if (ok_md && (!ok_nd || d_md < d_nd || (d_md == d_nd && f_md < f_nd))) order.swap(md);
+    }
   }
   std::vector<int32_t> pinv(n);
   for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
