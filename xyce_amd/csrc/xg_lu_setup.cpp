@@ -221,19 +221,27 @@ bool match_rows(const Graph &rows, int32_t n, const std::vector<int32_t> &pair_r
 // Minimum-degree order of the symmetric graph g (sorted, duplicate-free
 // neighbour lists): repeatedly eliminate a vertex of least current degree
 // (smallest index on ties), joining its neighbours into a clique of the
-// explicit elimination graph.  False when that graph would exceed `budget`
-// (directed) edges or the list merges `work` element steps (bounding the
-// setup time where the cliques grow large).
-bool min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t budget,
-                      int64_t work) {
+// explicit elimination graph.  The vertices in `last` (supply rails, whose
+// rows are dense anyway) stay out of the elimination graph and follow in
+// the given order.  False when that graph would exceed `budget` (directed)
+// edges or the list merges `work` element steps (bounding the setup time).
+bool min_degree_order(const Graph &g, const std::vector<int32_t> &last,
+                      std::vector<int32_t> &order, int64_t budget, int64_t work) {
   const int32_t n = (int32_t)g.size();
-  Graph e(g);
-  int64_t edges = 0;
-  for (auto &a : e) edges += (int64_t)a.size();
   std::vector<char> gone(n, 0);
+  for (int32_t v : last) gone[v] = 1;
+  Graph e(n);
+  int64_t edges = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    if (gone[v]) continue;
+    for (int32_t u : g[v])
+      if (!gone[u]) e[v].push_back(u);
+    edges += (int64_t)e[v].size();
+  }
   using Item = std::pair<int32_t, int32_t>;  // (degree, vertex)
   std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
-  for (int32_t v = 0; v < n; ++v) pq.push({(int32_t)e[v].size(), v});
+  for (int32_t v = 0; v < n; ++v)
+    if (!gone[v]) pq.push({(int32_t)e[v].size(), v});
   order.clear();
   order.reserve(n);
   std::vector<int32_t> merged;
@@ -262,6 +270,7 @@ bool min_degree_order(const Graph &g, std::vector<int32_t> &order, int64_t budge
       pq.push({(int32_t)e[a].size(), a});
     }
   }
+  for (int32_t v : last) order.push_back(v);
   return true;
 }
 
@@ -364,11 +373,11 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
   // ordering: nested dissection; rows of very high degree last
   const double avg = n ? (double)edges / n : 0.0;
   const size_t hub = (size_t)std::max(32.0, std::max(8.0 * avg, 4.0 * std::sqrt((double)n)));
-  std::vector<int32_t> order;
+  std::vector<int32_t> order, hubs;
   order.reserve(n);
   {
     Dissector d(sym, order);
-    std::vector<int32_t> rest, hubs;
+    std::vector<int32_t> rest;
     for (int32_t v = 0; v < n; ++v) (sym[v].size() > hub ? hubs : rest).push_back(v);
     for (int32_t v : rest) d.in[v] = 0;
     d.next_tag = 1;
@@ -383,7 +392,7 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
   // ladders keep the dissection's logarithmic depth)
   if (n <= kMinDegreeMaxN) {
     std::vector<int32_t> md;
-    if (min_degree_order(sym, md, 2 * kLuMaxFill, kLuMaxOps)) {
+    if (min_degree_order(sym, hubs, md, 2 * kLuMaxFill, kLuMaxOps)) {
       int64_t f_nd = 0, f_md = 0;
       int32_t d_nd = 0, d_md = 0;
       const bool ok_nd = order_cost(sym, order, f_nd, d_nd);
