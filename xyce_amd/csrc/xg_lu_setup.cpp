@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -672,7 +673,11 @@ int lu_analyse(Plan &plan, std::string &err) {
       std::max(1, std::min<int32_t>({nc, 16, (int32_t)std::thread::hardware_concurrency()}));
   {
     std::vector<std::thread> pool;
-    for (int32_t t = 1; t < n_threads; ++t) pool.emplace_back(work);
+    try {
+      for (int32_t t = 1; t < n_threads; ++t) pool.emplace_back(work);
+    } catch (const std::system_error &) {
+      // no more threads: the ones started and this one share the work
+    }
     work();
     for (auto &th : pool) th.join();
   }
