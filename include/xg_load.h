@@ -140,6 +140,14 @@ int xg_add_bsim3_instance(xg_circuit *c, int32_t model_id, int32_t n,
 int xg_add_resistor(xg_circuit *c, int32_t pos, int32_t neg, double r);
 int xg_add_capacitor(xg_circuit *c, int32_t pos, int32_t neg, double cap,
                      int32_t sta);
+/* n resistors / capacitors in one call, as n calls of the above in array
+   order (no counterpart in the reference: one call instead of one per
+   device for large interconnect).  Stops at the first invalid device; the
+   ones before it stay added, and the error message names its index. */
+int xg_add_resistors(xg_circuit *c, int32_t n, const int32_t *pos,
+                     const int32_t *neg, const double *r);
+int xg_add_capacitors(xg_circuit *c, int32_t n, const int32_t *pos,
+                      const int32_t *neg, const double *cap, const int32_t *sta);
 int xg_add_vsource(xg_circuit *c, int32_t pos, int32_t neg, int32_t branch,
                    double dc, const double *pulse);
 

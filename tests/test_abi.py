@@ -132,3 +132,55 @@ def test_pattern_missing_entry_is_reported(L):
     ci2 = ci[1:].copy()
     with pytest.raises(XgError, match="PATTERN|pattern"):
         DeviceLoad(nl, plan_only=True, pattern=(rp2, ci2))
+
+
+def test_bulk_linear_adds_match_single_adds(L):
+    """xg_add_resistors / xg_add_capacitors give the plan of the same devices
+    added one by one (pattern and lists), and name the first bad device."""
+    from xyce_amd._lib import XG_E_ARG, XG_E_PARAM
+    rng = np.random.default_rng(2)
+    n, k = 50, 120
+    pos = rng.integers(-1, n, k).astype(np.int32)
+    neg = rng.integers(-1, n, k).astype(np.int32)
+    val = rng.uniform(1.0, 10.0, k)
+    sta = np.arange(k, dtype=np.int32)
+
+    def plan(bulk):
+        c = ctypes.c_void_p()
+        assert L.xg_circuit_new(ctypes.byref(c), n, k, 0) == 0
+        if bulk:
+            assert L.xg_add_resistors(c, k, pos.ctypes.data, neg.ctypes.data, val.ctypes.data) == 0
+            assert L.xg_add_capacitors(c, k, pos.ctypes.data, neg.ctypes.data, val.ctypes.data,
+                                       sta.ctypes.data) == 0
+        else:
+            for i in range(k):
+                assert L.xg_add_resistor(c, int(pos[i]), int(neg[i]), float(val[i])) == 0
+            for i in range(k):
+                assert L.xg_add_capacitor(c, int(pos[i]), int(neg[i]), float(val[i]),
+                                          int(sta[i])) == 0
+        h = ctypes.c_void_p()
+        assert L.xg_finalize_plan(c, ctypes.byref(h)) == 0
+        L.xg_circuit_free(c)
+        nr, nnz = ctypes.c_int32(), ctypes.c_int64()
+        assert L.xg_pattern_size(h, ctypes.byref(nr), ctypes.byref(nnz)) == 0
+        rp = np.zeros(nr.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        assert L.xg_get_pattern(h, rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                ci.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))) == 0
+        info = (ctypes.c_int64 * 8)()
+        assert L.xg_lu_info(h, info) == 0
+        L.xg_destroy(h)
+        return rp, ci, list(info)
+
+    a, b = plan(False), plan(True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    # a zero resistance at index 3 is refused and named
+    c = ctypes.c_void_p()
+    assert L.xg_circuit_new(ctypes.byref(c), n, k, 0) == 0
+    bad = val.copy()
+    bad[3] = 0.0
+    assert L.xg_add_resistors(c, k, pos.ctypes.data, neg.ctypes.data, bad.ctypes.data) \
+        == XG_E_PARAM
+    assert b"device 3" in L.xg_last_error()
+    assert L.xg_add_capacitors(c, -1, None, None, None, None) == XG_E_ARG
+    L.xg_circuit_free(c)

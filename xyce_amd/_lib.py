@@ -67,6 +67,11 @@ SIGNATURES = [
                                         ctypes.c_double]),
     ("xg_add_capacitor", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_double, ctypes.c_int32]),
+    ("xg_add_resistors", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_add_capacitors", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
     ("xg_add_vsource", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, ctypes.c_double,
                                        ctypes.POINTER(ctypes.c_double)]),

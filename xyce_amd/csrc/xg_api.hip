@@ -256,6 +256,34 @@ int xg_add_capacitor(xg_circuit *c, int32_t pos, int32_t neg, double cap,
                  "xg_add_capacitor");
 }
 
+int xg_add_resistors(xg_circuit *c, int32_t n, const int32_t *pos,
+                     const int32_t *neg, const double *r) {
+  if (!c || n < 0 || (n > 0 && (!pos || !neg || !r)))
+    return fail(XG_E_ARG, "xg_add_resistors: bad argument");
+  c->c.lin.reserve(c->c.lin.size() + (size_t)n);
+  for (int32_t i = 0; i < n; ++i)
+    if (int rc = add_lin(c, xg::LIN_R, pos[i], neg[i], -1, r[i], nullptr, 0,
+                         "xg_add_resistors")) {
+      g_err += " (device " + std::to_string(i) + ")";
+      return rc;
+    }
+  return XG_OK;
+}
+
+int xg_add_capacitors(xg_circuit *c, int32_t n, const int32_t *pos,
+                      const int32_t *neg, const double *cap, const int32_t *sta) {
+  if (!c || n < 0 || (n > 0 && (!pos || !neg || !cap || !sta)))
+    return fail(XG_E_ARG, "xg_add_capacitors: bad argument");
+  c->c.lin.reserve(c->c.lin.size() + (size_t)n);
+  for (int32_t i = 0; i < n; ++i)
+    if (int rc = add_lin(c, xg::LIN_C, pos[i], neg[i], -1, cap[i], nullptr, sta[i],
+                         "xg_add_capacitors")) {
+      g_err += " (device " + std::to_string(i) + ")";
+      return rc;
+    }
+  return XG_OK;
+}
+
 int xg_add_vsource(xg_circuit *c, int32_t pos, int32_t neg, int32_t branch,
                    double dc, const double *pulse) {
   return add_lin(c, xg::LIN_V, pos, neg, branch, dc, pulse, 0, "xg_add_vsource");

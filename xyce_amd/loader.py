@@ -71,11 +71,13 @@ class DeviceLoad:
                 check(L.xg_add_bsim3_instance(c, mid[rec["model"]], n, k, v, lids,
                                               rec["sta"], rec["sto"], ctypes.byref(out)),
                       "add BSIM3 " + rec["name"])
+            # resistors and capacitors in two calls (each type keeps its order)
+            rs, cs = [], []
             for kind, p, q, br, val, pulse, st in nl.linear_devices():
                 if kind == "R":
-                    check(L.xg_add_resistor(c, p, q, val), "add resistor")
+                    rs.append((p, q, val))
                 elif kind == "C":
-                    check(L.xg_add_capacitor(c, p, q, val, st), "add capacitor")
+                    cs.append((p, q, val, st))
                 elif kind == "D":
                     nm, km, vm = keys_vals(val[0])
                     ni, ki, vi = keys_vals(val[1])
@@ -83,6 +85,18 @@ class DeviceLoad:
                 else:
                     pp = (ctypes.c_double * 7)(*pulse) if pulse else None
                     check(L.xg_add_vsource(c, p, q, br, val, pp), "add source")
+            if rs:
+                a = np.array(rs, dtype=np.float64)
+                ip, iq = (np.ascontiguousarray(a[:, k], dtype=np.int32) for k in (0, 1))
+                vr = np.ascontiguousarray(a[:, 2])
+                check(L.xg_add_resistors(c, len(rs), ip.ctypes.data, iq.ctypes.data,
+                                         vr.ctypes.data), "add resistors")
+            if cs:
+                a = np.array(cs, dtype=np.float64)
+                ip, iq, ist = (np.ascontiguousarray(a[:, k], dtype=np.int32) for k in (0, 1, 3))
+                vc = np.ascontiguousarray(a[:, 2])
+                check(L.xg_add_capacitors(c, len(cs), ip.ctypes.data, iq.ctypes.data,
+                                          vc.ctypes.data, ist.ctypes.data), "add capacitors")
             if pattern is not None:
                 rp, ci = (np.ascontiguousarray(a, dtype=np.int32) for a in pattern)
                 check(L.xg_circuit_set_pattern(
