@@ -173,7 +173,16 @@ __global__ void __launch_bounds__(kBlock)
   for (int32_t c = rb; c < re; ++c) acc = acc + b.partial[p.pbase + c];
   double *out = b.out[k];
   double r = b.overwrite ? 0.0 : out[t];
-  for (int l = 0; l < 64; ++l) r = r + __shfl(acc, l, 64);
+  // lane runs in lane order; each lane's run sum read with readlane (the
+  // reads do not wait on the additions, so only the adds are serial)
+  const long long bits = __double_as_longlong(acc);
+  const int lo = (int)bits, hi = (int)(bits >> 32);
+#pragma unroll
+  for (int l = 0; l < 64; ++l) {
+    const unsigned a = (unsigned)__builtin_amdgcn_readlane(lo, l);
+    const long long h = (long long)__builtin_amdgcn_readlane(hi, l);
+    r = r + __longlong_as_double((long long)((unsigned long long)h << 32 | a));
+  }
   if (lane == 0) out[t] = r;
 }
 
