@@ -442,7 +442,8 @@ int xg_tran_uic(xg_tran *t);
 /* One step of dt (order 1 for the first two steps, then 2); a step whose
    Newton solve fails is replaced by two of dt/2, recursively. */
 int xg_tran_step(xg_tran *t, double dt);
-/* out[k] = x_curr[ids[k]] on the driver's stream (device arrays). */
+/* out[k] = x_curr[ids[k]] on the driver's stream (device arrays).  An id
+   outside 0 .. n_nodes - 1 (ground included) reads nothing and gives NaN. */
 int xg_tran_probe(xg_tran *t, const int32_t *ids, int32_t n, double *out);
 /* Time of the accepted solution; counters[0..3] = steps, Newton
    iterations, failed steps, last failure word. */

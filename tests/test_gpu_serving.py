@@ -34,6 +34,10 @@ def test_mixed_stream_batches_and_matches_alone():
         print("%s: batch %s, alone %d step failures, max difference %.3g V"
               % (r.kind, r.batch_info, alone.step_failures, np.abs(w - wb).max()), flush=True)
         assert np.array_equal(t, tb)
-        vmax = 1.2 if r.kind != "ladder" else 1.0
-        assert np.abs(w - wb).max() <= 1e-3 * vmax + 1e-6, r.kind
+        if r.batch_info["step_failures"] == 0 and alone.step_failures == 0:
+            # no step of either run was halved: the same bits
+            assert np.array_equal(w, wb), r.kind
+        else:
+            vmax = 1.2 if r.kind != "ladder" else 1.0
+            assert np.abs(w - wb).max() <= 1e-3 * vmax + 1e-6, r.kind
     assert checked == {"ring", "logic", "ladder"}

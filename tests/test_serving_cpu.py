@@ -26,3 +26,40 @@ def test_footprint_grows_with_the_circuit():
     for kind, v in sizes.items():
         v.sort()
         assert all(b2 >= b1 for (_, b1), (_, b2) in zip(v, v[1:])), kind
+
+
+def test_oversized_request_is_refused_and_serve_returns():
+    """A request whose footprint exceeds the whole HBM budget is refused at
+    arrival (error set, completion time recorded) and the loop ends; no
+    batch is admitted, so no GPU is needed."""
+    import threading
+    from xyce_amd.serving import serve
+
+    reqs = poisson_requests(n=3, seed=4)
+    budget = min(footprint_bytes(r.netlist()) for r in reqs) - 1
+    done = []
+    th = threading.Thread(target=lambda: done.append(serve(reqs, hbm_budget=budget,
+                                                           realtime=False)), daemon=True)
+    th.start()
+    th.join(timeout=30)
+    assert not th.is_alive(), "serve() did not return"
+    out = done[0]
+    assert len(out) == 3
+    for r in out:
+        assert r.error and r.error.startswith("refused"), r.error
+        assert r.t_done is not None and r.result is None
+
+
+def test_probe_lookup_is_read_only_and_checked():
+    """Probes resolve against the laid-out node table without adding
+    nodes: unknown names raise KeyError and ground raises ValueError."""
+    import pytest
+    from xyce_amd import circuits
+    nl = circuits.ring_oscillator_bsim3(5, seed=1).layout()
+    n_before = len(nl.node_ids)
+    assert nl.probe_id("out0") == nl.node_ids["out0"]
+    with pytest.raises(KeyError):
+        nl.probe_id("out_typo")
+    with pytest.raises(ValueError):
+        nl.probe_id("0")
+    assert len(nl.node_ids) == n_before

@@ -80,6 +80,20 @@ class Netlist:
             self.node_ids[name] = len(self.node_ids)
         return self.node_ids[name]
 
+    def probe_id(self, name):
+        """Solution id of an existing node, for reading results: never
+        creates a node.  KeyError for a name the netlist does not hold;
+        ValueError for ground, which has no unknown of its own."""
+        if name in GROUND_NAMES:
+            raise ValueError("ground (%r) is not a probe: it has no unknown" % name)
+        if name not in self.node_ids:
+            raise KeyError("no node named %r in %r" % (name, self.title))
+        i = self.node_ids[name]
+        n = getattr(self, "n_nodes", len(self.node_ids))
+        if not 0 <= i < n:
+            raise KeyError("node %r has id %d outside 0..%d" % (name, i, n - 1))
+        return i
+
     def add_model(self, name, kind, dtype, **params):
         self.models[name] = Model(name, kind, dtype, dict(params))
 

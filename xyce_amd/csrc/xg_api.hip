@@ -2,7 +2,10 @@
 // per-Newton load entry points.  See include/xg_load.h for the contract.
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <memory>
+#include <new>
+#include <stdexcept>
 #include <mutex>
 #include <string>
 
@@ -260,7 +263,8 @@ int xg_add_resistors(xg_circuit *c, int32_t n, const int32_t *pos,
                      const int32_t *neg, const double *r) {
   if (!c || n < 0 || (n > 0 && (!pos || !neg || !r)))
     return fail(XG_E_ARG, "xg_add_resistors: bad argument");
-  c->c.lin.reserve(c->c.lin.size() + (size_t)n);
+  if (c->c.lin.size() + (size_t)n > c->c.lin.capacity())
+    c->c.lin.reserve(std::max(c->c.lin.size() + (size_t)n, 2 * c->c.lin.capacity()));
   for (int32_t i = 0; i < n; ++i)
     if (int rc = add_lin(c, xg::LIN_R, pos[i], neg[i], -1, r[i], nullptr, 0,
                          "xg_add_resistors")) {
@@ -274,7 +278,8 @@ int xg_add_capacitors(xg_circuit *c, int32_t n, const int32_t *pos,
                       const int32_t *neg, const double *cap, const int32_t *sta) {
   if (!c || n < 0 || (n > 0 && (!pos || !neg || !cap || !sta)))
     return fail(XG_E_ARG, "xg_add_capacitors: bad argument");
-  c->c.lin.reserve(c->c.lin.size() + (size_t)n);
+  if (c->c.lin.size() + (size_t)n > c->c.lin.capacity())
+    c->c.lin.reserve(std::max(c->c.lin.size() + (size_t)n, 2 * c->c.lin.capacity()));
   for (int32_t i = 0; i < n; ++i)
     if (int rc = add_lin(c, xg::LIN_C, pos[i], neg[i], -1, cap[i], nullptr, sta[i],
                          "xg_add_capacitors")) {
@@ -800,17 +805,39 @@ static bool upload_handle(xg_handle *H) {
 
 extern "C" {
 
+namespace {
+// Host setup allocates freely; no exception may cross the C boundary.
+int build_plan_nothrow(xg::Circuit &c, xg::Plan &plan, std::string &err) {
+  try {
+    return xg::build_plan(c, plan, err);
+  } catch (const std::bad_alloc &) {
+    err = "setup: out of host memory";
+    return XG_E_NOMEM;
+  } catch (const std::exception &e) {
+    err = std::string("setup: ") + e.what();
+    return XG_E_ARG;
+  }
+}
+bool upload_handle_nothrow(xg_handle *H) {
+  try {
+    return upload_handle(H);
+  } catch (const std::exception &) {
+    return false;
+  }
+}
+}  // namespace
+
 int xg_finalize(xg_circuit *c, xg_handle **out) {
   if (!c || !out) return fail(XG_E_ARG, "xg_finalize: bad argument");
   auto *H = new (std::nothrow) xg_handle;
   if (!H) return fail(XG_E_NOMEM, "xg_finalize: out of memory");
   std::string err;
-  int rc = xg::build_plan(c->c, H->h.plan, err);
+  int rc = build_plan_nothrow(c->c, H->h.plan, err);
   if (rc) {
     delete H;
     return fail(rc, err);
   }
-  if (!upload_handle(H)) {
+  if (!upload_handle_nothrow(H)) {
     delete H;
     return fail(XG_E_NOMEM, "xg_finalize: device allocation/upload failed");
   }
@@ -821,7 +848,7 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
 int xg_upload(xg_handle *H) {
   if (!H) return fail(XG_E_ARG, "xg_upload: bad argument");
   if (!H->h.plan_only) return fail(XG_E_STATE, "xg_upload: handle is already on the device");
-  if (!upload_handle(H)) return fail(XG_E_NOMEM, "xg_upload: device allocation/upload failed");
+  if (!upload_handle_nothrow(H)) return fail(XG_E_NOMEM, "xg_upload: device allocation/upload failed");
   H->h.plan_only = false;
   return XG_OK;
 }
@@ -831,7 +858,7 @@ int xg_finalize_plan(xg_circuit *c, xg_handle **out) {
   auto *H = new (std::nothrow) xg_handle;
   if (!H) return fail(XG_E_NOMEM, "xg_finalize_plan: out of memory");
   std::string err;
-  int rc = xg::build_plan(c->c, H->h.plan, err);
+  int rc = build_plan_nothrow(c->c, H->h.plan, err);
   if (rc) {
     delete H;
     return fail(rc, err);

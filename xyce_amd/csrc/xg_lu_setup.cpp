@@ -36,6 +36,8 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <new>
+#include <stdexcept>
 #include <system_error>
 #include <thread>
 #include <vector>
@@ -663,22 +665,35 @@ int lu_analyse(Plan &plan, std::string &err) {
            plan.comp_start[b + 1] - plan.comp_start[b];
   });
   std::atomic<int32_t> next{0};
+  // an allocation failure inside a component's analysis is that
+  // component's result (XG_E_NOMEM), never an exception leaving a thread
   auto work = [&]() {
     for (int32_t k; (k = next.fetch_add(1)) < nc;) {
       const int32_t c = by_size[k];
-      analyse_component(plan, L, c, comp_pairs[c], res[c]);
+      try {
+        analyse_component(plan, L, c, comp_pairs[c], res[c]);
+      } catch (const std::bad_alloc &) {
+        res[c] = CompLu();
+        res[c].rc = XG_E_NOMEM;
+        res[c].err = "sparse LU analysis: out of host memory";
+      } catch (const std::exception &e) {
+        res[c] = CompLu();
+        res[c].rc = XG_E_ARG;
+        res[c].err = std::string("sparse LU analysis: ") + e.what();
+      }
     }
   };
   const int32_t n_threads =
       std::max(1, std::min<int32_t>({nc, 16, (int32_t)std::thread::hardware_concurrency()}));
   {
     std::vector<std::thread> pool;
+    pool.reserve(n_threads);
     try {
       for (int32_t t = 1; t < n_threads; ++t) pool.emplace_back(work);
-    } catch (const std::system_error &) {
+    } catch (const std::exception &) {
       // no more threads: the ones started and this one share the work
     }
-    work();
+    work();  // catches inside, so the pool is always joined below
     for (auto &th : pool) th.join();
   }
   for (int32_t c = 0; c < nc; ++c) {

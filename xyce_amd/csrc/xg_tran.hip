@@ -189,10 +189,16 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n_store) sto_c[i] = sto_n[i];
 }
 
-__global__ void probe_kernel(const double *x, const int32_t *ids, int32_t n,
-                             double *out) {
+// ids outside 0..n_nodes-1 (ground, a typo mapped to nothing) read no
+// memory: their output is NaN, so a bad probe is visible, never another
+// node's waveform.
+__global__ void probe_kernel(const double *x, int64_t n_nodes, const int32_t *ids,
+                             int32_t n, double *out) {
   const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i < n) out[i] = x[ids[i]];
+  if (i < n) {
+    const int32_t id = ids[i];
+    out[i] = (id >= 0 && id < n_nodes) ? x[id] : __builtin_nan("");
+  }
 }
 
 unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -482,7 +488,7 @@ int xg_tran_probe(xg_tran *T, const int32_t *ids, int32_t n, double *out) {
   if (!T || (n > 0 && (!ids || !out)) || n < 0) return XG_E_ARG;
   if (n == 0) return XG_OK;
   hipLaunchKernelGGL(xg::probe_kernel, dim3((n + 63) / 64), dim3(64), 0, T->stream,
-                     (const double *)T->b.x_curr, ids, n, out);
+                     (const double *)T->b.x_curr, T->n1 - 1, ids, n, out);
   return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
 }
 

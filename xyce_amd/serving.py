@@ -153,10 +153,18 @@ def serve(requests, max_batch=32, workers=3, window=0.005, hbm_budget=64 << 30,
           realtime=True, log=None):
     """Runs the request stream; returns the requests with results,
     admission and completion times (seconds on the stream's clock)."""
-    import torch
-    torch.cuda.init()            # the HIP runtime starts here, not in a worker
-    from . import _lib
-    _lib.lib()
+    runtime = []
+
+    def start_runtime():
+        # the HIP runtime starts on this thread, not in a worker, and only
+        # once a batch is admitted
+        if not runtime:
+            import torch
+            torch.cuda.init()
+            from . import _lib
+            _lib.lib()
+            runtime.append(True)
+
     sync = None
     pending = {}             # kind -> list of arrived requests
     lock = threading.Lock()
@@ -195,10 +203,22 @@ def serve(requests, max_batch=32, workers=3, window=0.005, hbm_budget=64 << 30,
     n = len(reqs)
     while True:
         t = now()
-        # arrivals
+        # arrivals; a request larger than the whole budget can never be
+        # admitted: it is refused at once instead of waiting forever
         while i < n and (reqs[i].arrival <= t or not realtime):
-            pending.setdefault(reqs[i].kind, []).append(reqs[i])
+            r = reqs[i]
             i += 1
+            if r._bytes > hbm_budget:
+                r.error = ("refused: footprint %d bytes exceeds the HBM budget of %d bytes"
+                           % (r._bytes, hbm_budget))
+                with lock:
+                    r.t_done = now()
+                    state["done"] += 1
+                if log:
+                    log("t=%.3f refused request %d (%.1f MB > budget)" % (r.t_done, r.rid,
+                                                                       r._bytes / 1e6))
+                continue
+            pending.setdefault(r.kind, []).append(r)
         # admission: the oldest kind first, as slots and memory allow
         with lock:
             kinds = sorted((q[0].arrival, k) for k, q in pending.items() if q)
@@ -213,6 +233,7 @@ def serve(requests, max_batch=32, workers=3, window=0.005, hbm_budget=64 << 30,
                     need += r._bytes
                 if not take:
                     continue
+                start_runtime()
                 b = _Batch(take)
                 b.bytes = need
                 state["free"] -= need

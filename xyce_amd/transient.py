@@ -316,6 +316,16 @@ class Transient:
         self.t = t_next
         self.n_steps += 1
 
+    def _probe_ids(self, probes):
+        """Solution ids of the probed nodes, checked on the host before any
+        device read: unknown names raise KeyError, ground ValueError."""
+        ids = [self.nl.probe_id(p) for p in probes]
+        n = self.nl.n_nodes
+        for p, i in zip(probes, ids):
+            if not 0 <= i < n:
+                raise KeyError("probe %r maps to id %d outside 0..%d" % (p, i, n - 1))
+        return ids
+
     def run(self, tstop, dt, probes, every=1, uic=False):
         """DC operating point (or, uic=True, the zero start), then steps of
         dt to tstop.  Returns (times, waveforms[probe, time]) for the given
@@ -323,7 +333,7 @@ class Transient:
         torch = self.torch
         if self._tr is not None:
             return self._run_native(tstop, dt, probes, every, uic)
-        idx = torch.tensor([self.nl.node(p) for p in probes], device=self.x.device)
+        idx = torch.tensor(self._probe_ids(probes), device=self.x.device)
         self.uic() if uic else self.dcop()
         n = int(round(tstop / dt))
         out = torch.zeros(len(probes), n // every + 1, dtype=torch.float64,
@@ -344,7 +354,7 @@ class Transient:
         L = lib()
         n = int(round(tstop / dt))
         with DEVICE_LOCK:
-            ids = torch.tensor([self.nl.node(p) for p in probes], dtype=torch.int32,
+            ids = torch.tensor(self._probe_ids(probes), dtype=torch.int32,
                                device=self.x.device)
             out = torch.zeros(n // every + 1, len(probes), dtype=torch.float64,
                               device=self.x.device)
