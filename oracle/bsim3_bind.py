@@ -111,7 +111,7 @@ class RefBsim3:
         return lim[:n], org[:n]
 
     def sdp(self, i):
-        out = np.zeros(27)
+        out = np.zeros(30)
         self.L.ref_b3_sdp_probe(self.M, i, out.ctypes.data_as(_dp))
         return out
 

@@ -1944,13 +1944,15 @@ void ref_b3_flags(const ref_b3_master *M, int *limited, int *orig) {
 /* Size-dependent parameters of instance i, for checking other setups:
    out[0..]=leff, weff, leffCV, weffCV, vth0, vfb, vfbzb, k1, k2, k1ox,
    phi, u0temp, vsattemp, rds0, cgso, cgdo, cgbo, abulkCVfactor, litl,
-   thetaRout, theta0vb0, cdep0, ldeb, acde, Xdep0, vbsc, vbi. */
+   thetaRout, theta0vb0, cdep0, ldeb, acde, Xdep0, vbsc, vbi, and the
+   temperature-adjusted mobility coefficients ua, ub, uc (27..29). */
 void ref_b3_sdp_probe(const ref_b3_master *M, int i, double *out) {
   const ref_b3_sdp *p = M->inst[i].paramPtr;
-  double v[27] = {p->leff, p->weff, p->leffCV, p->weffCV, p->vth0, p->vfb,
+  double v[30] = {p->leff, p->weff, p->leffCV, p->weffCV, p->vth0, p->vfb,
                   p->vfbzb, p->k1, p->k2, p->k1ox, p->phi, p->u0temp,
                   p->vsattemp, p->rds0, p->cgso, p->cgdo, p->cgbo,
                   p->abulkCVfactor, p->litl, p->thetaRout, p->theta0vb0,
-                  p->cdep0, p->ldeb, p->acde, p->Xdep0, p->vbsc, p->vbi};
+                  p->cdep0, p->ldeb, p->acde, p->Xdep0, p->vbsc, p->vbi,
+                  p->ua, p->ub, p->uc};
   memcpy(out, v, sizeof(v));
 }

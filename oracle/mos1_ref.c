@@ -868,3 +868,34 @@ void ref_mos1_flags(const ref_mos1_master *M, int *limited, int *orig) {
     orig[i] = M->inst[i].origFlag;
   }
 }
+
+/* ---- Probes for pinning tests (no counterpart in the reference) ---- */
+
+/* The SPICE3 limiters and Meyer capacitances of ref_common.h as callable
+   functions (DeviceSupport::fetlim / limvds / pnjlim / qmeyer,
+   N_DEV_DeviceSupport.C:186-251, 337-421, 545-614). */
+double ref_x_fetlim(double vnew, double vold, double vto) {
+  return ref_fetlim(vnew, vold, vto);
+}
+double ref_x_limvds(double vnew, double vold) { return ref_limvds(vnew, vold); }
+double ref_x_pnjlim(double vnew, double vold, double vt, double vcrit, int *icheck) {
+  return ref_pnjlim(vnew, vold, vt, vcrit, icheck);
+}
+void ref_x_qmeyer(double vgs, double vgd, double vgb, double von, double vdsat,
+                  double phi, double cox, double *out3) {
+  ref_qmeyer(vgs, vgd, vgb, von, vdsat, &out3[0], &out3[1], &out3[2], phi, cox);
+}
+
+/* Temperature- and geometry-derived fields of instance i
+   (Instance::updateTemperature, N_DEV_MOSFET1.C:3437-3680): out[0..13] =
+   vt, tPhi, tVto, tVbi, tSatCur, tTransconductance, tSurfMob, tBulkPot,
+   tDepCap, tCbd, tCbs, sourceVcrit, EffectiveLength, OxideCap. */
+void ref_mos1_probe(const ref_mos1_master *M, int i, double *out) {
+  const ref_mos1_inst *in = &M->inst[i];
+  double v[14] = {in->vt, in->tPhi, in->tVto, in->tVbi, in->tSatCur,
+                  in->tTransconductance, in->tSurfMob, in->tBulkPot, in->tDepCap,
+                  in->tCbd, in->tCbs, in->sourceVcrit, in->EffectiveLength,
+                  in->OxideCap};
+  int k;
+  for (k = 0; k < 14; ++k) out[k] = v[k];
+}
