@@ -62,9 +62,9 @@ def _kv(params):
 
 
 class RefBsim3:
-    def __init__(self, parent, nl, gmin, limiter, temp):
+    def __init__(self, parent, nl, gmin, limiter, temp, L=None):
         from .oracle import lib
-        L = lib()
+        L = L if L is not None else lib()
         self.L = L
         self.nl = nl
         self.models = {}
