@@ -47,11 +47,32 @@ B3_EVAL_TRAFFIC_BYTES = 29.66e6
 B3_STAGED_MAX_WAVES = 2 * 256
 
 
-def b3_kernel_name(n_b3):
+def b3_kernel_name(n_b3, variant=0):
+    """The BSIM3 eval kernel a launch uses: the library's default rule for
+    variant 0 (automatic), else the form the autotuner chose."""
     waves = (n_b3 + 63) // 64
-    if waves > B3_STAGED_MAX_WAVES:
+    if variant == 0:
+        variant = 1 if waves > B3_STAGED_MAX_WAVES else 3
+    if variant == 1:
         return "bsim3_eval_kernel"
+    if variant == 2:
+        return "bsim3_eval_staged_kernel<false, 1>"
     return "bsim3_eval_staged_kernel<true, %d>" % (2 if waves > 256 else 1)
+
+
+def autotune_choice(dev):
+    """The autotuner's launch choice for a handle (xg_autotune_info)."""
+    from xyce_amd._lib import lib
+    ch = np.zeros(3, np.int32)
+    us = np.zeros(6)
+    lib().xg_autotune_info(dev.h, ch.ctypes.data, us.ctypes.data)
+    names = {0: "auto", 1: "hbm", 2: "staged", 3: "staged_split"}
+    return {"eval_form": names[int(ch[0])], "linear_in_bsim3_launch": bool(ch[1]),
+            "source": ["default", "timed", "cached"][int(ch[2])],
+            "us_per_launch": {"hbm": us[0], "hbm_linear_apart": us[1], "staged": us[2],
+                              "staged_linear_apart": us[3], "staged_split": us[4],
+                              "staged_split_linear_apart": us[5]},
+            "variant": int(ch[0])}
 
 
 def _finite(o):
@@ -104,6 +125,11 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-grid", action="store_true",
                     help="skip the power-grid (100k inverter) load timing")
+    ap.add_argument("--no-big-tran", action="store_true",
+                    help="skip the large single-circuit transients (5,000-gate logic "
+                         "5 ns, power grid 2 ns)")
+    ap.add_argument("--big-dt", type=float, default=10e-12,
+                    help="fixed step of the large single-circuit transients")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 512 MiB device copy, the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -267,10 +293,65 @@ def power_grid_leg(torch, args):
     return res
 
 
+def large_transient_leg(torch, args, which):
+    """One large single circuit's whole transient on the device: "logic" =
+    the 5,000-gate random BSIM3 logic (20,000 FETs, one 20k-unknown
+    component; .TRAN 5 ns from a zero start with the supply ramped over
+    50 ps), "grid" = the 500 x 500 power grid with 100,000 BSIM3 inverters
+    (550,004 unknowns; DC operating point, then .TRAN 2 ns).  Fixed steps
+    of --big-dt.  Each Newton iteration is one graph: device load, ONESTEP
+    assembly, the dense-front (multifrontal) LU of the component, update,
+    norms and the convergence test.  Setup (netlist, analysis, upload,
+    graph capture) is reported apart from the run."""
+    from xyce_amd import circuits
+    from xyce_amd._lib import lib
+    from xyce_amd.transient import Transient
+    if which == "logic":
+        nl = circuits.random_logic_bsim3(n_gates=args.gates, seed=2, ramp=50e-12)
+        tstop, probes, uic = 5e-9, ["g0", "g1"], True
+        name = ("random CMOS logic, %d NAND2/NOR2 gates, %d BSIM3 (seed 2), supply ramped "
+                "over 50 ps" % (args.gates, 4 * args.gates))
+    else:
+        nl = circuits.power_grid_bsim3()
+        tstop, probes, uic = 2e-9, ["out0", "g250_250"], False
+        name = "500x500 power grid + 100,000 BSIM3 inverters (seed 5)"
+    progress("large transient (%s): setup" % which)
+    t0 = time.perf_counter()
+    tr = Transient(nl)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    info = np.zeros(6, np.int64)
+    lib().xg_lu_front_info(tr.dev.h, info.ctypes.data)
+    progress("large transient (%s): %d unknowns, setup %.1f s, %d fronts, %d levels, "
+             "%.3g flops per factorisation" % (which, nl.n_nodes, setup, info[1], info[2],
+                                               info[5]))
+    t0 = time.perf_counter()
+    t, w = tr.run(tstop, args.big_dt, probes, every=10, uic=uic)
+    torch.cuda.synchronize()
+    run = time.perf_counter() - t0
+    tr.close()
+    progress("large transient (%s): %.2f s, %d Newton iterations" % (which, run,
+                                                                     tr.newton_total))
+    return {
+        "workload": name + "; .TRAN %.3g s at %.3g s steps" % (tstop, args.big_dt),
+        "unknowns": nl.n_nodes,
+        "solves_per_s": 1.0 / run,
+        "run_s": run,
+        "setup_s": setup,
+        "steps": int(tr.n_steps),
+        "newton_iterations": int(tr.newton_total),
+        "step_failures": int(tr.step_failures),
+        "ms_per_newton_iteration": 1e3 * run / max(tr.newton_total, 1),
+        "fronts": {"count": int(info[1]), "levels": int(info[2]), "values": int(info[3]),
+                   "largest": int(info[4]), "flops_per_factorisation": int(info[5])},
+        "final_probe_V": [float(v) for v in w[:, -1]],
+    }
+
+
 def serving_leg(args):
     """A Poisson stream of transient-simulation requests (seed 4): 60%
     BSIM3 rings of 11-101 stages (0.1 ns), 30% random BSIM3 logic of
-    125-400 gates (0.05 ns), 10% RC ladders of 10,000 nodes with 1,000
+    125-1,250 gates (500-5,000 FETs, 0.05 ns), 10% RC ladders of 10,000 nodes with 1,000
     diodes (0.05 ns), all at 1 ps steps.  Batched: continuous batching
     (micro-batches of one kind every 5 ms, up to 64 requests, 3 batches in
     flight).  Plain: the same stream (its first --serve-plain requests), one
@@ -278,17 +359,18 @@ def serving_leg(args):
     including the library setup and upload of each batch."""
     from xyce_amd.serving import poisson_requests, serve, summary
     progress("serving: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
-    reqs = poisson_requests(rate=args.serve_rate, n=args.serve_requests, seed=4)
+    reqs = poisson_requests(rate=args.serve_rate, n=args.serve_requests, seed=4,
+                            logic_gates=(125, 1250))
     b = summary(serve(reqs, max_batch=64, workers=3, window=0.005))
     progress("serving: batched %.1f solves/s, p99 %.3f s" % (b["solves_per_s"],
                                                             b["latency_p99_s"]))
     res = {"workload": "Poisson %.0f req/s, %d requests: 60%% BSIM3 rings 11-101 stages, "
-                       "30%% random BSIM3 logic 125-400 gates, 10%% RC ladders 10k nodes + "
+                       "30%% random BSIM3 logic 125-1250 gates, 10%% RC ladders 10k nodes + "
                        "1k diodes (seed 4)" % (args.serve_rate, args.serve_requests),
            "batched": b}
     if args.serve_plain > 0:
-        plain = poisson_requests(rate=args.serve_rate, n=args.serve_requests,
-                                 seed=4)[:args.serve_plain]
+        plain = poisson_requests(rate=args.serve_rate, n=args.serve_requests, seed=4,
+                                 logic_gates=(125, 1250))[:args.serve_plain]
         p = summary(serve(plain, max_batch=1, workers=1, window=0.001))
         progress("serving: plain %.1f solves/s, p99 %.3f s" % (p["solves_per_s"],
                                                               p["latency_p99_s"]))
@@ -369,6 +451,7 @@ def main():
     sto = rng.uniform(-0.2, 1.5, nl.n_store)
 
     dev = DeviceLoad(nl)
+    tune = _finite(autotune_choice(dev))
     stream = torch.cuda.current_stream()
     dev.set_stream(stream.cuda_stream)
     dev.set_solver_state(transient=1, newtonIter=1)
@@ -469,8 +552,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": B3_EVAL_TRAFFIC_BYTES if args.gates == 5000 else None,
-                     "kernel": b3_kernel_name(n_b3), "kernel_ms": eval_ms,
+                     "kernel": b3_kernel_name(n_b3, tune["variant"]), "kernel_ms": eval_ms,
                      "bytes_per_eval": B3_BYTES_PER_EVAL},
+        "autotune": tune,
         "wall_s_timed_region": wall,
         "launch": launch,
         "ms_per_step_eager": ms_eager,
@@ -484,6 +568,9 @@ def main():
         res["batched_transient"] = batched_transient_leg(torch, args)
     if not args.no_grid:
         res["power_grid"] = power_grid_leg(torch, args)
+    if not args.no_big_tran:
+        res["logic_transient"] = large_transient_leg(torch, args, "logic")
+        res["grid_transient"] = large_transient_leg(torch, args, "grid")
     if not args.no_serve:
         res["serving"] = serving_leg(args)
     if not args.no_cpu:

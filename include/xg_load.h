@@ -27,7 +27,7 @@
  *    reproducible run to run;
  *  - all pointers passed to the xg_update_state / xg_load_* calls are
  *    device (HBM) pointers; the xg_*_host variants take host pointers and
- *    stage through pinned buffers;
+ *    stage through a pinned buffer (synchronous);
  *  - every call returns an int status: XG_OK (0) or a negative XG_E_* code.
  *    The library never aborts.  xg_last_error() gives a message.
  *
@@ -112,7 +112,8 @@ int xg_add_mos1_instance(xg_circuit *c, int32_t model_id, int32_t n,
                          int32_t sto_base, int32_t *inst_id);
 
 /* BSIM3 v3.2 model card (parameter table N_DEV_MOSFET_B3.C:132-1820;
-   Model::processParams N_DEV_MOSFET_B3.C:8488-8596). */
+   Model::processParams N_DEV_MOSFET_B3.C:8488-8596).  CAPMOD 0, 1, 2 and 3
+   (default) are evaluated; any other value is XG_E_PARAM. */
 int xg_add_bsim3_model(xg_circuit *c, int32_t dtype, int32_t n,
                        const char *const *keys, const double *vals,
                        int32_t *model_id);
@@ -231,6 +232,22 @@ int xg_load_all(xg_handle *h, const double *x_next, double *sta_curr,
                 double *F, double *Q, double *dFdxdVp, double *dQdxdVp,
                 double *dFdx_vals, double *dQdx_vals);
 
+/* The three per-Newton calls with host pointers, for a caller whose
+   vectors and CSR value arrays stay in host memory (the reference's
+   N_LAS objects): the same arguments and results as xg_update_state,
+   xg_load_vectors and xg_load_matrices, staged through a pinned buffer to
+   device mirrors and back; each returns when its results are in the
+   caller's arrays.  xg_update_state_host reads x_next, sta_curr, sto_curr
+   and sto_next, and writes sta_next, sto_next and sta_curr (initTran);
+   the loads then reduce its statement values into the caller's arrays
+   (added to them, or written in XG_OUT_OVERWRITE).  The mirrors and the
+   staging buffer are allocated by the first call. */
+int xg_update_state_host(xg_handle *h, const double *x_next, double *sta_curr,
+                         const double *sto_curr, double *sta_next, double *sto_next);
+int xg_load_vectors_host(xg_handle *h, double *F, double *Q, double *dFdxdVp,
+                         double *dQdxdVp);
+int xg_load_matrices_host(xg_handle *h, double *dFdx_vals, double *dQdx_vals);
+
 int xg_get_stats(xg_handle *h, xg_stats *st);
 int xg_sync(xg_handle *h);
 
@@ -260,6 +277,19 @@ int xg_set_output_mode(xg_handle *h, int32_t mode);
 #define XG_EVAL_STAGED 2       /* 1 wave per block, records staged in LDS   */
 #define XG_EVAL_STAGED_SPLIT 3 /* as STAGED, DC and charge parts on two waves */
 int xg_set_eval_variant(xg_handle *h, int32_t variant);
+
+/* Autotuner (no counterpart in the reference): xg_finalize / xg_upload time
+   the BSIM3 eval kernel forms above, with the linear devices in trailing
+   workgroups of the BSIM3 launch or in a launch of their own, on the
+   circuit itself (circuits of 8 eval waves or more), keep the fastest, and
+   cache the choice by circuit shape for later circuits.  All candidates
+   compute the same bits.  choice[0] = the eval form, [1] = 1 if the linear
+   devices share the BSIM3 launch, [2] = 0 not tuned (default choice), 1
+   timed, 2 taken from the cache; us[6] = microseconds per launch of HBM,
+   STAGED and STAGED_SPLIT with the linear devices shared then separate
+   (NaN where not timed).  Either pointer may be NULL.  xg_set_eval_variant
+   overrides the form afterwards. */
+int xg_autotune_info(const xg_handle *h, int32_t choice[3], double us[6]);
 
 /* Junction diode (N_DEV_Diode.C, level 1, RS = 0 so the internal node is
    the positive one): .MODEL parameters IS (JS), N, TT, CJO (CJ, CJ0), VJ,
@@ -329,6 +359,38 @@ int xg_lu_solve(xg_handle *h, const double *J, const double *rhs, double *x,
    [7] factor and solve tasks.  No counterpart in the reference. */
 int xg_lu_info(xg_handle *h, int64_t info[8]);
 
+/* Components too large for the task lists of xg_lu_info (more than 4 M
+   fill entries or 64 M multiply-adds) are factored by a supernodal
+   multifrontal LU on dense fronts (partial pivoting among each front's
+   fully summed rows).  info[0] such components, [1] fronts, [2] front
+   levels (heights of the elimination tree), [3] doubles of front storage,
+   [4] largest front order, [5] floating-point operations per
+   factorisation.  No counterpart in the reference. */
+int xg_lu_front_info(const xg_handle *h, int64_t info[6]);
+
+/* Host copy of the front plan, for inspection: sizes[0..5] = fronts,
+   front rows (sum of orders), children, levels, matrix entries mapped,
+   components.  i32 (entries may be NULL) = fr_comp[fronts], m[fronts],
+   k[fronts], rows[front rows] (global positions, ascending per front),
+   cbmap[front rows] (row t >= k of a front: its index in the parent
+   front), ch_ptr[fronts + 1], ch[children], lv_ptr[levels + 1],
+   lv[fronts] (fronts by height), ent_k[entries] (CSR entries),
+   front_comp[components]; i64 = fo[fronts + 1] (value offsets; front s
+   is m x m row-major), ro[fronts + 1] (row offsets), ent_fpos[entries]
+   (value position of each mapped CSR entry).  No counterpart in the
+   reference. */
+int xg_lu_front_export(const xg_handle *h, int64_t sizes[6], int32_t *const *i32,
+                       int64_t *const *i64);
+
+/* Linear-solve form of a circuit, set before xg_finalize: XG_LU_AUTO
+   (default) factors components by task lists within their budgets and by
+   dense fronts beyond them; XG_LU_FRONTS factors every component by dense
+   fronts.  No counterpart in the reference (a solver choice, like its
+   linear-solver options). */
+#define XG_LU_AUTO 0
+#define XG_LU_FRONTS 1
+int xg_circuit_set_lu_mode(xg_circuit *c, int32_t mode);
+
 /* Host copy of the sparse LU plan, for inspection: sizes[0..5] = n_nodes,
    CSR entries, components, levels + 1, tasks, multiply-add terms; arrays
    (may be NULL, then only sizes) = prow[n_nodes], qcol[n_nodes],
@@ -395,12 +457,13 @@ int xg_sizes(const xg_handle *h, int32_t *n_nodes, int32_t *n_state,
 /* ---- Transient driver: the DC operating point and ONESTEP steps with the
    whole Newton loop on the device (replaces the host-side loop of
    N_NLS_DampedNewton::solve, N_NLS_DampedNewton.C:400-700, and the step
-   sequence of N_TIA_OneStep with fixed steps; no LTE step control).  Every
-   iteration is one hipGraph launch (xg_lu_solve, xg_load_all,
+   sequence of N_TIA_OneStep with fixed steps; no LTE step control).  The
+   iterations run as hipGraph launches (xg_lu_solve, xg_load_all,
    xg_tia_assemble, xg_rhs_norms, xg_component_limited, the non-finite
    count and the convergence test of N_NLS_DampedNewton.C:1787-1830 per
-   independent sub-circuit); the host reads four control words per
-   iteration and halves a step whose Newton solve fails.  Parameters:
+   independent sub-circuit); several iterations share one graph launch and
+   the host reads the control words once per launch (xg_tran_counters),
+   halving a step whose Newton solve fails.  Parameters:
    DC reltol 1e-3, abstol 1e-12, RHSTol 1e-6, deltaXTol 1, 200 iterations;
    transient 1e-2, 1e-6, 1e-2, 0.33, 20 (N_NLS_NLParams.C:108-125).
 
@@ -452,6 +515,13 @@ int xg_tran_info(xg_tran *t, double *time, int64_t counters[4]);
    (the caller restores the vectors it owns). */
 int xg_tran_restore(xg_tran *t, double time, int64_t n_steps, int64_t newton_total,
                     int64_t step_failures);
+/* Driver counters: out[0] control-word reads (host round trips), [1]
+   steps, [2] Newton iterations, [3] iterations the next step's graph
+   holds.  A step's graph holds as many iterations as the previous step
+   took (at most 4), every launch of an iteration after convergence doing
+   nothing, so a step usually costs one launch and one read.  No
+   counterpart in the reference. */
+int xg_tran_counters(xg_tran *t, int64_t out[4]);
 /* 0: launch every iteration's kernels one by one instead of a graph. */
 int xg_tran_set_graphs(xg_tran *t, int32_t on);
 int xg_tran_sync(xg_tran *t);

@@ -95,7 +95,7 @@ class RefBsim3:
         rc = self.L.ref_b3_update_state(self.M, *[a.ctypes.data_as(_dp) for a in
                                                   (x, sta_c, sto_c, sta_n, sto_n)])
         if rc:
-            raise ValueError("oracle BSIM3 load does not cover this model (capMod != 3)")
+            raise ValueError("oracle BSIM3 load does not cover this model (capMod outside 0-3)")
 
     def load_vectors(self, F, Q, fl, ql):
         self.L.ref_b3_load_vectors(self.M, *[a.ctypes.data_as(_dp) for a in (F, Q, fl, ql)])

@@ -11,11 +11,11 @@
  * Restated: Model::Model/processParams (N_DEV_MOSFET_B3.C:8481-9100),
  * Instance::Instance/processParams (1845-2785, stamp part handled by
  * stamp_ref.py), updateTemperature with the SizeDependParam cache
- * (3611-4518), updateIntermediateVars (4526-7273) for capMod 3 (the
- * default) and every mobMod, setupCapacitors_oldDAE/newDAE (7558-7885),
+ * (3611-4518), updateIntermediateVars (4526-7273) for capMod 0/1/2/3
+ * and every mobMod, setupCapacitors_oldDAE/newDAE (7558-7885),
  * auxChargeCalculations (7465-7555), Master::updateState /
  * loadDAEVectors / loadDAEMatrices (9222-9928).
- * Not restated: capMod 0/1/2, nqsMod, IC= (extra branch-current unknowns),
+ * Not restated: nqsMod, IC= (extra branch-current unknowns),
  * homotopy (artParameterFlag), lead currents.  Instances using them are
  * rejected at setup.
  */
@@ -586,8 +586,452 @@ static void b3_junction(ref_b3_inst *in, double v, double czb, double czbsw,
   }
 }
 
-/* Instance::updateIntermediateVars, N_DEV_MOSFET_B3.C:4526-7273, capMod 3,
-   nqsMod 0, no homotopy.  Returns 0, or -1 for a model this restatement
+/* Channel charge and its capacitances for capMod 0 (N_DEV_MOSFET_B3.C:
+   6005-6308): the long-channel model on the flat-band voltage vfbcv.  The
+   body bias is Vbs when Vbseff < 0, else phi - Phis.  Accumulation
+   (Arg1 <= 0) gives qgate = CoxWL*Arg1; depletion (Vgst <= 0) the
+   square-root body charge; inversion the saturation/linear forms of the
+   0/100 (xpart > 0.5), 40/60 (xpart < 0.5) or 50/50 drain/source partition.
+   qsrc is left alone here: the companion step rebuilds it from the others. */
+static void b3_charge_cap0(ref_b3_model *m, ref_b3_sdp *p, ref_b3_inst *in,
+                           double Vbs, double Vbseff, double Phis,
+                           double dPhis_dVb, double sqrtPhis,
+                           double dsqrtPhis_dVb, double Vgs_eff,
+                           double dVgs_eff_dVg, double Vds, double Abulk0,
+                           double dAbulk0_dVb) {
+  double vb, dvb_dVb;
+  if (Vbseff < 0.0) {
+    vb = Vbs;
+    dvb_dVb = 1.0;
+  } else {
+    vb = p->phi - Phis;
+    dvb_dVb = -dPhis_dVb;
+  }
+  const double Vfb = p->vfbcv;
+  const double Vth = Vfb + p->phi + p->k1ox * sqrtPhis;
+  const double Vgst = Vgs_eff - Vth;
+  const double dVth_dVb = p->k1ox * dsqrtPhis_dVb;
+  const double CoxWL = m->cox * p->weffCV * p->leffCV;
+  const double Arg1 = Vgs_eff - vb - Vfb;
+
+  in->cgdb = in->cdgb = in->cddb = in->cdsb = in->cbdb = 0.0;
+  if (Arg1 <= 0.0) {
+    in->qgate = CoxWL * Arg1;
+    in->qbulk = -in->qgate;
+    in->qdrn = 0.0;
+    in->cggb = CoxWL * dVgs_eff_dVg;
+    in->cgsb = CoxWL * (dvb_dVb - dVgs_eff_dVg);
+    in->cbgb = -CoxWL * dVgs_eff_dVg;
+    in->cbsb = -in->cgsb;
+    in->qinv = 0.0;
+    return;
+  }
+  if (Vgst <= 0.0) {
+    const double h = 0.5 * p->k1ox;
+    const double r = sqrt(h * h + Arg1);
+    in->qgate = CoxWL * p->k1ox * (r - h);
+    in->qbulk = -in->qgate;
+    in->qdrn = 0.0;
+    const double c = CoxWL * h / r;
+    in->cggb = c * dVgs_eff_dVg;
+    in->cgsb = c * (dvb_dVb - dVgs_eff_dVg);
+    in->cbgb = -in->cggb;
+    in->cbsb = -in->cgsb;
+    in->qinv = 0.0;
+    return;
+  }
+
+  const double c3 = CoxWL / 3.0, c23 = 2.0 * c3;
+  const double AbulkCV = Abulk0 * p->abulkCVfactor;
+  const double dAbulkCV_dVb = p->abulkCVfactor * dAbulk0_dVb;
+  const double Vdsat = Vgst / AbulkCV;
+  const double dVdsat_dVg = dVgs_eff_dVg / AbulkCV;
+  const double dVdsat_dVb = -(Vdsat * dAbulkCV_dVb + dVth_dVb) / AbulkCV;
+  const int part = (m->xpart > 0.5) ? 0 : (m->xpart < 0.5) ? 1 : 2;
+
+  if (Vdsat <= Vds) {
+    /* saturation: the gate and bulk charges do not depend on the split */
+    in->qgate = CoxWL * (Vgs_eff - Vfb - p->phi - Vdsat / 3.0);
+    const double qi = -c23 * Vgst;
+    in->qbulk = -(in->qgate + qi);
+    in->cggb = c3 * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+    const double gb = -c3 * dVdsat_dVb;
+    in->cgsb = -(in->cggb + gb);
+    in->cbgb = -(in->cggb - c23 * dVgs_eff_dVg);
+    const double bb = -(gb + c23 * dVth_dVb);
+    in->cbsb = -(in->cbgb + bb);
+    if (part == 0) {
+      in->qdrn = 0.0;
+    } else if (part == 1) {
+      in->qdrn = 0.4 * qi;
+      const double k = 0.4 * c23;
+      in->cdgb = -k * dVgs_eff_dVg;
+      in->cdsb = -(k * dVth_dVb + in->cdgb);
+    } else {
+      in->qdrn = 0.5 * qi;
+      in->cdgb = -c3 * dVgs_eff_dVg;
+      in->cdsb = -(c3 * dVth_dVb + in->cdgb);
+    }
+    in->qinv = -(in->qgate + in->qbulk);
+    return;
+  }
+
+  /* linear region */
+  const double Alphaz = Vgst / Vdsat;
+  const double T1 = 2.0 * Vdsat - Vds;
+  const double T2 = Vds / (3.0 * T1);
+  const double T3 = T2 * Vds;
+  const double q4 = 0.25 * CoxWL;
+  const double T4 = q4 * Alphaz;
+  in->qgate = CoxWL * (Vgs_eff - Vfb - p->phi - 0.5 * (Vds - T3));
+  const double T5 = T3 / T1;
+  in->cggb = CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+  const double gb = -CoxWL * T5 * dVdsat_dVb;
+  in->cgdb = CoxWL * (T2 - 0.5 + 0.5 * T5);
+  if (part == 0) in->cgsb = -(in->cggb + gb + in->cgdb);
+  else in->cgsb = -(in->cggb + in->cgdb + gb);
+  const double rV = 1.0 / Vdsat;
+  const double dAz_dVg = rV * (1.0 - Alphaz * dVdsat_dVg);
+  const double dAz_dVb = -rV * (dVth_dVb + Alphaz * dVdsat_dVb);
+
+  if (part == 0) {
+    const double a = 2.0 * Vds - T1 - 3.0 * T3;
+    const double b = T3 - T1 - 2.0 * Vds;
+    in->qdrn = T4 * a;
+    in->qbulk = -(in->qgate + in->qdrn + T4 * b);
+    const double qa = q4 * a, qb = q4 * b;
+    double k = 2.0 * T4 * (1.0 - 3.0 * T5);
+    in->cdgb = (qa * dAz_dVg - k * dVdsat_dVg) * dVgs_eff_dVg;
+    const double db = qa * dAz_dVb - k * dVdsat_dVb;
+    in->cddb = T4 * (3.0 - 6.0 * T2 - 3.0 * T5);
+    in->cdsb = -(in->cdgb + db + in->cddb);
+    k = 2.0 * T4 * (1.0 + T5);
+    const double bg = (qb * dAz_dVg - k * dVdsat_dVg) * dVgs_eff_dVg;
+    const double bbv = qb * dAz_dVb - k * dVdsat_dVb;
+    const double bd = T4 * (2.0 * T2 + T5 - 1.0);
+    const double bs = -(bg + bbv + bd);
+    in->cbgb = -(in->cggb + in->cdgb + bg);
+    in->cbdb = -(in->cgdb + in->cddb + bd);
+    in->cbsb = -(in->cgsb + in->cdsb + bs);
+  } else if (part == 1) {
+    const double w = 8.0 * Vdsat * Vdsat - 6.0 * Vdsat * Vds + 1.2 * Vds * Vds;
+    const double u = T2 / T1;
+    double a = Vds - T1 - u * w;
+    in->qdrn = T4 * a;
+    a *= q4;
+    const double v = u / T1;
+    const double k = T4 * (2.0 - 4.0 * v * w + u * (16.0 * Vdsat - 6.0 * Vds));
+    in->cdgb = (a * dAz_dVg - k * dVdsat_dVg) * dVgs_eff_dVg;
+    const double db = a * dAz_dVb - k * dVdsat_dVb;
+    in->cddb = T4 * (2.0 - (1.0 / (3.0 * T1 * T1) + 2.0 * v) * w +
+                     u * (6.0 * Vdsat - 2.4 * Vds));
+    in->cdsb = -(in->cdgb + db + in->cddb);
+    double e = 2.0 * (T1 + T3);
+    in->qbulk = -(in->qgate - T4 * e);
+    e *= q4;
+    const double f = 4.0 * T4 * (1.0 - T5);
+    const double bg = (-e * dAz_dVg - in->cdgb - f * dVdsat_dVg) * dVgs_eff_dVg;
+    const double bbv = -e * dAz_dVb - db - f * dVdsat_dVb;
+    const double bd = -4.0 * T4 * (T2 - 0.5 + 0.5 * T5) - in->cddb;
+    const double bs = -(bd + bbv + bg);
+    in->cbgb = -(in->cggb + in->cdgb + bg);
+    /* the Vb term, as N_DEV_MOSFET_B3.C:6238 has it */
+    in->cbdb = -(in->cgdb + in->cddb + bbv);
+    in->cbsb = -(in->cgsb + in->cdsb + bs);
+  } else {
+    double a = T1 + T3;
+    in->qdrn = -T4 * a;
+    in->qbulk = -(in->qgate + in->qdrn + in->qdrn);
+    a *= q4;
+    const double k = T4 * (2.0 * T5 - 2.0);
+    in->cdgb = (k * dVdsat_dVg - a * dAz_dVg) * dVgs_eff_dVg;
+    const double db = k * dVdsat_dVb - a * dAz_dVb;
+    in->cddb = T4 * (1.0 - 2.0 * T2 - T5);
+    in->cdsb = -(in->cdgb + db + in->cddb);
+    in->cbgb = -(in->cggb + 2.0 * in->cdgb);
+    in->cbdb = -(in->cgdb + 2.0 * in->cddb);
+    in->cbsb = -(in->cgsb + 2.0 * in->cdsb);
+  }
+  in->qinv = -(in->qgate + in->qbulk);
+}
+
+/* Channel charge for capMod 1 (N_DEV_MOSFET_B3.C:6360-6520) and capMod 2
+   (6521-6685), on the smoothed Vgsteff of the capMod > 0 preamble and the
+   flat-band voltage vfbzb.  capMod 1 is piecewise: a body charge from
+   Arg1 = Vgs - Vbs - Vfb - Vgsteff (accumulation when <= 0), then the
+   inversion charge in saturation (VdsatCV < Vds) or the linear form.
+   capMod 2 is smooth: an accumulation charge Qac0 on the smoothed
+   Vfbeff, a body charge Qsub0, and the inversion charge on the smoothed
+   VdseffCV.  xpart selects the 0/100, 40/60 or 50/50 source share. */
+static void b3_charge_cap12(ref_b3_model *m, ref_b3_sdp *p, ref_b3_inst *in,
+                            int capmod, double VbseffCV, double dVbseffCV_dVb,
+                            double dVbseff_dVb, double CoxWL, double Vgs_eff,
+                            double dVgs_eff_dVg, double Vds, double Abulk0,
+                            double dAbulk0_dVb, double Vgsteff,
+                            double dVgsteff_dVg, double dVgsteff_dVd,
+                            double dVgsteff_dVb) {
+  const double k1ox = p->k1ox;
+  const double AbulkCV = Abulk0 * p->abulkCVfactor;
+  const double dAbulkCV_dVb = p->abulkCVfactor * dAbulk0_dVb;
+  const double VdsatCV = Vgsteff / AbulkCV;
+  const int part = (m->xpart > 0.5) ? 0 : (m->xpart < 0.5) ? 1 : 2;
+  double qgate, qbulk, qsrc;
+  double Cgg, Cgd, Cgb, Cbg, Cbd, Cbb, Csg, Csd, Csb;
+  double Cgg1, Cgd1, Cgb1, Cbg1, Cbd1, Cbb1;
+  double T0, T1, T2, T3, T4, T5, T6, T7;
+
+  if (capmod == 1) {
+    const double Arg1 = Vgs_eff - VbseffCV - p->vfbzb - Vgsteff;
+    if (Arg1 <= 0.0) {
+      qgate = CoxWL * Arg1;
+      Cgg = CoxWL * (dVgs_eff_dVg - dVgsteff_dVg);
+      Cgd = -CoxWL * dVgsteff_dVd;
+      Cgb = -CoxWL * (dVbseffCV_dVb + dVgsteff_dVb);
+    } else {
+      T0 = 0.5 * k1ox;
+      T1 = sqrt(T0 * T0 + Arg1);
+      T2 = CoxWL * T0 / T1;
+      qgate = CoxWL * k1ox * (T1 - T0);
+      Cgg = T2 * (dVgs_eff_dVg - dVgsteff_dVg);
+      Cgd = -T2 * dVgsteff_dVd;
+      Cgb = -T2 * (dVbseffCV_dVb + dVgsteff_dVb);
+    }
+    qbulk = -qgate;
+    Cbg = -Cgg;
+    Cbd = -Cgd;
+    Cbb = -Cgb;
+    const double c3 = CoxWL / 3.0, c23 = 2.0 * c3;
+    if (VdsatCV < Vds) {
+      const double dVdsatCV_dVg = 1.0 / AbulkCV;
+      const double dVdsatCV_dVb = -VdsatCV * dAbulkCV_dVb / AbulkCV;
+      T0 = Vgsteff - VdsatCV / 3.0;
+      double dg = 1.0 - dVdsatCV_dVg / 3.0, db = -dVdsatCV_dVb / 3.0;
+      qgate += CoxWL * T0;
+      Cgg1 = CoxWL * dg;
+      Cgb1 = CoxWL * db + Cgg1 * dVgsteff_dVb;
+      Cgd1 = Cgg1 * dVgsteff_dVd;
+      Cgg1 *= dVgsteff_dVg;
+      Cgg += Cgg1;
+      Cgb += Cgb1;
+      Cgd += Cgd1;
+      T0 = VdsatCV - Vgsteff;
+      dg = dVdsatCV_dVg - 1.0;
+      db = dVdsatCV_dVb;
+      qbulk += c3 * T0;
+      Cbg1 = c3 * dg;
+      Cbb1 = c3 * db + Cbg1 * dVgsteff_dVb;
+      Cbd1 = Cbg1 * dVgsteff_dVd;
+      Cbg1 *= dVgsteff_dVg;
+      Cbg += Cbg1;
+      Cbb += Cbb1;
+      Cbd += Cbd1;
+      T0 = (part == 0) ? -c23 : (part == 1) ? -0.4 * CoxWL : -c3;
+      qsrc = T0 * Vgsteff;
+      Csg = T0 * dVgsteff_dVg;
+      Csb = T0 * dVgsteff_dVb;
+      Csd = T0 * dVgsteff_dVd;
+    } else {
+      T0 = AbulkCV * Vds;
+      T1 = 12.0 * (Vgsteff - 0.5 * T0 + 1.e-20);
+      T2 = Vds / T1;
+      T3 = T0 * T2;
+      const double dT3_dVg = -12.0 * T2 * T2 * AbulkCV;
+      const double dT3_dVd = 6.0 * T0 * (4.0 * Vgsteff - T0) / T1 / T1 - 0.5;
+      const double dT3_dVb = 12.0 * T2 * T2 * dAbulkCV_dVb * Vgsteff;
+      qgate += CoxWL * (Vgsteff - 0.5 * Vds + T3);
+      Cgg1 = CoxWL * (1.0 + dT3_dVg);
+      Cgb1 = CoxWL * dT3_dVb + Cgg1 * dVgsteff_dVb;
+      Cgd1 = CoxWL * dT3_dVd + Cgg1 * dVgsteff_dVd;
+      Cgg1 *= dVgsteff_dVg;
+      Cgg += Cgg1;
+      Cgb += Cgb1;
+      Cgd += Cgd1;
+      qbulk += CoxWL * (1.0 - AbulkCV) * (0.5 * Vds - T3);
+      Cbg1 = -CoxWL * ((1.0 - AbulkCV) * dT3_dVg);
+      Cbb1 = -CoxWL * ((1.0 - AbulkCV) * dT3_dVb + (0.5 * Vds - T3) * dAbulkCV_dVb) +
+             Cbg1 * dVgsteff_dVb;
+      Cbd1 = -CoxWL * (1.0 - AbulkCV) * dT3_dVd + Cbg1 * dVgsteff_dVd;
+      Cbg1 *= dVgsteff_dVg;
+      Cbg += Cbg1;
+      Cbb += Cbb1;
+      Cbd += Cbd1;
+      if (part == 0) {
+        T1 = T1 + T1;
+        qsrc = -CoxWL * (0.5 * Vgsteff + 0.25 * T0 - T0 * T0 / T1);
+        Csg = -CoxWL * (0.5 + 24.0 * T0 * Vds / T1 / T1 * AbulkCV);
+        Csb = -CoxWL * (0.25 * Vds * dAbulkCV_dVb -
+                        12.0 * T0 * Vds / T1 / T1 * (4.0 * Vgsteff - T0) * dAbulkCV_dVb) +
+              Csg * dVgsteff_dVb;
+        Csd = -CoxWL * (0.25 * AbulkCV - 12.0 * AbulkCV * T0 / T1 / T1 * (4.0 * Vgsteff - T0)) +
+              Csg * dVgsteff_dVd;
+        Csg *= dVgsteff_dVg;
+      } else if (part == 1) {
+        T1 = T1 / 12.0;
+        T2 = 0.5 * CoxWL / (T1 * T1);
+        T3 = Vgsteff * (2.0 * T0 * T0 / 3.0 + Vgsteff * (Vgsteff - 4.0 * T0 / 3.0)) -
+             2.0 * T0 * T0 * T0 / 15.0;
+        qsrc = -T2 * T3;
+        T4 = 4.0 / 3.0 * Vgsteff * (Vgsteff - T0) + 0.4 * T0 * T0;
+        Csg = -2.0 * qsrc / T1 -
+              T2 * (Vgsteff * (3.0 * Vgsteff - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
+        Csb = (qsrc / T1 * Vds + T2 * T4 * Vds) * dAbulkCV_dVb + Csg * dVgsteff_dVb;
+        Csd = (qsrc / T1 + T2 * T4) * AbulkCV + Csg * dVgsteff_dVd;
+        Csg *= dVgsteff_dVg;
+      } else {
+        qsrc = -0.5 * (qgate + qbulk);
+        Csg = -0.5 * (Cgg1 + Cbg1);
+        Csb = -0.5 * (Cgb1 + Cbb1);
+        Csd = -0.5 * (Cgd1 + Cbd1);
+      }
+    }
+    Cgb *= dVbseff_dVb;
+    Cbb *= dVbseff_dVb;
+    Csb *= dVbseff_dVb;
+    in->qinv = -(qgate + qbulk);
+  } else {
+    const double vfbzb = p->vfbzb;
+    const double V3 = vfbzb - Vgs_eff + VbseffCV - RC_DELTA_3;
+    if (vfbzb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * RC_DELTA_3 * vfbzb);
+    else T0 = sqrt(V3 * V3 + 4.0 * RC_DELTA_3 * vfbzb);
+    T1 = 0.5 * (1.0 + V3 / T0);
+    const double Vfbeff = vfbzb - 0.5 * (V3 + T0);
+    const double dVfbeff_dVg = T1 * dVgs_eff_dVg;
+    const double dVfbeff_dVb = -T1 * dVbseffCV_dVb;
+    const double Qac0 = CoxWL * (Vfbeff - vfbzb);
+    const double dQac0_dVg = CoxWL * dVfbeff_dVg;
+    const double dQac0_dVb = CoxWL * dVfbeff_dVb;
+    T0 = 0.5 * k1ox;
+    T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteff;
+    if (k1ox == 0.0) {
+      T1 = 0.0;
+      T2 = 0.0;
+    } else if (T3 < 0.0) {
+      T1 = T0 + T3 / k1ox;
+      T2 = CoxWL;
+    } else {
+      T1 = sqrt(T0 * T0 + T3);
+      T2 = CoxWL * T0 / T1;
+    }
+    const double Qsub0 = CoxWL * k1ox * (T1 - T0);
+    const double dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg);
+    const double dQsub0_dVd = -T2 * dVgsteff_dVd;
+    const double dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteff_dVb);
+
+    const double V4 = VdsatCV - Vds - RC_DELTA_4;
+    T0 = sqrt(V4 * V4 + 4.0 * RC_DELTA_4 * VdsatCV);
+    double VdseffCV = VdsatCV - 0.5 * (V4 + T0);
+    T1 = 0.5 * (1.0 + V4 / T0);
+    T2 = RC_DELTA_4 / T0;
+    T3 = (1.0 - T1 - T2) / AbulkCV;
+    double dVdseffCV_dVg = T3;
+    const double dVdseffCV_dVd = T1;
+    double dVdseffCV_dVb = -T3 * VdsatCV * dAbulkCV_dVb;
+    if (Vds == 0.0) {
+      VdseffCV = 0.0;
+      dVdseffCV_dVg = 0.0;
+      dVdseffCV_dVb = 0.0;
+    }
+    T0 = AbulkCV * VdseffCV;
+    T1 = 12.0 * (Vgsteff - 0.5 * T0 + 1e-20);
+    T2 = VdseffCV / T1;
+    T3 = T0 * T2;
+    T4 = (1.0 - 12.0 * T2 * T2 * AbulkCV);
+    T5 = (6.0 * T0 * (4.0 * Vgsteff - T0) / (T1 * T1) - 0.5);
+    T6 = 12.0 * T2 * T2 * Vgsteff;
+    in->qinv = -CoxWL * (Vgsteff - 0.5 * T0 + AbulkCV * T3);
+    qgate = CoxWL * (Vgsteff - 0.5 * VdseffCV + T3);
+    Cgg1 = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cgd1 = CoxWL * T5 * dVdseffCV_dVd + Cgg1 * dVgsteff_dVd;
+    Cgb1 = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cgg1 * dVgsteff_dVb;
+    Cgg1 *= dVgsteff_dVg;
+    T7 = 1.0 - AbulkCV;
+    qbulk = CoxWL * T7 * (0.5 * VdseffCV - T3);
+    T4 = -T7 * (T4 - 1.0);
+    T5 = -T7 * T5;
+    T6 = -(T7 * T6 + (0.5 * VdseffCV - T3));
+    Cbg1 = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cbd1 = CoxWL * T5 * dVdseffCV_dVd + Cbg1 * dVgsteff_dVd;
+    Cbb1 = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cbg1 * dVgsteff_dVb;
+    Cbg1 *= dVgsteff_dVg;
+    if (part == 0) {
+      T1 = T1 + T1;
+      qsrc = -CoxWL * (0.5 * Vgsteff + 0.25 * T0 - T0 * T0 / T1);
+      T7 = (4.0 * Vgsteff - T0) / (T1 * T1);
+      T4 = -(0.5 + 24.0 * T0 * T0 / (T1 * T1));
+      T5 = -(0.25 * AbulkCV - 12.0 * AbulkCV * T0 * T7);
+      T6 = -(0.25 * VdseffCV - 12.0 * T0 * VdseffCV * T7);
+      Csg = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+      Csd = CoxWL * T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else if (part == 1) {
+      T1 = T1 / 12.0;
+      T2 = 0.5 * CoxWL / (T1 * T1);
+      T3 = Vgsteff * (2.0 * T0 * T0 / 3.0 + Vgsteff * (Vgsteff - 4.0 * T0 / 3.0)) -
+           2.0 * T0 * T0 * T0 / 15.0;
+      qsrc = -T2 * T3;
+      T7 = 4.0 / 3.0 * Vgsteff * (Vgsteff - T0) + 0.4 * T0 * T0;
+      T4 = -2.0 * qsrc / T1 -
+           T2 * (Vgsteff * (3.0 * Vgsteff - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
+      T5 = (qsrc / T1 + T2 * T7) * AbulkCV;
+      T6 = (qsrc / T1 * VdseffCV + T2 * T7 * VdseffCV);
+      Csg = (T4 + T5 * dVdseffCV_dVg);
+      Csd = T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else {
+      qsrc = -0.5 * (qgate + qbulk);
+      Csg = -0.5 * (Cgg1 + Cbg1);
+      Csb = -0.5 * (Cgb1 + Cbb1);
+      Csd = -0.5 * (Cgd1 + Cbd1);
+    }
+    qgate += Qac0 + Qsub0;
+    qbulk -= (Qac0 + Qsub0);
+    Cgg = dQac0_dVg + dQsub0_dVg + Cgg1;
+    Cgd = dQsub0_dVd + Cgd1;
+    Cgb = dQac0_dVb + dQsub0_dVb + Cgb1;
+    Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
+    Cbd = Cbd1 - dQsub0_dVd;
+    Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
+    Cgb *= dVbseff_dVb;
+    Cbb *= dVbseff_dVb;
+    Csb *= dVbseff_dVb;
+  }
+  in->qgate = qgate;
+  in->qbulk = qbulk;
+  in->qsrc = qsrc;
+  in->qdrn = -(qgate + qbulk + qsrc);
+  in->cggb = Cgg;
+  in->cgsb = -(Cgg + Cgd + Cgb);
+  in->cgdb = Cgd;
+  in->cdgb = -(Cgg + Cbg + Csg);
+  in->cdsb = (Cgg + Cgd + Cgb + Cbg + Cbd + Cbb + Csg + Csd + Csb);
+  in->cddb = -(Cgd + Cbd + Csd);
+  in->cbgb = Cbg;
+  in->cbsb = -(Cbg + Cbd + Cbb);
+  in->cbdb = Cbd;
+}
+
+/* Gate overlap charge and capacitance of one side for capMod 0 (constant
+   cgxo, N_DEV_MOSFET_B3.C:7164-7187) and capMod 1 (7188-7215: the bias-
+   dependent overlap cgxl over weffCV, sqrt form for v < 0, linear above). */
+static void b3_overlap_cap01(ref_b3_model *m, ref_b3_sdp *p, double v,
+                             double cgxo, double cgxl, double *c, double *q) {
+  if (m->capMod == 0) {
+    *c = cgxo;
+    *q = cgxo * v;
+  } else if (v < 0.0) {
+    const double r = sqrt(1.0 - 4.0 * v / p->ckappa);
+    *c = cgxo + p->weffCV * cgxl / r;
+    *q = cgxo * v - p->weffCV * 0.5 * cgxl * p->ckappa * (r - 1.0);
+  } else {
+    *c = cgxo + p->weffCV * cgxl;
+    *q = (p->weffCV * cgxl + cgxo) * v;
+  }
+}
+
+/* Instance::updateIntermediateVars, N_DEV_MOSFET_B3.C:4526-7273, capMod
+   0-3, nqsMod 0, no homotopy.  Returns 0, or -1 for a model this restatement
    does not cover. */
 static int b3_uiv(ref_b3_master *M, ref_b3_inst *in) {
   ref_b3_model *m = in->model;
@@ -636,7 +1080,8 @@ static int b3_uiv(ref_b3_master *M, ref_b3_inst *in) {
   int Check = 1;
   (void)MJ; (void)MJSW; (void)MJSWG; (void)arg; (void)sarg;
 
-  if (m->capMod != 3) return -1;
+  if (m->capMod != 0 && m->capMod != 1 && m->capMod != 2 && m->capMod != 3)
+    return -1;
 
   in->ChargeComputationNeeded = (ss->tranopFlag || ss->acopFlag || ss->transientFlag) ? 1 : 0;
 
@@ -1344,6 +1789,11 @@ static int b3_uiv(ref_b3_master *M, ref_b3_inst *in) {
     in->gtau = 0.0;
     goto finished;
   }
+  if (m->capMod == 0) {
+    b3_charge_cap0(m, p, in, Vbs, Vbseff, Phis, dPhis_dVb, sqrtPhis,
+                   dsqrtPhis_dVb, Vgs_eff, dVgs_eff_dVg, Vds, Abulk0, dAbulk0_dVb);
+    goto finished;
+  }
   /* capMod > 0 preamble (N_DEV_MOSFET_B3.C:6310-6358) */
   if (Vbseff < 0.0) {
     VbseffCV = Vbseff;
@@ -1377,6 +1827,13 @@ static int b3_uiv(ref_b3_master *M, ref_b3_inst *in) {
     dVgsteff_dVd = -dVgsteff_dVg * (dVth_dVd + (Vgst - voffcv) / noff * dnoff_dVd) + Vgsteff / noff * dnoff_dVd;
     dVgsteff_dVb = -dVgsteff_dVg * (dVth_dVb + (Vgst - voffcv) / noff * dnoff_dVb) + Vgsteff / noff * dnoff_dVb;
     dVgsteff_dVg *= dVgs_eff_dVg;
+  }
+  if (m->capMod == 1 || m->capMod == 2) {
+    b3_charge_cap12(m, p, in, (int)m->capMod, VbseffCV, dVbseffCV_dVb,
+                    dVbseff_dVb, CoxWL, Vgs_eff, dVgs_eff_dVg, Vds, Abulk0,
+                    dAbulk0_dVb, Vgsteff, dVgsteff_dVg, dVgsteff_dVd,
+                    dVgsteff_dVb);
+    goto finished;
   }
   /* capMod 3 (charge-thickness model), N_DEV_MOSFET_B3.C:6688-6959 */
   V3 = p->vfbzb - Vgs_eff + VbseffCV - RC_DELTA_3;
@@ -1609,7 +2066,12 @@ finished:
     b3_junction(in, in->vbs, czbs, czbssw, czbsswg, &in->qbs, &in->capbs);
     b3_junction(in, in->vbd, czbd, czbdsw, czbdswg, &in->qbd, &in->capbd);
 
-    /* overlap capacitances, capMod 3 branch (N_DEV_MOSFET_B3.C:7217-7238) */
+    /* overlap capacitances (N_DEV_MOSFET_B3.C:7164-7238) */
+    if (m->capMod == 0 || m->capMod == 1) {
+      b3_overlap_cap01(m, p, in->vgd, p->cgdo, p->cgdl, &in->cgdo, &in->qgdo);
+      b3_overlap_cap01(m, p, in->vgs, p->cgso, p->cgsl, &in->cgso, &in->qgso);
+      goto overlaps_done;
+    }
     T0 = in->vgd + RC_DELTA_1;
     T1 = sqrt(T0 * T0 + 4.0 * RC_DELTA_1);
     T2 = 0.5 * (T0 - T1);
@@ -1624,6 +2086,7 @@ finished:
     T4 = sqrt(1.0 - 4.0 * T2 / p->ckappa);
     in->cgso = p->cgso + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
     in->qgso = (p->cgso + T3) * in->vgs - T3 * (T2 + 0.5 * p->ckappa * (T4 - 1.0));
+  overlaps_done:
 
     /* setupCapacitors_oldDAE, charge part (N_DEV_MOSFET_B3.C:7632-7800) */
     (void)ag0;

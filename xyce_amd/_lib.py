@@ -60,6 +60,9 @@ SIGNATURES = [
     ("xg_load_vectors", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     ("xg_load_matrices", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 2),
     ("xg_load_all", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 11),
+    ("xg_update_state_host", ctypes.c_int, [ctypes.c_void_p] * 6),
+    ("xg_load_vectors_host", ctypes.c_int, [ctypes.c_void_p] * 5),
+    ("xg_load_matrices_host", ctypes.c_int, [ctypes.c_void_p] * 3),
     ("xg_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("xg_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -92,6 +95,9 @@ SIGNATURES = [
     ("xg_lu_solve", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 +
      [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_lu_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_lu_front_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_lu_front_export", ctypes.c_int, [ctypes.c_void_p] * 4),
+    ("xg_circuit_set_lu_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_lu_export", ctypes.c_int, [ctypes.c_void_p] * 4),
     ("xg_set_step_source", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_set_active_mask", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -106,11 +112,13 @@ SIGNATURES = [
     ("xg_tran_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_tran_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_tran_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("xg_tran_counters", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_tran_restore", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64,
                                        ctypes.c_int64, ctypes.c_int64]),
     ("xg_instance_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_void_p]),
     ("xg_tran_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("xg_autotune_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_set_eval_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_set_output_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),

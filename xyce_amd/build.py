@@ -25,6 +25,7 @@ SOURCES = [
     "xg_linear.hip",
     "xg_tia.hip",
     "xg_lu.hip",
+    "xg_front.hip",
     "xg_tran.hip",
     "xg_lu_setup.cpp",
     "xg_setup.cpp",

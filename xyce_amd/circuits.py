@@ -345,14 +345,16 @@ def _diffusion(w, ext=0.3e-6):
     return dict(AD=w * ext, AS=w * ext, PD=2 * (w + ext), PS=2 * (w + ext))
 
 
-def bsim3_variety(seed=0, n_nodes=30, n_dev=128):
+def bsim3_variety(seed=0, n_nodes=30, n_dev=128, capmod=None):
     """Parity workload covering the BSIM3 options a card can set: mobMod
     1/2/3, xpart 0 / 0.4 / 0.5 / 1 (the three charge partitions), RSH sheet
     resistance (internal d'/s'), IJTH junction linearisation, NGATE
     poly depletion, K1/K2 given vs. derived from NCH/GAMMA1/GAMMA2/VBX,
     VFB given, length/width binning (L*, W*, P* coefficients), DWG/DWB,
     TNOM and instance TEMP away from nominal, diffusion perimeters smaller
-    than Weff, OFF, M=, both polarities and grounded terminals."""
+    than Weff, OFF, M=, both polarities and grounded terminals.  With
+    capmod given, every card uses that charge model (VFBCV set per card
+    for capMod 0)."""
     rng = np.random.default_rng(seed)
     nl = Netlist("BSIM3 variety")
     nl.add_model("N1", "bsim3", 1, **BSIM3_NMOS_130)
@@ -366,6 +368,11 @@ def bsim3_variety(seed=0, n_nodes=30, n_dev=128):
     n4 = {k: v for k, v in BSIM3_NMOS_130.items() if k not in ("K1", "K2", "VTH0")}
     n4.update(GAMMA1=0.6, GAMMA2=0.45, VBX=-1.5, XPART=0.4, CGSO=0.0, CGDO=0.0)
     nl.add_model("N4", "bsim3", 1, **n4)
+    if capmod is not None:
+        for k, name in enumerate(nl.models):
+            nl.models[name].params["CAPMOD"] = float(capmod)
+            if capmod == 0:
+                nl.models[name].params["VFBCV"] = (-1.0, -0.8, 0.3, -0.95)[k % 4]
     names = list(nl.models)
     nodes = ["n%d" % k for k in range(n_nodes)] + ["0"]
     for i in range(n_dev):

@@ -6,7 +6,10 @@
 #include <memory>
 #include <new>
 #include <stdexcept>
+#include <cmath>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 
 #include "xg_bsim3.h"
@@ -43,6 +46,8 @@ struct Handle {
   // per-component dense solve, and the static-pivot sparse LU
   DenseSolveArgs ds{};
   SparseLuArgs lu{};
+  FrontArgs fr{};
+  std::vector<int32_t> fr_lv_ptr, fr_lv_maxm;  // host copies for the launches
   int32_t *d_rowptr = nullptr, *d_colidx = nullptr;
   int32_t *slot_comp = nullptr;
   int32_t *node_comp = nullptr;  // node -> component (ground excluded)
@@ -53,10 +58,23 @@ struct Handle {
   int64_t *count_buf = nullptr;
   int32_t *limited_all = nullptr;  // m1 then b3 eval slots
   int32_t n_limited_slots = 0;
-  // host staging for the *_host variants
+  // the *_host variants: device mirrors of the caller's host vectors and
+  // one pinned staging buffer, allocated by their first call
   double *pinned = nullptr;
+  double *hx = nullptr, *hsta_c = nullptr, *hsto_c = nullptr, *hsta_n = nullptr,
+         *hsto_n = nullptr, *hvec = nullptr, *hmat = nullptr;
+  int64_t pinned_len = 0;
   bool plan_only = false;
   bool overwrite = false;  // XG_OUT_OVERWRITE
+  // stop word of the transient driver's multi-iteration graphs: set only
+  // while the driver enqueues (its launches then skip once it is nonzero)
+  const int32_t *gate = nullptr;
+  // autotuned launch choices (result-neutral): linear devices in trailing
+  // workgroups of the BSIM3 launch or their own launch
+  bool fuse_linear = true;
+  int32_t tune_source = 0;   // 0 not tuned, 1 timed at upload, 2 cached
+  double tune_us[6] = {};    // per candidate, NaN where not timed
+  int device = -1;         // the GPU the handle's buffers live on
 
   template <class T>
   T *alloc(size_t n) {
@@ -120,6 +138,13 @@ int xg_circuit_new(xg_circuit **c, int32_t n_nodes, int32_t n_state,
 }
 
 void xg_circuit_free(xg_circuit *c) { delete c; }
+
+int xg_circuit_set_lu_mode(xg_circuit *c, int32_t mode) {
+  if (!c || (mode != XG_LU_AUTO && mode != XG_LU_FRONTS))
+    return fail(XG_E_ARG, "xg_circuit_set_lu_mode: bad argument");
+  c->c.opt.lu_fronts = mode == XG_LU_FRONTS ? 1 : 0;
+  return XG_OK;
+}
 
 int xg_circuit_set_options(xg_circuit *c, double gmin, int32_t limiter,
                            double temp, double tnom) {
@@ -350,6 +375,7 @@ static int tia_assemble(xg_handle *H, int32_t transient, int32_t order, double d
   a.limiter = h.plan.limiter;
   a.inv_dt = transient ? 1.0 / dt : 0.0;
   a.dstep = h.dstep;
+  a.gate = h.gate;
   a.qn0 = qn0;
   a.fprev = fprev ? fprev : F;
   a.F = F;
@@ -415,8 +441,18 @@ int xg_lu_solve(xg_handle *H, const double *J, const double *rhs, double *x,
   u.abstol = abstol;
   u.active = active;
   u.stats = stats;
+  u.gate = h.gate;
   hipError_t e = xg::launch_sparse_lu(u, h.stream);
   if (e != hipSuccess) return hip_fail(e, "sparse LU");
+  if (h.fr.n_fronts > 0) {
+    xg::FrontArgs f = h.fr;
+    f.active = active;
+    f.gate = h.gate;
+    f.lv_ptr_host = h.fr_lv_ptr.data();
+    f.lv_maxm_host = h.fr_lv_maxm.data();
+    e = xg::launch_fronts(f, u, h.stream);
+    if (e != hipSuccess) return hip_fail(e, "front LU");
+  }
   // components whose static pivots fail the threshold test: dense LU
   // with partial pivoting, in the same stream, no host round trip
   xg::DenseSolveArgs a = h.ds;
@@ -430,6 +466,7 @@ int xg_lu_solve(xg_handle *H, const double *J, const double *rhs, double *x,
   a.active = h.lu.fallback;
   a.only_flagged = 1;
   a.stats = stats;
+  a.gate = h.gate;
   e = xg::launch_dense_solve(a, h.stream);
   return e == hipSuccess ? XG_OK : hip_fail(e, "pivoting LU");
 }
@@ -460,6 +497,49 @@ int xg_lu_info(xg_handle *H, int64_t *info) {
     info[6] = 0;
   }
   info[7] = (int64_t)L.t_dst.size();
+  return XG_OK;
+}
+
+int xg_lu_front_info(const xg_handle *H, int64_t *info) {
+  if (!H || !info) return fail(XG_E_ARG, "xg_lu_front_info: bad argument");
+  const xg::LuPlan &L = H->h.plan.lu;
+  info[0] = L.n_front_comp;
+  info[1] = (int64_t)L.fr_m.size();
+  info[2] = L.fr_lv_ptr.empty() ? 0 : (int64_t)L.fr_lv_ptr.size() - 1;
+  info[3] = L.fr_fo.empty() ? 0 : L.fr_fo.back();
+  info[4] = L.fr_max_m;
+  info[5] = L.fr_flops;
+  return XG_OK;
+}
+
+int xg_lu_front_export(const xg_handle *H, int64_t *sizes, int32_t *const *i32,
+                       int64_t *const *i64) {
+  if (!H || !sizes) return fail(XG_E_ARG, "xg_lu_front_export: bad argument");
+  const xg::Plan &P = H->h.plan;
+  const xg::LuPlan &L = P.lu;
+  const int64_t nf = (int64_t)L.fr_m.size();
+  sizes[0] = nf;
+  sizes[1] = nf ? L.fr_ro.back() : 0;
+  sizes[2] = (int64_t)L.fr_ch.size();
+  sizes[3] = L.fr_lv_ptr.empty() ? 0 : (int64_t)L.fr_lv_ptr.size() - 1;
+  sizes[4] = (int64_t)L.fr_ent_k.size();
+  sizes[5] = (int64_t)P.comp_start.size() - 1;
+  if (!i32 && !i64) return XG_OK;
+  if (i32) {
+    const std::vector<int32_t> *src[11] = {&L.fr_comp,  &L.fr_m,     &L.fr_k,
+                                           &L.fr_rows,  &L.fr_cbmap, &L.fr_ch_ptr,
+                                           &L.fr_ch,    &L.fr_lv_ptr, &L.fr_lv,
+                                           &L.fr_ent_k, &L.front_comp};
+    for (int k = 0; k < 11; ++k)
+      if (i32[k] && !src[k]->empty())
+        std::memcpy(i32[k], src[k]->data(), src[k]->size() * sizeof(int32_t));
+  }
+  if (i64) {
+    const std::vector<int64_t> *src[3] = {&L.fr_fo, &L.fr_ro, &L.fr_ent_fpos};
+    for (int k = 0; k < 3; ++k)
+      if (i64[k] && !src[k]->empty())
+        std::memcpy(i64[k], src[k]->data(), src[k]->size() * sizeof(int64_t));
+  }
   return XG_OK;
 }
 
@@ -538,8 +618,9 @@ int handle_assemble_norms(xg_handle *H, int32_t transient, const double *qn0,
 hipError_t handle_iteration_counts(xg_handle *H, int32_t *comp_cnt, int64_t *out128) {
   Handle &h = H->h;
   return launch_iteration_counts(h.limited_all, h.slot_comp, h.n_limited_slots, comp_cnt,
-                                 out128, h.stream);
+                                 out128, h.stream, h.gate);
 }
+void handle_set_gate(xg_handle *H, const int32_t *gate) { H->h.gate = gate; }
 hipStream_t handle_stream(const xg_handle *H) { return H->h.stream; }
 int32_t handle_n_comp(const xg_handle *H) {
   return (int32_t)H->h.plan.comp_start.size() - 1;
@@ -641,6 +722,7 @@ int xg_circuit_set_pattern(xg_circuit *c, const int32_t *rowptr,
 // Device allocation and upload of a built plan (the device half of
 // xg_finalize).  Returns false if an allocation or copy failed.
 static bool upload_handle(xg_handle *H) {
+  if (hipGetDevice(&H->h.device) != hipSuccess) return false;
   xg::Handle &h = H->h;
   int rc = 0;
   const xg::Plan &P = h.plan;
@@ -748,6 +830,36 @@ static bool upload_handle(xg_handle *H) {
     chk(u.val_work = h.alloc<double>((size_t)(gwork > 0 ? gwork : 1)));
     u.vec_work = d.bwork;
     chk(u.fallback = h.alloc<int32_t>((size_t)(u.n_comp > 0 ? u.n_comp : 1)));
+    // dense fronts
+    xg::FrontArgs &f = h.fr;
+    f.n_fronts = (int32_t)L.fr_m.size();
+    f.n_comp = u.n_comp;
+    if (f.n_fronts > 0) {
+      h.fr_lv_ptr = L.fr_lv_ptr;
+      h.fr_lv_maxm = L.fr_lv_maxm;
+      f.n_levels = (int32_t)L.fr_lv_maxm.size();
+      f.n_vals = L.fr_fo.back();
+      f.n_ent = (int64_t)L.fr_ent_k.size();
+      chk(f.comp = h.upload(L.fr_comp));
+      chk(f.m = h.upload(L.fr_m));
+      chk(f.k = h.upload(L.fr_k));
+      chk(f.rows = h.upload(L.fr_rows));
+      chk(f.cbmap = h.upload(L.fr_cbmap));
+      chk(f.ch_ptr = h.upload(L.fr_ch_ptr));
+      chk(f.ch = h.upload(L.fr_ch));
+      chk(f.lv = h.upload(L.fr_lv));
+      chk(f.ent_k = h.upload(L.fr_ent_k));
+      chk(f.front_comp = h.upload(L.front_comp));
+      chk(f.fo = h.upload(L.fr_fo));
+      chk(f.ro = h.upload(L.fr_ro));
+      chk(f.ent_fpos = h.upload(L.fr_ent_fpos));
+      chk(f.vals = h.alloc<double>((size_t)f.n_vals));
+      chk(f.wrhs = h.alloc<double>((size_t)L.fr_ro.back()));
+      chk(f.v = h.alloc<double>((size_t)(P.n_nodes > 0 ? P.n_nodes : 1)));
+      chk(f.piv = h.alloc<int32_t>((size_t)L.fr_ro.back()));
+      chk(f.bad = h.alloc<int32_t>((size_t)(u.n_comp > 0 ? u.n_comp : 1)));
+      u.front = f.front_comp;
+    }
     if (ok && hipMemset(u.fallback, 0, sizeof(int32_t) * (u.n_comp > 0 ? u.n_comp : 1)) !=
                   hipSuccess)
       ok = false;
@@ -827,6 +939,154 @@ bool upload_handle_nothrow(xg_handle *H) {
 }
 }  // namespace
 
+}  // extern "C"
+
+static int eval_types(xg::Handle &h, const xg::Mos1Args &m1, const xg::Bsim3Dev &b3,
+                      const xg::LinArgs &lin, int32_t type, const double *x_next,
+                      double *sta_curr, const double *sto_curr, double *sta_next,
+                      double *sto_next);
+
+// ---- autotuner: the BSIM3 eval kernel form and whether the linear devices
+// run in trailing workgroups of its launch are timed on the circuit itself
+// at upload and the fastest kept; the choice is cached by circuit shape
+// (eval slots per device type and model cards), so further circuits of the
+// same shape take it without timing.  Every candidate computes the same
+// bits; only the launch configuration differs.
+namespace {
+
+struct TuneKey {
+  int64_t m1, b3, lin, cards;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(m1, b3, lin, cards) < std::tie(o.m1, o.b3, o.lin, o.cards);
+  }
+};
+struct TuneChoice {
+  int32_t variant;
+  bool fuse;
+  double us[6];
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneChoice> g_tune_cache;
+
+constexpr int32_t kTuneMinWaves = 8;  // smaller circuits keep the default
+
+int autotune(xg::Handle &h) {
+  const xg::Plan &P = h.plan;
+  if (h.b3.n_slots < 64 * kTuneMinWaves) return XG_OK;
+  const TuneKey key{P.m1_slots, h.b3.n_slots, P.lin_slots, (int64_t)h.b3.n_sdp};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      h.b3.variant = it->second.variant;
+      h.fuse_linear = it->second.fuse;
+      std::memcpy(h.tune_us, it->second.us, sizeof(h.tune_us));
+      h.tune_source = 2;
+      return XG_OK;
+    }
+  }
+  const int64_t n1 = P.n_nodes + 1, ns = std::max(P.n_state, 1), nt = std::max(P.n_store, 1);
+  std::vector<double> hx(n1);
+  for (int64_t i = 0; i < n1; ++i) hx[i] = i + 1 < n1 ? 0.2 + std::fmod(0.618034 * i, 1.0) : 0.0;
+  double *x = nullptr, *st = nullptr, *dvd = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = XG_OK;
+  const int64_t nd = std::max<int64_t>(h.lin.n_slots, 1);
+  if (hipMalloc(&x, sizeof(double) * n1) != hipSuccess ||
+      hipMalloc(&st, sizeof(double) * 2 * (ns + nt)) != hipSuccess ||
+      hipMalloc(&dvd, sizeof(double) * nd) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = XG_E_NOMEM;
+  }
+  const xg::SolverStateDev ss_saved = h.ss;
+  const int32_t v_saved = h.b3.variant;
+  if (rc == XG_OK &&
+      (hipMemcpy(x, hx.data(), sizeof(double) * n1, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemset(st, 0, sizeof(double) * 2 * (ns + nt)) != hipSuccess ||
+       (h.lin.n_slots &&
+        hipMemcpy(dvd, h.lin.dvd, sizeof(double) * nd, hipMemcpyDeviceToDevice) != hipSuccess)))
+    rc = XG_E_HIP;
+  double best = -1.0;
+  TuneChoice ch{XG_EVAL_AUTO, true, {}};
+  for (double &u : ch.us) u = std::nan("");
+  if (rc == XG_OK) {
+    h.ss = xg::SolverStateDev{};
+    h.ss.transient = 1;
+    h.ss.newtonIter = 1;
+    const int32_t forms[3] = {XG_EVAL_HBM, XG_EVAL_STAGED, XG_EVAL_STAGED_SPLIT};
+    for (int f = 0; f < 3 && rc == XG_OK; ++f)
+      for (int fuse = 1; fuse >= 0 && rc == XG_OK; --fuse) {
+        if (!fuse && h.lin.n_slots == 0) continue;
+        h.b3.variant = forms[f];
+        h.fuse_linear = fuse != 0;
+        auto run = [&]() {
+          return eval_types(h, h.m1, h.b3, h.lin, 0, x, st, st + ns, st + ns + nt,
+                            st + 2 * ns + nt);
+        };
+        for (int k = 0; k < 2 && rc == XG_OK; ++k) rc = run();
+        if (rc) break;
+        const int reps = 5;
+        (void)hipEventRecord(e0, h.stream);
+        for (int k = 0; k < reps && rc == XG_OK; ++k) rc = run();
+        (void)hipEventRecord(e1, h.stream);
+        float ms = 0.0f;
+        if (rc == XG_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = XG_E_HIP;
+        if (rc) break;
+        const double us = 1e3 * ms / reps;
+        ch.us[2 * f + (fuse ? 0 : 1)] = us;
+        if (best < 0 || us < best) {
+          best = us;
+          ch.variant = forms[f];
+          ch.fuse = fuse != 0;
+        }
+      }
+  }
+  // the tuning launches leave no trace: statement values, flags and the
+  // diodes' junction voltages back to their state after upload
+  if (h.lin.n_slots && dvd)
+    (void)hipMemcpy(h.lin.dvd, dvd, sizeof(double) * nd, hipMemcpyDeviceToDevice);
+  if (P.n_slot_values) (void)hipMemset(h.slots, 0, sizeof(double) * P.n_slot_values);
+  if (h.n_limited_slots)
+    (void)hipMemset(h.limited_all, 0, sizeof(int32_t) * h.n_limited_slots);
+  (void)hipStreamSynchronize(h.stream);
+  h.ss = ss_saved;
+  if (x) (void)hipFree(x);
+  if (st) (void)hipFree(st);
+  if (dvd) (void)hipFree(dvd);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc != XG_OK || best < 0) {
+    h.b3.variant = v_saved;
+    h.fuse_linear = true;
+    return rc;
+  }
+  h.b3.variant = ch.variant;
+  h.fuse_linear = ch.fuse;
+  std::memcpy(h.tune_us, ch.us, sizeof(h.tune_us));
+  h.tune_source = 1;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = ch;
+  return XG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xg_autotune_info(const xg_handle *H, int32_t *choice, double *us) {
+  if (!H) return fail(XG_E_ARG, "xg_autotune_info: bad argument");
+  const xg::Handle &h = H->h;
+  if (choice) {
+    choice[0] = h.b3.variant;
+    choice[1] = h.fuse_linear ? 1 : 0;
+    choice[2] = h.tune_source;
+  }
+  if (us) std::memcpy(us, h.tune_us, sizeof(h.tune_us));
+  return XG_OK;
+}
+
 int xg_finalize(xg_circuit *c, xg_handle **out) {
   if (!c || !out) return fail(XG_E_ARG, "xg_finalize: bad argument");
   auto *H = new (std::nothrow) xg_handle;
@@ -841,6 +1101,10 @@ int xg_finalize(xg_circuit *c, xg_handle **out) {
     delete H;
     return fail(XG_E_NOMEM, "xg_finalize: device allocation/upload failed");
   }
+  if (int rc2 = autotune(H->h)) {
+    delete H;
+    return fail(rc2, "xg_finalize: autotuning failed");
+  }
   *out = H;
   return XG_OK;
 }
@@ -850,6 +1114,7 @@ int xg_upload(xg_handle *H) {
   if (!H->h.plan_only) return fail(XG_E_STATE, "xg_upload: handle is already on the device");
   if (!upload_handle_nothrow(H)) return fail(XG_E_NOMEM, "xg_upload: device allocation/upload failed");
   H->h.plan_only = false;
+  if (int rc = autotune(H->h)) return fail(rc, "xg_upload: autotuning failed");
   return XG_OK;
 }
 
@@ -952,51 +1217,71 @@ int xg_set_stream(xg_handle *H, void *stream) {
   return XG_OK;
 }
 
-// The eval launches of one device type (0 = every type, in load order).
-static int update_state_types(xg_handle *H, int32_t type, const double *x_next,
-                              double *sta_curr, const double *sto_curr,
-                              double *sta_next, double *sto_next) {
-  if (H && H->h.plan_only)
-    return fail(XG_E_STATE, "handle was finalized without a device");
-  if (!H || !x_next) return fail(XG_E_ARG, "xg_update_state: null pointer");
-  if (type < 0 || type > XG_DEV_LINEAR)
-    return fail(XG_E_ARG, "xg_update_state_type: unknown device type");
-  xg::Handle &h = H->h;
-  // state / store pointers may be null when the circuit has none
-  if ((h.plan.n_state > 0 && (!sta_curr || !sta_next)) ||
-      (h.plan.n_store > 0 && (!sto_curr || !sto_next)))
-    return fail(XG_E_ARG, "xg_update_state: null pointer");
+// The eval launches of one device type (0 = every type, in load order),
+// with the device argument sets given (the handle's own, or copies bound to
+// other statement-value memory).
+static int eval_types(xg::Handle &h, const xg::Mos1Args &m1, const xg::Bsim3Dev &b3,
+                      const xg::LinArgs &lin, int32_t type, const double *x_next,
+                      double *sta_curr, const double *sto_curr, double *sta_next,
+                      double *sto_next) {
   xg::LoadPtrs p{x_next, sta_curr, sto_curr, sta_next, sto_next};
   p.active = h.active;
+  p.gate = h.gate;
   hipError_t e = hipSuccess;
   if (type == 0 || type == XG_DEV_MOS1) {
     p.comp = h.slot_comp;
-    e = xg::launch_mos1_eval(h.m1, h.ss, p, h.stream);
+    e = xg::launch_mos1_eval(m1, h.ss, p, h.stream);
     if (e != hipSuccess) return hip_fail(e, "mos1 eval");
   }
   // every type: the linear devices ride in trailing workgroups of the
   // BSIM3 launch when there are both
-  const bool fused = type == 0 && h.b3.n_slots > 0 && h.lin.n_slots > 0;
+  const bool fused = type == 0 && b3.n_slots > 0 && lin.n_slots > 0 && h.fuse_linear;
   if (type == 0 || type == XG_DEV_BSIM3) {
     p.comp = h.slot_comp + h.plan.m1_slots;
     xg::LinTail tail;
     if (fused) {
-      tail.a = h.lin;
+      tail.a = lin;
       tail.p = p;
       tail.p.comp = h.slot_comp + h.plan.m1_slots + xg::bsim3_eval_slots(h.plan);
       tail.time = h.time;
       tail.bp_tol = h.bp_tol;
       tail.dstep = h.dstep;
     }
-    e = xg::launch_bsim3_eval(h.b3, h.ss, p, h.stream, fused ? &tail : nullptr);
+    e = xg::launch_bsim3_eval(b3, h.ss, p, h.stream, fused ? &tail : nullptr);
     if (e != hipSuccess) return hip_fail(e, "bsim3 eval");
   }
   if ((type == 0 && !fused) || type == XG_DEV_LINEAR) {
     p.comp = h.slot_comp + h.plan.m1_slots + xg::bsim3_eval_slots(h.plan);
-    e = xg::launch_linear_eval(h.lin, h.ss, p, h.time, h.bp_tol, h.dstep, h.stream);
+    e = xg::launch_linear_eval(lin, h.ss, p, h.time, h.bp_tol, h.dstep, h.stream);
     if (e != hipSuccess) return hip_fail(e, "linear eval");
   }
   return XG_OK;
+}
+
+static int check_update_args(xg_handle *H, const double *x_next, const double *sta_curr,
+                             const double *sto_curr, const double *sta_next,
+                             const double *sto_next) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H || !x_next) return fail(XG_E_ARG, "xg_update_state: null pointer");
+  const xg::Handle &h = H->h;
+  // state / store pointers may be null when the circuit has none
+  if ((h.plan.n_state > 0 && (!sta_curr || !sta_next)) ||
+      (h.plan.n_store > 0 && (!sto_curr || !sto_next)))
+    return fail(XG_E_ARG, "xg_update_state: null pointer");
+  return XG_OK;
+}
+
+static int update_state_types(xg_handle *H, int32_t type, const double *x_next,
+                              double *sta_curr, const double *sto_curr,
+                              double *sta_next, double *sto_next) {
+  if (int rc = check_update_args(H, x_next, sta_curr, sto_curr, sta_next, sto_next))
+    return rc;
+  if (type < 0 || type > XG_DEV_LINEAR)
+    return fail(XG_E_ARG, "xg_update_state_type: unknown device type");
+  xg::Handle &h = H->h;
+  return eval_types(h, h.m1, h.b3, h.lin, type, x_next, sta_curr, sto_curr, sta_next,
+                    sto_next);
 }
 
 int xg_update_state(xg_handle *H, const double *x_next, double *sta_curr,
@@ -1020,7 +1305,7 @@ int xg_load_vectors(xg_handle *H, double *F, double *Q, double *dFdxdVp,
     return fail(XG_E_ARG, "xg_load_vectors: null pointer");
   xg::Handle &h = H->h;
   double *outs[4] = {F, Q, dFdxdVp, dQdxdVp};
-  hipError_t e = xg::launch_reduce(4, h.red_vec, outs, h.slots, h.partial, h.overwrite, h.stream);
+  hipError_t e = xg::launch_reduce(4, h.red_vec, outs, h.slots, h.partial, h.overwrite, h.stream, h.gate);
   if (e != hipSuccess) return hip_fail(e, "vector reduction");
   return XG_OK;
 }
@@ -1032,7 +1317,7 @@ int xg_load_matrices(xg_handle *H, double *dFdx, double *dQdx) {
     return fail(XG_E_ARG, "xg_load_matrices: null pointer");
   xg::Handle &h = H->h;
   double *outs[2] = {dFdx, dQdx};
-  hipError_t e = xg::launch_reduce(2, h.red_mat, outs, h.slots, h.partial, h.overwrite, h.stream);
+  hipError_t e = xg::launch_reduce(2, h.red_mat, outs, h.slots, h.partial, h.overwrite, h.stream, h.gate);
   if (e != hipSuccess) return hip_fail(e, "matrix reduction");
   return XG_OK;
 }
@@ -1050,9 +1335,124 @@ int xg_load_all(xg_handle *H, const double *x_next, double *sta_curr,
   xg::ReduceDev progs[6] = {h.red_vec[0], h.red_vec[1], h.red_vec[2],
                             h.red_vec[3], h.red_mat[0], h.red_mat[1]};
   double *outs[6] = {F, Q, dFdxdVp, dQdxdVp, dFdx, dQdx};
-  hipError_t e = xg::launch_reduce(6, progs, outs, h.slots, h.partial, h.overwrite, h.stream);
+  hipError_t e = xg::launch_reduce(6, progs, outs, h.slots, h.partial, h.overwrite, h.stream, h.gate);
   if (e != hipSuccess) return hip_fail(e, "reduction");
   return XG_OK;
+}
+
+// ---- host-pointer variants: the caller's vectors stay in host memory
+// (the reference's N_LAS vectors); each call stages them through one
+// pinned buffer to device mirrors and back, and returns when the results
+// are in the caller's arrays.
+static int host_mirrors(xg_handle *H) {
+  xg::Handle &h = H->h;
+  if (h.hx) return XG_OK;
+  const xg::Plan &P = h.plan;
+  const int64_t n1 = P.n_nodes + 1, nnz = (int64_t)P.colidx.size();
+  const int64_t ns = std::max<int64_t>(P.n_state, 1), nt = std::max<int64_t>(P.n_store, 1);
+  h.pinned_len = std::max<int64_t>({n1 + 2 * ns + 2 * nt, 4 * n1, 2 * nnz});
+  if (hipHostMalloc((void **)&h.pinned, sizeof(double) * h.pinned_len) != hipSuccess) {
+    h.pinned = nullptr;
+    return fail(XG_E_NOMEM, "host-pointer loads: pinned staging allocation failed");
+  }
+  if (!(h.hx = h.alloc<double>(n1)) || !(h.hsta_c = h.alloc<double>(ns)) ||
+      !(h.hsto_c = h.alloc<double>(nt)) || !(h.hsta_n = h.alloc<double>(ns)) ||
+      !(h.hsto_n = h.alloc<double>(nt)) || !(h.hvec = h.alloc<double>(4 * n1)) ||
+      !(h.hmat = h.alloc<double>(2 * std::max<int64_t>(nnz, 1)))) {
+    h.hx = nullptr;
+    return fail(XG_E_NOMEM, "host-pointer loads: device mirrors allocation failed");
+  }
+  return XG_OK;
+}
+
+// Copies n doubles host -> device (through the staging buffer at off).
+static hipError_t stage_in(xg::Handle &h, double *dev, const double *host, int64_t n,
+                           int64_t off) {
+  if (n <= 0 || !host) return hipSuccess;
+  std::memcpy(h.pinned + off, host, sizeof(double) * n);
+  return hipMemcpyAsync(dev, h.pinned + off, sizeof(double) * n, hipMemcpyHostToDevice,
+                        h.stream);
+}
+
+static hipError_t stage_out(xg::Handle &h, const double *dev, int64_t n, int64_t off) {
+  if (n <= 0) return hipSuccess;
+  return hipMemcpyAsync(h.pinned + off, dev, sizeof(double) * n, hipMemcpyDeviceToHost,
+                        h.stream);
+}
+
+int xg_update_state_host(xg_handle *H, const double *x_next, double *sta_curr,
+                         const double *sto_curr, double *sta_next, double *sto_next) {
+  if (int rc = check_update_args(H, x_next, sta_curr, sto_curr, sta_next, sto_next)) return rc;
+  if (int rc = host_mirrors(H)) return rc;
+  xg::Handle &h = H->h;
+  const int64_t n1 = h.plan.n_nodes + 1, ns = h.plan.n_state, nt = h.plan.n_store;
+  // in: x, current state and store, and the next store (the previous Newton
+  // iteration's voltages, read by the limiters); sta_next is written whole
+  hipError_t e = stage_in(h, h.hx, x_next, n1, 0);
+  if (e == hipSuccess) e = stage_in(h, h.hsta_c, sta_curr, ns, n1);
+  if (e == hipSuccess) e = stage_in(h, h.hsto_c, sto_curr, nt, n1 + ns);
+  if (e == hipSuccess) e = stage_in(h, h.hsto_n, sto_next, nt, n1 + ns + nt);
+  if (e == hipSuccess) e = stage_in(h, h.hsta_n, sta_next, ns, n1 + ns + 2 * nt);
+  if (e != hipSuccess) return hip_fail(e, "xg_update_state_host: copy in");
+  if (int rc = eval_types(h, h.m1, h.b3, h.lin, 0, h.hx, h.hsta_c, h.hsto_c, h.hsta_n,
+                          h.hsto_n))
+    return rc;
+  // out: next state and store, and the current state (written at initTran)
+  e = stage_out(h, h.hsta_n, ns, 0);
+  if (e == hipSuccess) e = stage_out(h, h.hsto_n, nt, ns);
+  if (e == hipSuccess) e = stage_out(h, h.hsta_c, ns, ns + nt);
+  if (e == hipSuccess) e = hipStreamSynchronize(h.stream);
+  if (e != hipSuccess) return hip_fail(e, "xg_update_state_host: copy out");
+  if (ns > 0) {
+    std::memcpy(sta_next, h.pinned, sizeof(double) * ns);
+    std::memcpy(sta_curr, h.pinned + ns + nt, sizeof(double) * ns);
+  }
+  if (nt > 0) std::memcpy(sto_next, h.pinned + ns, sizeof(double) * nt);
+  return XG_OK;
+}
+
+// The reductions of the last xg_update_state_host into the caller's host
+// arrays (added to what they hold, or written, per the output mode).
+static int reduce_host(xg_handle *H, int nprog, double *const *outs, int64_t len,
+                       const char *who) {
+  if (H && H->h.plan_only)
+    return fail(XG_E_STATE, "handle was finalized without a device");
+  if (!H) return fail(XG_E_ARG, std::string(who) + ": null handle");
+  for (int k = 0; k < nprog; ++k)
+    if (!outs[k]) return fail(XG_E_ARG, std::string(who) + ": null pointer");
+  xg::Handle &h = H->h;
+  if (!h.hx) return fail(XG_E_STATE, std::string(who) + ": call xg_update_state_host first");
+  double *dev = nprog == 4 ? h.hvec : h.hmat;
+  hipError_t e = hipSuccess;
+  if (!h.overwrite)
+    for (int k = 0; k < nprog && e == hipSuccess; ++k)
+      e = stage_in(h, dev + k * len, outs[k], len, k * len);
+  if (e != hipSuccess) return hip_fail(e, (std::string(who) + ": copy in").c_str());
+  double *douts[6];
+  for (int k = 0; k < nprog; ++k) douts[k] = dev + k * len;
+  e = nprog == 4 ? xg::launch_reduce(4, h.red_vec, douts, h.slots, h.partial, h.overwrite,
+                                     h.stream)
+                 : xg::launch_reduce(2, h.red_mat, douts, h.slots, h.partial, h.overwrite,
+                                     h.stream);
+  if (e == hipSuccess) e = stage_out(h, dev, nprog * len, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(h.stream);
+  if (e != hipSuccess) return hip_fail(e, (std::string(who) + ": reduction / copy out").c_str());
+  for (int k = 0; k < nprog; ++k)
+    std::memcpy(outs[k], h.pinned + k * len, sizeof(double) * len);
+  return XG_OK;
+}
+
+int xg_load_vectors_host(xg_handle *H, double *F, double *Q, double *dFdxdVp,
+                         double *dQdxdVp) {
+  double *outs[4] = {F, Q, dFdxdVp, dQdxdVp};
+  return reduce_host(H, 4, outs, H ? (int64_t)H->h.plan.n_nodes + 1 : 0,
+                     "xg_load_vectors_host");
+}
+
+int xg_load_matrices_host(xg_handle *H, double *dFdx, double *dQdx) {
+  double *outs[2] = {dFdx, dQdx};
+  return reduce_host(H, 2, outs, H ? (int64_t)H->h.plan.colidx.size() : 0,
+                     "xg_load_matrices_host");
 }
 
 int xg_get_stats(xg_handle *H, xg_stats *st) {

@@ -94,6 +94,432 @@ __device__ __forceinline__ void b3_overlap(double v, double cgxo, double cgxl,
   q = (cgxo + T3) * v - T3 * (T2 + 0.5 * ckappa * (T4 - 1.0));
 }
 
+// Overlap charge/capacitance of one side for capMod 0 (constant cgxo,
+// N_DEV_MOSFET_B3.C:7164-7187) and capMod 1 (7188-7215: bias-dependent
+// overlap cgxl over weffCV, square-root form below zero, linear above).
+__device__ __forceinline__ void b3_overlap01(int capmod, double v, double cgxo,
+                                             double cgxl, double weffCV,
+                                             double ckappa, double &cap,
+                                             double &q) {
+  if (capmod == 0) {
+    cap = cgxo;
+    q = cgxo * v;
+  } else if (v < 0.0) {
+    const double r = sqrt(1.0 - 4.0 * v / ckappa);
+    cap = cgxo + weffCV * cgxl / r;
+    q = cgxo * v - weffCV * 0.5 * cgxl * ckappa * (r - 1.0);
+  } else {
+    cap = cgxo + weffCV * cgxl;
+    q = (weffCV * cgxl + cgxo) * v;
+  }
+}
+
+// Channel charges and their capacitances, as the charge section hands them
+// to the companion step (qsrc is rebuilt there from the others).
+struct B3ChanQ {
+  double qgate, qdrn, qsrc, qbulk;
+  double cggb, cgsb, cgdb, cdgb, cdsb, cddb, cbgb, cbsb, cbdb;
+};
+
+// capMod 0 (N_DEV_MOSFET_B3.C:6005-6308): long-channel charges on the
+// flat-band voltage vfbcv in accumulation (Arg1 <= 0), depletion
+// (Vgst <= 0) and inversion, where saturation and linear forms split the
+// channel charge 0/100 (xpart > 0.5), 40/60 (xpart < 0.5) or 50/50.  The
+// body bias is Vbs when Vbseff < 0, else phi - Phis.
+__device__ __forceinline__ void b3_charge_cap0(
+    double xpart, double vfbcv, double phi, double k1ox, double CoxWL,
+    double abulkCVfactor, double Vbs, double Vbseff, double Phis,
+    double dPhis_dVb, double sqrtPhis, double dsqrtPhis_dVb, double Vgs_eff,
+    double dVgs_eff_dVg, double Vds, double Abulk0, double dAbulk0_dVb,
+    B3ChanQ &o) {
+  double vb, dvb_dVb;
+  if (Vbseff < 0.0) {
+    vb = Vbs;
+    dvb_dVb = 1.0;
+  } else {
+    vb = phi - Phis;
+    dvb_dVb = -dPhis_dVb;
+  }
+  const double Vth = vfbcv + phi + k1ox * sqrtPhis;
+  const double Vgst = Vgs_eff - Vth;
+  const double dVth_dVb = k1ox * dsqrtPhis_dVb;
+  const double Arg1 = Vgs_eff - vb - vfbcv;
+  o.qsrc = 0.0;
+  o.cgdb = o.cdgb = o.cddb = o.cdsb = o.cbdb = 0.0;
+  if (Arg1 <= 0.0) {
+    o.qgate = CoxWL * Arg1;
+    o.qbulk = -o.qgate;
+    o.qdrn = 0.0;
+    o.cggb = CoxWL * dVgs_eff_dVg;
+    o.cgsb = CoxWL * (dvb_dVb - dVgs_eff_dVg);
+    o.cbgb = -CoxWL * dVgs_eff_dVg;
+    o.cbsb = -o.cgsb;
+    return;
+  }
+  if (Vgst <= 0.0) {
+    const double h = 0.5 * k1ox;
+    const double r = sqrt(h * h + Arg1);
+    o.qgate = CoxWL * k1ox * (r - h);
+    o.qbulk = -o.qgate;
+    o.qdrn = 0.0;
+    const double c = CoxWL * h / r;
+    o.cggb = c * dVgs_eff_dVg;
+    o.cgsb = c * (dvb_dVb - dVgs_eff_dVg);
+    o.cbgb = -o.cggb;
+    o.cbsb = -o.cgsb;
+    return;
+  }
+  const double c3 = CoxWL / 3.0, c23 = 2.0 * c3;
+  const double AbulkCV = Abulk0 * abulkCVfactor;
+  const double dAbulkCV_dVb = abulkCVfactor * dAbulk0_dVb;
+  const double Vdsat = Vgst / AbulkCV;
+  const double dVdsat_dVg = dVgs_eff_dVg / AbulkCV;
+  const double dVdsat_dVb = -(Vdsat * dAbulkCV_dVb + dVth_dVb) / AbulkCV;
+  const int part = (xpart > 0.5) ? 0 : (xpart < 0.5) ? 1 : 2;
+  if (Vdsat <= Vds) {
+    o.qgate = CoxWL * (Vgs_eff - vfbcv - phi - Vdsat / 3.0);
+    const double qi = -c23 * Vgst;
+    o.qbulk = -(o.qgate + qi);
+    o.cggb = c3 * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+    const double gb = -c3 * dVdsat_dVb;
+    o.cgsb = -(o.cggb + gb);
+    o.cbgb = -(o.cggb - c23 * dVgs_eff_dVg);
+    const double bb = -(gb + c23 * dVth_dVb);
+    o.cbsb = -(o.cbgb + bb);
+    if (part == 0) {
+      o.qdrn = 0.0;
+    } else if (part == 1) {
+      o.qdrn = 0.4 * qi;
+      const double k = 0.4 * c23;
+      o.cdgb = -k * dVgs_eff_dVg;
+      o.cdsb = -(k * dVth_dVb + o.cdgb);
+    } else {
+      o.qdrn = 0.5 * qi;
+      o.cdgb = -c3 * dVgs_eff_dVg;
+      o.cdsb = -(c3 * dVth_dVb + o.cdgb);
+    }
+    return;
+  }
+  const double Alphaz = Vgst / Vdsat;
+  const double T1 = 2.0 * Vdsat - Vds;
+  const double T2 = Vds / (3.0 * T1);
+  const double T3 = T2 * Vds;
+  const double q4 = 0.25 * CoxWL;
+  const double T4 = q4 * Alphaz;
+  o.qgate = CoxWL * (Vgs_eff - vfbcv - phi - 0.5 * (Vds - T3));
+  const double T5 = T3 / T1;
+  o.cggb = CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+  const double gb = -CoxWL * T5 * dVdsat_dVb;
+  o.cgdb = CoxWL * (T2 - 0.5 + 0.5 * T5);
+  if (part == 0) o.cgsb = -(o.cggb + gb + o.cgdb);
+  else o.cgsb = -(o.cggb + o.cgdb + gb);
+  const double rV = 1.0 / Vdsat;
+  const double dAz_dVg = rV * (1.0 - Alphaz * dVdsat_dVg);
+  const double dAz_dVb = -rV * (dVth_dVb + Alphaz * dVdsat_dVb);
+  if (part == 0) {
+    const double a = 2.0 * Vds - T1 - 3.0 * T3;
+    const double b = T3 - T1 - 2.0 * Vds;
+    o.qdrn = T4 * a;
+    o.qbulk = -(o.qgate + o.qdrn + T4 * b);
+    const double qa = q4 * a, qb = q4 * b;
+    double k = 2.0 * T4 * (1.0 - 3.0 * T5);
+    o.cdgb = (qa * dAz_dVg - k * dVdsat_dVg) * dVgs_eff_dVg;
+    const double db = qa * dAz_dVb - k * dVdsat_dVb;
+    o.cddb = T4 * (3.0 - 6.0 * T2 - 3.0 * T5);
+    o.cdsb = -(o.cdgb + db + o.cddb);
+    k = 2.0 * T4 * (1.0 + T5);
+    const double bg = (qb * dAz_dVg - k * dVdsat_dVg) * dVgs_eff_dVg;
+    const double bbv = qb * dAz_dVb - k * dVdsat_dVb;
+    const double bd = T4 * (2.0 * T2 + T5 - 1.0);
+    const double bs = -(bg + bbv + bd);
+    o.cbgb = -(o.cggb + o.cdgb + bg);
+    o.cbdb = -(o.cgdb + o.cddb + bd);
+    o.cbsb = -(o.cgsb + o.cdsb + bs);
+  } else if (part == 1) {
+    const double w = 8.0 * Vdsat * Vdsat - 6.0 * Vdsat * Vds + 1.2 * Vds * Vds;
+    const double u = T2 / T1;
+    double a = Vds - T1 - u * w;
+    o.qdrn = T4 * a;
+    a *= q4;
+    const double v = u / T1;
+    const double k = T4 * (2.0 - 4.0 * v * w + u * (16.0 * Vdsat - 6.0 * Vds));
+    o.cdgb = (a * dAz_dVg - k * dVdsat_dVg) * dVgs_eff_dVg;
+    const double db = a * dAz_dVb - k * dVdsat_dVb;
+    o.cddb = T4 * (2.0 - (1.0 / (3.0 * T1 * T1) + 2.0 * v) * w +
+                   u * (6.0 * Vdsat - 2.4 * Vds));
+    o.cdsb = -(o.cdgb + db + o.cddb);
+    double e = 2.0 * (T1 + T3);
+    o.qbulk = -(o.qgate - T4 * e);
+    e *= q4;
+    const double f = 4.0 * T4 * (1.0 - T5);
+    const double bg = (-e * dAz_dVg - o.cdgb - f * dVdsat_dVg) * dVgs_eff_dVg;
+    const double bbv = -e * dAz_dVb - db - f * dVdsat_dVb;
+    const double bd = -4.0 * T4 * (T2 - 0.5 + 0.5 * T5) - o.cddb;
+    const double bs = -(bd + bbv + bg);
+    o.cbgb = -(o.cggb + o.cdgb + bg);
+    // the Vb term, as N_DEV_MOSFET_B3.C:6238 has it
+    o.cbdb = -(o.cgdb + o.cddb + bbv);
+    o.cbsb = -(o.cgsb + o.cdsb + bs);
+  } else {
+    double a = T1 + T3;
+    o.qdrn = -T4 * a;
+    o.qbulk = -(o.qgate + o.qdrn + o.qdrn);
+    a *= q4;
+    const double k = T4 * (2.0 * T5 - 2.0);
+    o.cdgb = (k * dVdsat_dVg - a * dAz_dVg) * dVgs_eff_dVg;
+    const double db = k * dVdsat_dVb - a * dAz_dVb;
+    o.cddb = T4 * (1.0 - 2.0 * T2 - T5);
+    o.cdsb = -(o.cdgb + db + o.cddb);
+    o.cbgb = -(o.cggb + 2.0 * o.cdgb);
+    o.cbdb = -(o.cgdb + 2.0 * o.cddb);
+    o.cbsb = -(o.cgsb + 2.0 * o.cdsb);
+  }
+}
+
+// capMod 1 (N_DEV_MOSFET_B3.C:6360-6520, piecewise: body charge, then the
+// saturation or linear inversion charge) and capMod 2 (6521-6685, smooth:
+// accumulation charge Qac0 on Vfbeff, body charge Qsub0, inversion charge
+// on VdseffCV), on the smoothed Vgsteff of the capMod > 0 preamble and
+// the flat-band voltage vfbzb.  xpart selects the source share.
+__device__ __forceinline__ void b3_charge_cap12(
+    int capmod, double xpart, double vfbzb, double k1ox, double CoxWL,
+    double abulkCVfactor, double VbseffCV, double dVbseffCV_dVb,
+    double dVbseff_dVb, double Vgs_eff, double dVgs_eff_dVg, double Vds,
+    double Abulk0, double dAbulk0_dVb, double Vgsteff, double dVgsteff_dVg,
+    double dVgsteff_dVd, double dVgsteff_dVb, B3ChanQ &o) {
+  const double AbulkCV = Abulk0 * abulkCVfactor;
+  const double dAbulkCV_dVb = abulkCVfactor * dAbulk0_dVb;
+  const double VdsatCV = Vgsteff / AbulkCV;
+  const int part = (xpart > 0.5) ? 0 : (xpart < 0.5) ? 1 : 2;
+  double qgate, qbulk, qsrc;
+  double Cgg, Cgd, Cgb, Cbg, Cbd, Cbb, Csg, Csd, Csb;
+  double Cgg1, Cgd1, Cgb1, Cbg1, Cbd1, Cbb1;
+  double T0, T1, T2, T3, T4, T5, T6, T7;
+  if (capmod == 1) {
+    const double Arg1 = Vgs_eff - VbseffCV - vfbzb - Vgsteff;
+    if (Arg1 <= 0.0) {
+      qgate = CoxWL * Arg1;
+      Cgg = CoxWL * (dVgs_eff_dVg - dVgsteff_dVg);
+      Cgd = -CoxWL * dVgsteff_dVd;
+      Cgb = -CoxWL * (dVbseffCV_dVb + dVgsteff_dVb);
+    } else {
+      T0 = 0.5 * k1ox;
+      T1 = sqrt(T0 * T0 + Arg1);
+      T2 = CoxWL * T0 / T1;
+      qgate = CoxWL * k1ox * (T1 - T0);
+      Cgg = T2 * (dVgs_eff_dVg - dVgsteff_dVg);
+      Cgd = -T2 * dVgsteff_dVd;
+      Cgb = -T2 * (dVbseffCV_dVb + dVgsteff_dVb);
+    }
+    qbulk = -qgate;
+    Cbg = -Cgg;
+    Cbd = -Cgd;
+    Cbb = -Cgb;
+    const double c3 = CoxWL / 3.0, c23 = 2.0 * c3;
+    if (VdsatCV < Vds) {
+      const double dVdsatCV_dVg = 1.0 / AbulkCV;
+      const double dVdsatCV_dVb = -VdsatCV * dAbulkCV_dVb / AbulkCV;
+      T0 = Vgsteff - VdsatCV / 3.0;
+      double dg = 1.0 - dVdsatCV_dVg / 3.0, db = -dVdsatCV_dVb / 3.0;
+      qgate += CoxWL * T0;
+      Cgg1 = CoxWL * dg;
+      Cgb1 = CoxWL * db + Cgg1 * dVgsteff_dVb;
+      Cgd1 = Cgg1 * dVgsteff_dVd;
+      Cgg1 *= dVgsteff_dVg;
+      Cgg += Cgg1;
+      Cgb += Cgb1;
+      Cgd += Cgd1;
+      T0 = VdsatCV - Vgsteff;
+      dg = dVdsatCV_dVg - 1.0;
+      db = dVdsatCV_dVb;
+      qbulk += c3 * T0;
+      Cbg1 = c3 * dg;
+      Cbb1 = c3 * db + Cbg1 * dVgsteff_dVb;
+      Cbd1 = Cbg1 * dVgsteff_dVd;
+      Cbg1 *= dVgsteff_dVg;
+      Cbg += Cbg1;
+      Cbb += Cbb1;
+      Cbd += Cbd1;
+      T0 = (part == 0) ? -c23 : (part == 1) ? -0.4 * CoxWL : -c3;
+      qsrc = T0 * Vgsteff;
+      Csg = T0 * dVgsteff_dVg;
+      Csb = T0 * dVgsteff_dVb;
+      Csd = T0 * dVgsteff_dVd;
+    } else {
+      T0 = AbulkCV * Vds;
+      T1 = 12.0 * (Vgsteff - 0.5 * T0 + 1.e-20);
+      T2 = Vds / T1;
+      T3 = T0 * T2;
+      const double dT3_dVg = -12.0 * T2 * T2 * AbulkCV;
+      const double dT3_dVd = 6.0 * T0 * (4.0 * Vgsteff - T0) / T1 / T1 - 0.5;
+      const double dT3_dVb = 12.0 * T2 * T2 * dAbulkCV_dVb * Vgsteff;
+      qgate += CoxWL * (Vgsteff - 0.5 * Vds + T3);
+      Cgg1 = CoxWL * (1.0 + dT3_dVg);
+      Cgb1 = CoxWL * dT3_dVb + Cgg1 * dVgsteff_dVb;
+      Cgd1 = CoxWL * dT3_dVd + Cgg1 * dVgsteff_dVd;
+      Cgg1 *= dVgsteff_dVg;
+      Cgg += Cgg1;
+      Cgb += Cgb1;
+      Cgd += Cgd1;
+      qbulk += CoxWL * (1.0 - AbulkCV) * (0.5 * Vds - T3);
+      Cbg1 = -CoxWL * ((1.0 - AbulkCV) * dT3_dVg);
+      Cbb1 = -CoxWL * ((1.0 - AbulkCV) * dT3_dVb + (0.5 * Vds - T3) * dAbulkCV_dVb) +
+             Cbg1 * dVgsteff_dVb;
+      Cbd1 = -CoxWL * (1.0 - AbulkCV) * dT3_dVd + Cbg1 * dVgsteff_dVd;
+      Cbg1 *= dVgsteff_dVg;
+      Cbg += Cbg1;
+      Cbb += Cbb1;
+      Cbd += Cbd1;
+      if (part == 0) {
+        T1 = T1 + T1;
+        qsrc = -CoxWL * (0.5 * Vgsteff + 0.25 * T0 - T0 * T0 / T1);
+        Csg = -CoxWL * (0.5 + 24.0 * T0 * Vds / T1 / T1 * AbulkCV);
+        Csb = -CoxWL * (0.25 * Vds * dAbulkCV_dVb -
+                        12.0 * T0 * Vds / T1 / T1 * (4.0 * Vgsteff - T0) * dAbulkCV_dVb) +
+              Csg * dVgsteff_dVb;
+        Csd = -CoxWL * (0.25 * AbulkCV - 12.0 * AbulkCV * T0 / T1 / T1 * (4.0 * Vgsteff - T0)) +
+              Csg * dVgsteff_dVd;
+        Csg *= dVgsteff_dVg;
+      } else if (part == 1) {
+        T1 = T1 / 12.0;
+        T2 = 0.5 * CoxWL / (T1 * T1);
+        T3 = Vgsteff * (2.0 * T0 * T0 / 3.0 + Vgsteff * (Vgsteff - 4.0 * T0 / 3.0)) -
+             2.0 * T0 * T0 * T0 / 15.0;
+        qsrc = -T2 * T3;
+        T4 = 4.0 / 3.0 * Vgsteff * (Vgsteff - T0) + 0.4 * T0 * T0;
+        Csg = -2.0 * qsrc / T1 -
+              T2 * (Vgsteff * (3.0 * Vgsteff - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
+        Csb = (qsrc / T1 * Vds + T2 * T4 * Vds) * dAbulkCV_dVb + Csg * dVgsteff_dVb;
+        Csd = (qsrc / T1 + T2 * T4) * AbulkCV + Csg * dVgsteff_dVd;
+        Csg *= dVgsteff_dVg;
+      } else {
+        qsrc = -0.5 * (qgate + qbulk);
+        Csg = -0.5 * (Cgg1 + Cbg1);
+        Csb = -0.5 * (Cgb1 + Cbb1);
+        Csd = -0.5 * (Cgd1 + Cbd1);
+      }
+    }
+  } else {
+    const double V3 = vfbzb - Vgs_eff + VbseffCV - kDelta3;
+    if (vfbzb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * vfbzb);
+    else T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * vfbzb);
+    T1 = 0.5 * (1.0 + V3 / T0);
+    const double Vfbeff = vfbzb - 0.5 * (V3 + T0);
+    const double dVfbeff_dVg = T1 * dVgs_eff_dVg;
+    const double dVfbeff_dVb = -T1 * dVbseffCV_dVb;
+    const double Qac0 = CoxWL * (Vfbeff - vfbzb);
+    const double dQac0_dVg = CoxWL * dVfbeff_dVg;
+    const double dQac0_dVb = CoxWL * dVfbeff_dVb;
+    T0 = 0.5 * k1ox;
+    T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteff;
+    if (k1ox == 0.0) {
+      T1 = 0.0;
+      T2 = 0.0;
+    } else if (T3 < 0.0) {
+      T1 = T0 + T3 / k1ox;
+      T2 = CoxWL;
+    } else {
+      T1 = sqrt(T0 * T0 + T3);
+      T2 = CoxWL * T0 / T1;
+    }
+    const double Qsub0 = CoxWL * k1ox * (T1 - T0);
+    const double dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg);
+    const double dQsub0_dVd = -T2 * dVgsteff_dVd;
+    const double dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteff_dVb);
+    const double V4 = VdsatCV - Vds - kDelta4;
+    T0 = sqrt(V4 * V4 + 4.0 * kDelta4 * VdsatCV);
+    double VdseffCV = VdsatCV - 0.5 * (V4 + T0);
+    T1 = 0.5 * (1.0 + V4 / T0);
+    T2 = kDelta4 / T0;
+    T3 = (1.0 - T1 - T2) / AbulkCV;
+    double dVdseffCV_dVg = T3;
+    const double dVdseffCV_dVd = T1;
+    double dVdseffCV_dVb = -T3 * VdsatCV * dAbulkCV_dVb;
+    if (Vds == 0.0) {
+      VdseffCV = 0.0;
+      dVdseffCV_dVg = 0.0;
+      dVdseffCV_dVb = 0.0;
+    }
+    T0 = AbulkCV * VdseffCV;
+    T1 = 12.0 * (Vgsteff - 0.5 * T0 + 1e-20);
+    T2 = VdseffCV / T1;
+    T3 = T0 * T2;
+    T4 = (1.0 - 12.0 * T2 * T2 * AbulkCV);
+    T5 = (6.0 * T0 * (4.0 * Vgsteff - T0) / (T1 * T1) - 0.5);
+    T6 = 12.0 * T2 * T2 * Vgsteff;
+    qgate = CoxWL * (Vgsteff - 0.5 * VdseffCV + T3);
+    Cgg1 = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cgd1 = CoxWL * T5 * dVdseffCV_dVd + Cgg1 * dVgsteff_dVd;
+    Cgb1 = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cgg1 * dVgsteff_dVb;
+    Cgg1 *= dVgsteff_dVg;
+    T7 = 1.0 - AbulkCV;
+    qbulk = CoxWL * T7 * (0.5 * VdseffCV - T3);
+    T4 = -T7 * (T4 - 1.0);
+    T5 = -T7 * T5;
+    T6 = -(T7 * T6 + (0.5 * VdseffCV - T3));
+    Cbg1 = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cbd1 = CoxWL * T5 * dVdseffCV_dVd + Cbg1 * dVgsteff_dVd;
+    Cbb1 = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cbg1 * dVgsteff_dVb;
+    Cbg1 *= dVgsteff_dVg;
+    if (part == 0) {
+      T1 = T1 + T1;
+      qsrc = -CoxWL * (0.5 * Vgsteff + 0.25 * T0 - T0 * T0 / T1);
+      T7 = (4.0 * Vgsteff - T0) / (T1 * T1);
+      T4 = -(0.5 + 24.0 * T0 * T0 / (T1 * T1));
+      T5 = -(0.25 * AbulkCV - 12.0 * AbulkCV * T0 * T7);
+      T6 = -(0.25 * VdseffCV - 12.0 * T0 * VdseffCV * T7);
+      Csg = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+      Csd = CoxWL * T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else if (part == 1) {
+      T1 = T1 / 12.0;
+      T2 = 0.5 * CoxWL / (T1 * T1);
+      T3 = Vgsteff * (2.0 * T0 * T0 / 3.0 + Vgsteff * (Vgsteff - 4.0 * T0 / 3.0)) -
+           2.0 * T0 * T0 * T0 / 15.0;
+      qsrc = -T2 * T3;
+      T7 = 4.0 / 3.0 * Vgsteff * (Vgsteff - T0) + 0.4 * T0 * T0;
+      T4 = -2.0 * qsrc / T1 -
+           T2 * (Vgsteff * (3.0 * Vgsteff - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
+      T5 = (qsrc / T1 + T2 * T7) * AbulkCV;
+      T6 = (qsrc / T1 * VdseffCV + T2 * T7 * VdseffCV);
+      Csg = (T4 + T5 * dVdseffCV_dVg);
+      Csd = T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else {
+      qsrc = -0.5 * (qgate + qbulk);
+      Csg = -0.5 * (Cgg1 + Cbg1);
+      Csb = -0.5 * (Cgb1 + Cbb1);
+      Csd = -0.5 * (Cgd1 + Cbd1);
+    }
+    qgate += Qac0 + Qsub0;
+    qbulk -= (Qac0 + Qsub0);
+    Cgg = dQac0_dVg + dQsub0_dVg + Cgg1;
+    Cgd = dQsub0_dVd + Cgd1;
+    Cgb = dQac0_dVb + dQsub0_dVb + Cgb1;
+    Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
+    Cbd = Cbd1 - dQsub0_dVd;
+    Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
+  }
+  Cgb *= dVbseff_dVb;
+  Cbb *= dVbseff_dVb;
+  Csb *= dVbseff_dVb;
+  o.qgate = qgate;
+  o.qbulk = qbulk;
+  o.qsrc = qsrc;
+  o.qdrn = -(qgate + qbulk + qsrc);
+  o.cggb = Cgg;
+  o.cgsb = -(Cgg + Cgd + Cgb);
+  o.cgdb = Cgd;
+  o.cdgb = -(Cgg + Cbg + Csg);
+  o.cdsb = (Cgg + Cgd + Cgb + Cbg + Cbd + Cbb + Csg + Csd + Csb);
+  o.cddb = -(Cgd + Cbd + Csd);
+  o.cbgb = Cbg;
+  o.cbsb = -(Cbg + Cbd + Cbb);
+  o.cbdb = Cbd;
+}
+
 // What one instance's evaluation reads from HBM before any arithmetic: its
 // flags, slot bases, the six terminal voltages and the five previous-store
 // values.  Loaded in the kernels' prologues so that every load is in flight
@@ -920,17 +1346,28 @@ __device__ __forceinline__ void b3_eval(const Bsim3Dev &a,
   csub = Isub;
   }  // doDC
 
-  // ---- charges (capMod 3) and capacitances (N_DEV_MOSFET_B3.C:5994-7252)
+  // ---- charges (capMod 0-3) and capacitances (N_DEV_MOSFET_B3.C:5994-7252)
   double qg = 0.0, qd = 0.0, qb = 0.0, qbs = 0.0, qbd = 0.0;
   double CAPcggb = 0.0, CAPcgdb = 0.0, CAPcgsb = 0.0, CAPcdgb = 0.0,
          CAPcddb = 0.0, CAPcdsb = 0.0, CAPcsgb = 0.0, CAPcsdb = 0.0,
          CAPcssb = 0.0, CAPcbgb = 0.0, CAPcbdb = 0.0, CAPcbsb = 0.0;
   if (doCV && charge) {
     const double xpart = cd[B3C_XPART];
+    const int capmod = (int)cd[B3C_CAPMOD];
     double qgate = 0.0, qdrn = 0.0, qsrc = 0.0, qbulk = 0.0;
     double cggb = 0.0, cgsb = 0.0, cgdb = 0.0, cdgb = 0.0, cdsb = 0.0,
            cddb = 0.0, cbgb = 0.0, cbsb = 0.0, cbdb = 0.0;
-    if (!(xpart < 0)) {
+    if (!(xpart < 0) && capmod == 0) {
+      B3ChanQ o;
+      b3_charge_cap0(xpart, P(B3R_VFBQ), phi, k1ox,
+                     cox * P(B3R_WEFFCV) * P(B3R_LEFFCV), P(B3R_ABULKCVF), Vbs,
+                     Vbseff, Phis, dPhis_dVb, sqrtPhis, dsqrtPhis_dVb, Vgs_eff,
+                     dVgs_eff_dVg, Vds, Abulk0, dAbulk0_dVb, o);
+      qgate = o.qgate; qdrn = o.qdrn; qsrc = o.qsrc; qbulk = o.qbulk;
+      cggb = o.cggb; cgsb = o.cgsb; cgdb = o.cgdb; cdgb = o.cdgb;
+      cdsb = o.cdsb; cddb = o.cddb; cbgb = o.cbgb; cbsb = o.cbsb;
+      cbdb = o.cbdb;
+    } else if (!(xpart < 0)) {
       double VbseffCV, dVbseffCV_dVb;
       if (Vbseff < 0.0) {
         VbseffCV = Vbseff;
@@ -966,214 +1403,226 @@ __device__ __forceinline__ void b3_eval(const Bsim3Dev &a,
         dVgsteffc_dVb = -dVgsteffc_dVg * (dVth_dVb + (Vgst - voffcv) / noff * dnoff_dVb) + Vgsteffc / noff * dnoff_dVb;
         dVgsteffc_dVg *= dVgs_eff_dVg;
       }
-      const double vfbzb = P(B3R_VFBZB);
-      double V3 = vfbzb - Vgs_eff + VbseffCV - kDelta3;
-      if (vfbzb <= 0.0) {
-        T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * vfbzb);
+      if (capmod != 3) {
+        B3ChanQ o;
+        b3_charge_cap12(capmod, xpart, P(B3R_VFBQ), k1ox, CoxWL, P(B3R_ABULKCVF),
+                        VbseffCV, dVbseffCV_dVb, dVbseff_dVb, Vgs_eff, dVgs_eff_dVg,
+                        Vds, Abulk0, dAbulk0_dVb, Vgsteffc, dVgsteffc_dVg,
+                        dVgsteffc_dVd, dVgsteffc_dVb, o);
+        qgate = o.qgate; qdrn = o.qdrn; qsrc = o.qsrc; qbulk = o.qbulk;
+        cggb = o.cggb; cgsb = o.cgsb; cgdb = o.cgdb; cdgb = o.cdgb;
+        cdsb = o.cdsb; cddb = o.cddb; cbgb = o.cbgb; cbsb = o.cbsb;
+        cbdb = o.cbdb;
       } else {
-        T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * vfbzb);
+        const double vfbzb = P(B3R_VFBQ);
+        double V3 = vfbzb - Vgs_eff + VbseffCV - kDelta3;
+        if (vfbzb <= 0.0) {
+          T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * vfbzb);
+        } else {
+          T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * vfbzb);
+        }
+        T1 = 0.5 * (1.0 + V3 / T0);
+        const double Vfbeff = vfbzb - 0.5 * (V3 + T0);
+        const double dVfbeff_dVg = T1 * dVgs_eff_dVg;
+        const double dVfbeff_dVb = -T1 * dVbseffCV_dVb;
+        const double Cox = cox;
+        double Tox = 1.0e8 * tox;
+        T0 = (Vgs_eff - VbseffCV - vfbzb) / Tox;
+        const double dT0c_dVg = dVgs_eff_dVg / Tox;
+        const double dT0c_dVb = -dVbseffCV_dVb / Tox;
+        const double acde = P(B3R_ACDE), ldeb = P(B3R_LDEB);
+        double tmp = T0 * acde;
+        double Tcen, dTcen_dVg, dTcen_dVd, dTcen_dVb;
+        if ((-kExpThreshold < tmp) && (tmp < kExpThreshold)) {
+          Tcen = ldeb * dm_exp(tmp);
+          dTcen_dVg = acde * Tcen;
+          dTcen_dVb = dTcen_dVg * dT0c_dVb;
+          dTcen_dVg *= dT0c_dVg;
+        } else if (tmp <= -kExpThreshold) {
+          Tcen = ldeb * kMinExp;
+          dTcen_dVg = dTcen_dVb = 0.0;
+        } else {
+          Tcen = ldeb * kMaxExp;
+          dTcen_dVg = dTcen_dVb = 0.0;
+        }
+        const double LINK = 1.0e-3 * tox;
+        V3 = ldeb - Tcen - LINK;
+        double V4 = sqrt(V3 * V3 + 4.0 * LINK * ldeb);
+        Tcen = ldeb - 0.5 * (V3 + V4);
+        T1 = 0.5 * (1.0 + V3 / V4);
+        dTcen_dVg *= T1;
+        dTcen_dVb *= T1;
+        double Ccen = kEpsSi / Tcen;
+        T2 = Cox / (Cox + Ccen);
+        double Coxeff = T2 * Ccen;
+        T3 = -Ccen / Tcen;
+        double dCoxeff_dVg = T2 * T2 * T3;
+        double dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
+        dCoxeff_dVg *= dTcen_dVg;
+        double CoxWLcen = CoxWL * Coxeff / Cox;
+        const double Qac0 = CoxWLcen * (Vfbeff - vfbzb);
+        double QovCox = Qac0 / Coxeff;
+        const double dQac0_dVg = CoxWLcen * dVfbeff_dVg + QovCox * dCoxeff_dVg;
+        const double dQac0_dVb = CoxWLcen * dVfbeff_dVb + QovCox * dCoxeff_dVb;
+        T0 = 0.5 * k1ox;
+        T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteffc;
+        if (k1ox == 0.0) {
+          T1 = 0.0;
+          T2 = 0.0;
+        } else if (T3 < 0.0) {
+          T1 = T0 + T3 / k1ox;
+          T2 = CoxWLcen;
+        } else {
+          T1 = sqrt(T0 * T0 + T3);
+          T2 = CoxWLcen * T0 / T1;
+        }
+        const double Qsub0 = CoxWLcen * k1ox * (T1 - T0);
+        QovCox = Qsub0 / Coxeff;
+        const double dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteffc_dVg) + QovCox * dCoxeff_dVg;
+        const double dQsub0_dVd = -T2 * dVgsteffc_dVd;
+        const double dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteffc_dVb) + QovCox * dCoxeff_dVb;
+        double Denomi;
+        const double moin = P(B3R_MOIN);
+        if (k1ox <= 0.0) {
+          Denomi = 0.25 * moin * Vtm;
+          T0 = 0.5 * sqrtPhi;
+        } else {
+          Denomi = moin * Vtm * k1ox * k1ox;
+          T0 = k1ox * sqrtPhi;
+        }
+        T1 = 2.0 * T0 + Vgsteffc;
+        const double DeltaPhi = Vtm * dm_log(1.0 + T1 * Vgsteffc / Denomi);
+        const double dDeltaPhi_dVg = 2.0 * Vtm * (T1 - T0) / (Denomi + T1 * Vgsteffc);
+        T3 = 4.0 * (Vth - vfbzb - phi);
+        Tox += Tox;
+        double dT0c_dVd, dT0c_dVb2;
+        if (T3 >= 0.0) {
+          T0 = (Vgsteffc + T3) / Tox;
+          dT0c_dVd = (dVgsteffc_dVd + 4.0 * dVth_dVd) / Tox;
+          dT0c_dVb2 = (dVgsteffc_dVb + 4.0 * dVth_dVb) / Tox;
+        } else {
+          T0 = (Vgsteffc + 1.0e-20) / Tox;
+          dT0c_dVd = dVgsteffc_dVd / Tox;
+          dT0c_dVb2 = dVgsteffc_dVb / Tox;
+        }
+        tmp = dm_exp(0.7 * dm_log(T0));
+        T1 = 1.0 + tmp;
+        T2 = 0.7 * tmp / (T0 * Tox);
+        Tcen = 1.9e-9 / T1;
+        dTcen_dVg = -1.9e-9 * T2 / T1 / T1;
+        dTcen_dVd = Tox * dTcen_dVg;
+        dTcen_dVb = dTcen_dVd * dT0c_dVb2;
+        dTcen_dVd *= dT0c_dVd;
+        dTcen_dVg *= dVgsteffc_dVg;
+        Ccen = kEpsSi / Tcen;
+        T0 = Cox / (Cox + Ccen);
+        Coxeff = T0 * Ccen;
+        T1 = -Ccen / Tcen;
+        dCoxeff_dVg = T0 * T0 * T1;
+        const double dCoxeff_dVd = dCoxeff_dVg * dTcen_dVd;
+        dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
+        dCoxeff_dVg *= dTcen_dVg;
+        CoxWLcen = CoxWL * Coxeff / Cox;
+        const double abulkCVfactor = P(B3R_ABULKCVF);
+        const double AbulkCV = Abulk0 * abulkCVfactor;
+        const double dAbulkCV_dVb = abulkCVfactor * dAbulk0_dVb;
+        const double VdsatCV = (Vgsteffc - DeltaPhi) / AbulkCV;
+        V4 = VdsatCV - Vds - kDelta4;
+        T0 = sqrt(V4 * V4 + 4.0 * kDelta4 * VdsatCV);
+        double VdseffCV = VdsatCV - 0.5 * (V4 + T0);
+        T1 = 0.5 * (1.0 + V4 / T0);
+        T2 = kDelta4 / T0;
+        T3 = (1.0 - T1 - T2) / AbulkCV;
+        T4 = T3 * (1.0 - dDeltaPhi_dVg);
+        double dVdseffCV_dVg = T4;
+        const double dVdseffCV_dVd = T1;
+        double dVdseffCV_dVb = -T3 * VdsatCV * dAbulkCV_dVb;
+        if (Vds == 0.0) {
+          VdseffCV = 0.0;
+          dVdseffCV_dVg = 0.0;
+          dVdseffCV_dVb = 0.0;
+        }
+        T0 = AbulkCV * VdseffCV;
+        T1 = Vgsteffc - DeltaPhi;
+        T2 = 12.0 * (T1 - 0.5 * T0 + 1.0e-20);
+        T3 = T0 / T2;
+        T4 = 1.0 - 12.0 * T3 * T3;
+        T5 = AbulkCV * (6.0 * T0 * (4.0 * T1 - T0) / (T2 * T2) - 0.5);
+        T6 = T5 * VdseffCV / AbulkCV;
+        qgate = CoxWLcen * (T1 - T0 * (0.5 - T3));
+        QovCox = qgate / Coxeff;
+        double Cgg1 = CoxWLcen * (T4 * (1.0 - dDeltaPhi_dVg) + T5 * dVdseffCV_dVg);
+        const double Cgd1 = CoxWLcen * T5 * dVdseffCV_dVd + Cgg1 * dVgsteffc_dVd + QovCox * dCoxeff_dVd;
+        const double Cgb1 = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cgg1 * dVgsteffc_dVb + QovCox * dCoxeff_dVb;
+        Cgg1 = Cgg1 * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
+        T7 = 1.0 - AbulkCV;
+        T8 = T2 * T2;
+        T9 = 12.0 * T7 * T0 * T0 / (T8 * AbulkCV);
+        T10 = T9 * (1.0 - dDeltaPhi_dVg);
+        const double T11 = -T7 * T5 / AbulkCV;
+        const double T12 = -(T9 * T1 / AbulkCV + VdseffCV * (0.5 - T0 / T2));
+        qbulk = CoxWLcen * T7 * (0.5 * VdseffCV - T0 * VdseffCV / T2);
+        QovCox = qbulk / Coxeff;
+        double Cbg1 = CoxWLcen * (T10 + T11 * dVdseffCV_dVg);
+        const double Cbd1 = CoxWLcen * T11 * dVdseffCV_dVd + Cbg1 * dVgsteffc_dVd + QovCox * dCoxeff_dVd;
+        const double Cbb1 = CoxWLcen * (T11 * dVdseffCV_dVb + T12 * dAbulkCV_dVb) + Cbg1 * dVgsteffc_dVb + QovCox * dCoxeff_dVb;
+        Cbg1 = Cbg1 * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
+        double Csg, Csd, Csb;
+        if (xpart > 0.5) {
+          qsrc = -CoxWLcen * (T1 / 2.0 + T0 / 4.0 - 0.5 * T0 * T0 / T2);
+          QovCox = qsrc / Coxeff;
+          T2 += T2;
+          T3 = T2 * T2;
+          T7 = -(0.25 - 12.0 * T0 * (4.0 * T1 - T0) / T3);
+          T4 = -(0.5 + 24.0 * T0 * T0 / T3) * (1.0 - dDeltaPhi_dVg);
+          T5 = T7 * AbulkCV;
+          T6 = T7 * VdseffCV;
+          Csg = CoxWLcen * (T4 + T5 * dVdseffCV_dVg);
+          Csd = CoxWLcen * T5 * dVdseffCV_dVd + Csg * dVgsteffc_dVd + QovCox * dCoxeff_dVd;
+          Csb = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteffc_dVb + QovCox * dCoxeff_dVb;
+          Csg = Csg * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
+        } else if (xpart < 0.5) {
+          T2 = T2 / 12.0;
+          T3 = 0.5 * CoxWLcen / (T2 * T2);
+          T4 = T1 * (2.0 * T0 * T0 / 3.0 + T1 * (T1 - 4.0 * T0 / 3.0)) - 2.0 * T0 * T0 * T0 / 15.0;
+          qsrc = -T3 * T4;
+          QovCox = qsrc / Coxeff;
+          T8 = 4.0 / 3.0 * T1 * (T1 - T0) + 0.4 * T0 * T0;
+          T5 = -2.0 * qsrc / T2 - T3 * (T1 * (3.0 * T1 - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
+          T6 = AbulkCV * (qsrc / T2 + T3 * T8);
+          T7 = T6 * VdseffCV / AbulkCV;
+          Csg = T5 * (1.0 - dDeltaPhi_dVg) + T6 * dVdseffCV_dVg;
+          Csd = Csg * dVgsteffc_dVd + T6 * dVdseffCV_dVd + QovCox * dCoxeff_dVd;
+          Csb = Csg * dVgsteffc_dVb + T6 * dVdseffCV_dVb + T7 * dAbulkCV_dVb + QovCox * dCoxeff_dVb;
+          Csg = Csg * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
+        } else {
+          qsrc = -0.5 * qgate;
+          Csg = -0.5 * Cgg1;
+          Csd = -0.5 * Cgd1;
+          Csb = -0.5 * Cgb1;
+        }
+        qgate += Qac0 + Qsub0 - qbulk;
+        qbulk -= (Qac0 + Qsub0);
+        qdrn = -(qgate + qbulk + qsrc);
+        const double Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
+        const double Cbd = Cbd1 - dQsub0_dVd;
+        double Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
+        const double Cgg = Cgg1 - Cbg;
+        const double Cgd = Cgd1 - Cbd;
+        double Cgb = Cgb1 - Cbb;
+        Cgb *= dVbseff_dVb;
+        Cbb *= dVbseff_dVb;
+        Csb *= dVbseff_dVb;
+        cggb = Cgg;
+        cgsb = -(Cgg + Cgd + Cgb);
+        cgdb = Cgd;
+        cdgb = -(Cgg + Cbg + Csg);
+        cdsb = (Cgg + Cgd + Cgb + Cbg + Cbd + Cbb + Csg + Csd + Csb);
+        cddb = -(Cgd + Cbd + Csd);
+        cbgb = Cbg;
+        cbsb = -(Cbg + Cbd + Cbb);
+        cbdb = Cbd;
       }
-      T1 = 0.5 * (1.0 + V3 / T0);
-      const double Vfbeff = vfbzb - 0.5 * (V3 + T0);
-      const double dVfbeff_dVg = T1 * dVgs_eff_dVg;
-      const double dVfbeff_dVb = -T1 * dVbseffCV_dVb;
-      const double Cox = cox;
-      double Tox = 1.0e8 * tox;
-      T0 = (Vgs_eff - VbseffCV - vfbzb) / Tox;
-      const double dT0c_dVg = dVgs_eff_dVg / Tox;
-      const double dT0c_dVb = -dVbseffCV_dVb / Tox;
-      const double acde = P(B3R_ACDE), ldeb = P(B3R_LDEB);
-      double tmp = T0 * acde;
-      double Tcen, dTcen_dVg, dTcen_dVd, dTcen_dVb;
-      if ((-kExpThreshold < tmp) && (tmp < kExpThreshold)) {
-        Tcen = ldeb * dm_exp(tmp);
-        dTcen_dVg = acde * Tcen;
-        dTcen_dVb = dTcen_dVg * dT0c_dVb;
-        dTcen_dVg *= dT0c_dVg;
-      } else if (tmp <= -kExpThreshold) {
-        Tcen = ldeb * kMinExp;
-        dTcen_dVg = dTcen_dVb = 0.0;
-      } else {
-        Tcen = ldeb * kMaxExp;
-        dTcen_dVg = dTcen_dVb = 0.0;
-      }
-      const double LINK = 1.0e-3 * tox;
-      V3 = ldeb - Tcen - LINK;
-      double V4 = sqrt(V3 * V3 + 4.0 * LINK * ldeb);
-      Tcen = ldeb - 0.5 * (V3 + V4);
-      T1 = 0.5 * (1.0 + V3 / V4);
-      dTcen_dVg *= T1;
-      dTcen_dVb *= T1;
-      double Ccen = kEpsSi / Tcen;
-      T2 = Cox / (Cox + Ccen);
-      double Coxeff = T2 * Ccen;
-      T3 = -Ccen / Tcen;
-      double dCoxeff_dVg = T2 * T2 * T3;
-      double dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
-      dCoxeff_dVg *= dTcen_dVg;
-      double CoxWLcen = CoxWL * Coxeff / Cox;
-      const double Qac0 = CoxWLcen * (Vfbeff - vfbzb);
-      double QovCox = Qac0 / Coxeff;
-      const double dQac0_dVg = CoxWLcen * dVfbeff_dVg + QovCox * dCoxeff_dVg;
-      const double dQac0_dVb = CoxWLcen * dVfbeff_dVb + QovCox * dCoxeff_dVb;
-      T0 = 0.5 * k1ox;
-      T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteffc;
-      if (k1ox == 0.0) {
-        T1 = 0.0;
-        T2 = 0.0;
-      } else if (T3 < 0.0) {
-        T1 = T0 + T3 / k1ox;
-        T2 = CoxWLcen;
-      } else {
-        T1 = sqrt(T0 * T0 + T3);
-        T2 = CoxWLcen * T0 / T1;
-      }
-      const double Qsub0 = CoxWLcen * k1ox * (T1 - T0);
-      QovCox = Qsub0 / Coxeff;
-      const double dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteffc_dVg) + QovCox * dCoxeff_dVg;
-      const double dQsub0_dVd = -T2 * dVgsteffc_dVd;
-      const double dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteffc_dVb) + QovCox * dCoxeff_dVb;
-      double Denomi;
-      const double moin = P(B3R_MOIN);
-      if (k1ox <= 0.0) {
-        Denomi = 0.25 * moin * Vtm;
-        T0 = 0.5 * sqrtPhi;
-      } else {
-        Denomi = moin * Vtm * k1ox * k1ox;
-        T0 = k1ox * sqrtPhi;
-      }
-      T1 = 2.0 * T0 + Vgsteffc;
-      const double DeltaPhi = Vtm * dm_log(1.0 + T1 * Vgsteffc / Denomi);
-      const double dDeltaPhi_dVg = 2.0 * Vtm * (T1 - T0) / (Denomi + T1 * Vgsteffc);
-      T3 = 4.0 * (Vth - vfbzb - phi);
-      Tox += Tox;
-      double dT0c_dVd, dT0c_dVb2;
-      if (T3 >= 0.0) {
-        T0 = (Vgsteffc + T3) / Tox;
-        dT0c_dVd = (dVgsteffc_dVd + 4.0 * dVth_dVd) / Tox;
-        dT0c_dVb2 = (dVgsteffc_dVb + 4.0 * dVth_dVb) / Tox;
-      } else {
-        T0 = (Vgsteffc + 1.0e-20) / Tox;
-        dT0c_dVd = dVgsteffc_dVd / Tox;
-        dT0c_dVb2 = dVgsteffc_dVb / Tox;
-      }
-      tmp = dm_exp(0.7 * dm_log(T0));
-      T1 = 1.0 + tmp;
-      T2 = 0.7 * tmp / (T0 * Tox);
-      Tcen = 1.9e-9 / T1;
-      dTcen_dVg = -1.9e-9 * T2 / T1 / T1;
-      dTcen_dVd = Tox * dTcen_dVg;
-      dTcen_dVb = dTcen_dVd * dT0c_dVb2;
-      dTcen_dVd *= dT0c_dVd;
-      dTcen_dVg *= dVgsteffc_dVg;
-      Ccen = kEpsSi / Tcen;
-      T0 = Cox / (Cox + Ccen);
-      Coxeff = T0 * Ccen;
-      T1 = -Ccen / Tcen;
-      dCoxeff_dVg = T0 * T0 * T1;
-      const double dCoxeff_dVd = dCoxeff_dVg * dTcen_dVd;
-      dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
-      dCoxeff_dVg *= dTcen_dVg;
-      CoxWLcen = CoxWL * Coxeff / Cox;
-      const double abulkCVfactor = P(B3R_ABULKCVF);
-      const double AbulkCV = Abulk0 * abulkCVfactor;
-      const double dAbulkCV_dVb = abulkCVfactor * dAbulk0_dVb;
-      const double VdsatCV = (Vgsteffc - DeltaPhi) / AbulkCV;
-      V4 = VdsatCV - Vds - kDelta4;
-      T0 = sqrt(V4 * V4 + 4.0 * kDelta4 * VdsatCV);
-      double VdseffCV = VdsatCV - 0.5 * (V4 + T0);
-      T1 = 0.5 * (1.0 + V4 / T0);
-      T2 = kDelta4 / T0;
-      T3 = (1.0 - T1 - T2) / AbulkCV;
-      T4 = T3 * (1.0 - dDeltaPhi_dVg);
-      double dVdseffCV_dVg = T4;
-      const double dVdseffCV_dVd = T1;
-      double dVdseffCV_dVb = -T3 * VdsatCV * dAbulkCV_dVb;
-      if (Vds == 0.0) {
-        VdseffCV = 0.0;
-        dVdseffCV_dVg = 0.0;
-        dVdseffCV_dVb = 0.0;
-      }
-      T0 = AbulkCV * VdseffCV;
-      T1 = Vgsteffc - DeltaPhi;
-      T2 = 12.0 * (T1 - 0.5 * T0 + 1.0e-20);
-      T3 = T0 / T2;
-      T4 = 1.0 - 12.0 * T3 * T3;
-      T5 = AbulkCV * (6.0 * T0 * (4.0 * T1 - T0) / (T2 * T2) - 0.5);
-      T6 = T5 * VdseffCV / AbulkCV;
-      qgate = CoxWLcen * (T1 - T0 * (0.5 - T3));
-      QovCox = qgate / Coxeff;
-      double Cgg1 = CoxWLcen * (T4 * (1.0 - dDeltaPhi_dVg) + T5 * dVdseffCV_dVg);
-      const double Cgd1 = CoxWLcen * T5 * dVdseffCV_dVd + Cgg1 * dVgsteffc_dVd + QovCox * dCoxeff_dVd;
-      const double Cgb1 = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cgg1 * dVgsteffc_dVb + QovCox * dCoxeff_dVb;
-      Cgg1 = Cgg1 * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
-      T7 = 1.0 - AbulkCV;
-      T8 = T2 * T2;
-      T9 = 12.0 * T7 * T0 * T0 / (T8 * AbulkCV);
-      T10 = T9 * (1.0 - dDeltaPhi_dVg);
-      const double T11 = -T7 * T5 / AbulkCV;
-      const double T12 = -(T9 * T1 / AbulkCV + VdseffCV * (0.5 - T0 / T2));
-      qbulk = CoxWLcen * T7 * (0.5 * VdseffCV - T0 * VdseffCV / T2);
-      QovCox = qbulk / Coxeff;
-      double Cbg1 = CoxWLcen * (T10 + T11 * dVdseffCV_dVg);
-      const double Cbd1 = CoxWLcen * T11 * dVdseffCV_dVd + Cbg1 * dVgsteffc_dVd + QovCox * dCoxeff_dVd;
-      const double Cbb1 = CoxWLcen * (T11 * dVdseffCV_dVb + T12 * dAbulkCV_dVb) + Cbg1 * dVgsteffc_dVb + QovCox * dCoxeff_dVb;
-      Cbg1 = Cbg1 * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
-      double Csg, Csd, Csb;
-      if (xpart > 0.5) {
-        qsrc = -CoxWLcen * (T1 / 2.0 + T0 / 4.0 - 0.5 * T0 * T0 / T2);
-        QovCox = qsrc / Coxeff;
-        T2 += T2;
-        T3 = T2 * T2;
-        T7 = -(0.25 - 12.0 * T0 * (4.0 * T1 - T0) / T3);
-        T4 = -(0.5 + 24.0 * T0 * T0 / T3) * (1.0 - dDeltaPhi_dVg);
-        T5 = T7 * AbulkCV;
-        T6 = T7 * VdseffCV;
-        Csg = CoxWLcen * (T4 + T5 * dVdseffCV_dVg);
-        Csd = CoxWLcen * T5 * dVdseffCV_dVd + Csg * dVgsteffc_dVd + QovCox * dCoxeff_dVd;
-        Csb = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteffc_dVb + QovCox * dCoxeff_dVb;
-        Csg = Csg * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
-      } else if (xpart < 0.5) {
-        T2 = T2 / 12.0;
-        T3 = 0.5 * CoxWLcen / (T2 * T2);
-        T4 = T1 * (2.0 * T0 * T0 / 3.0 + T1 * (T1 - 4.0 * T0 / 3.0)) - 2.0 * T0 * T0 * T0 / 15.0;
-        qsrc = -T3 * T4;
-        QovCox = qsrc / Coxeff;
-        T8 = 4.0 / 3.0 * T1 * (T1 - T0) + 0.4 * T0 * T0;
-        T5 = -2.0 * qsrc / T2 - T3 * (T1 * (3.0 * T1 - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
-        T6 = AbulkCV * (qsrc / T2 + T3 * T8);
-        T7 = T6 * VdseffCV / AbulkCV;
-        Csg = T5 * (1.0 - dDeltaPhi_dVg) + T6 * dVdseffCV_dVg;
-        Csd = Csg * dVgsteffc_dVd + T6 * dVdseffCV_dVd + QovCox * dCoxeff_dVd;
-        Csb = Csg * dVgsteffc_dVb + T6 * dVdseffCV_dVb + T7 * dAbulkCV_dVb + QovCox * dCoxeff_dVb;
-        Csg = Csg * dVgsteffc_dVg + QovCox * dCoxeff_dVg;
-      } else {
-        qsrc = -0.5 * qgate;
-        Csg = -0.5 * Cgg1;
-        Csd = -0.5 * Cgd1;
-        Csb = -0.5 * Cgb1;
-      }
-      qgate += Qac0 + Qsub0 - qbulk;
-      qbulk -= (Qac0 + Qsub0);
-      qdrn = -(qgate + qbulk + qsrc);
-      const double Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
-      const double Cbd = Cbd1 - dQsub0_dVd;
-      double Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
-      const double Cgg = Cgg1 - Cbg;
-      const double Cgd = Cgd1 - Cbd;
-      double Cgb = Cgb1 - Cbb;
-      Cgb *= dVbseff_dVb;
-      Cbb *= dVbseff_dVb;
-      Csb *= dVbseff_dVb;
-      cggb = Cgg;
-      cgsb = -(Cgg + Cgd + Cgb);
-      cgdb = Cgd;
-      cdgb = -(Cgg + Cbg + Csg);
-      cdsb = (Cgg + Cgd + Cgb + Cbg + Cbd + Cbb + Csg + Csd + Csb);
-      cddb = -(Cgd + Cbd + Csd);
-      cbgb = Cbg;
-      cbsb = -(Cbg + Cbd + Cbb);
-      cbdb = Cbd;
     }
     // junctions (N_DEV_MOSFET_B3.C:6981-7134)
     double capbs, capbd;
@@ -1182,8 +1631,13 @@ __device__ __forceinline__ void b3_eval(const Bsim3Dev &a,
     // overlaps and the companion charges (7217-7252, 7632-7800)
     const double weffCV = P(B3R_WEFFCV), ckappa = P(B3R_CKAPPA);
     double cgdo, qgdo, cgso, qgso;
-    b3_overlap(vgd, P(B3R_CGDO), P(B3R_CGDL), weffCV, ckappa, cgdo, qgdo);
-    b3_overlap(vgs, P(B3R_CGSO), P(B3R_CGSL), weffCV, ckappa, cgso, qgso);
+    if (capmod >= 2) {
+      b3_overlap(vgd, P(B3R_CGDO), P(B3R_CGDL), weffCV, ckappa, cgdo, qgdo);
+      b3_overlap(vgs, P(B3R_CGSO), P(B3R_CGSL), weffCV, ckappa, cgso, qgso);
+    } else {
+      b3_overlap01(capmod, vgd, P(B3R_CGDO), P(B3R_CGDL), weffCV, ckappa, cgdo, qgdo);
+      b3_overlap01(capmod, vgs, P(B3R_CGSO), P(B3R_CGSL), weffCV, ckappa, cgso, qgso);
+    }
     const double cgbo = P(B3R_CGBO);
     const double qgb = cgbo * vgb;
     qgate += qgdo + qgso + qgb;
@@ -1447,6 +1901,7 @@ __global__ void __launch_bounds__(kBlock)
     bsim3_eval_kernel(Bsim3Dev a, SolverStateDev ss, LoadPtrs p, LinTail lt,
                       int32_t b3_blocks) {
   __shared__ double card_lds[kWavesPerBlock][B3C_N];
+  if (p.gate && *p.gate) return;
   if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
     lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
                          ((int)blockIdx.x - b3_blocks) * kBlock + threadIdx.x);
@@ -1500,6 +1955,7 @@ __global__ void __launch_bounds__((kSplit ? 128 : 64) * kGroups)
   constexpr int kThreads = 64 * kWaves * kGroups;
   __shared__ double rec[kGroups][kRecN * 64];
   __shared__ double card[kGroups][B3C_N];
+  if (p.gate && *p.gate) return;
   if ((int)blockIdx.x >= b3_blocks) {  // a trailing linear-device workgroup
     lin::linear_eval_one(lt.a, ss, lt.p, lt.time, lt.bp_tol, lt.dstep,
                          ((int)blockIdx.x - b3_blocks) * kThreads + threadIdx.x);

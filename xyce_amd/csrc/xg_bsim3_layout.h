@@ -11,12 +11,14 @@ namespace xg {
 enum B3Card : int {
   B3C_DTYPE = 0, B3C_MOBMOD, B3C_XPART, B3C_VCRIT, B3C_NVTM, B3C_VTM,
   B3C_IJTH, B3C_TEMPRATIO, B3C_PHIB, B3C_PHIBSW, B3C_PHIBSWG, B3C_MJ,
-  B3C_MJSW, B3C_MJSWG, B3C_GMIN, B3C_LIMITER,
-  B3C_N = 16
+  B3C_MJSW, B3C_MJSWG, B3C_GMIN, B3C_LIMITER, B3C_CAPMOD,
+  B3C_N = 17
 };
 
 // Size-dependent record (SizeDependParam plus the model fields a
-// Monte-Carlo sample may perturb: tox, cox, factor1).
+// Monte-Carlo sample may perturb: tox, cox, factor1).  B3R_VFBQ is the
+// flat-band voltage of the selected charge model: vfbcv for capMod 0,
+// vfbzb for capMod 1-3 (each model reads only its own).
 enum B3SdpField : int {
   B3R_LEFF = 0, B3R_WEFF, B3R_LEFFCV, B3R_WEFFCV, B3R_VTH0, B3R_VBSC,
   B3R_PHI, B3R_PHIS3, B3R_XDEP0, B3R_SQRTPHI, B3R_VBI, B3R_DVT2, B3R_DVT2W,
@@ -28,7 +30,7 @@ enum B3SdpField : int {
   B3R_KETA, B3R_UA, B3R_UC, B3R_UB, B3R_U0TEMP, B3R_VSATTEMP, B3R_A1,
   B3R_A2, B3R_DELTA, B3R_PCLM, B3R_LITL, B3R_THETAROUT, B3R_PDIBLB,
   B3R_PVAG, B3R_PSCBE1, B3R_PSCBE2, B3R_ALPHA0, B3R_ALPHA1, B3R_BETA0,
-  B3R_VFBZB, B3R_ACDE, B3R_LDEB, B3R_MOIN, B3R_ABULKCVF, B3R_NOFF,
+  B3R_VFBQ, B3R_ACDE, B3R_LDEB, B3R_MOIN, B3R_ABULKCVF, B3R_NOFF,
   B3R_VOFFCV, B3R_CGBO, B3R_CGDO, B3R_CGSO, B3R_CGDL, B3R_CGSL, B3R_CKAPPA,
   B3R_TOX, B3R_COX, B3R_FACTOR1,
   B3R_N

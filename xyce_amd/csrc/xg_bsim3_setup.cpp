@@ -112,8 +112,9 @@ static int b3_model_process(Bsim3Model &m, const Options &opt,
   auto G = [&](int k) { return m.g[k]; };
   if (G(P_TNOM)) v[P_TNOM] += 273.15;
   else v[P_TNOM] = opt.tnom;
-  if (v[P_CAPMOD] != 3.0) {
-    err = "BSIM3 capMod other than 3 is not supported by this library";
+  if (v[P_CAPMOD] != 0.0 && v[P_CAPMOD] != 1.0 && v[P_CAPMOD] != 2.0 &&
+      v[P_CAPMOD] != 3.0) {
+    err = "BSIM3 capMod must be 0, 1, 2 or 3";
     return XG_E_PARAM;
   }
   if (!G(P_VTH0)) v[P_VTH0] = (m.dtype == 1) ? 0.7 : -0.7;
@@ -316,7 +317,7 @@ static void b3_sdp_compute(const Bsim3Model &mm, double l, double w,
          beta0 = BIN(BETA0), elm = BIN(ELM), cgsl = BIN(CGSL),
          cgdl = BIN(CGDL), ckappa = BIN(CKAPPA), cf = BIN(CF), clc = BIN(CLC),
          cle = BIN(CLE), acde = BIN(ACDE), moin = BIN(MOIN), noff = BIN(NOFF),
-         voffcv = BIN(VOFFCV);
+         voffcv = BIN(VOFFCV), vfbcv = BIN(VFBCV);
 #undef BIN
   const double abulkCVfactor = 1.0 + std::pow((clc / leffCV), cle);
   T0 = (TRatio - 1.0);
@@ -421,7 +422,8 @@ static void b3_sdp_compute(const Bsim3Model &mm, double l, double w,
   r[B3R_DELTA] = delta; r[B3R_PCLM] = pclm; r[B3R_LITL] = litl;
   r[B3R_THETAROUT] = thetaRout; r[B3R_PDIBLB] = pdiblb; r[B3R_PVAG] = pvag;
   r[B3R_PSCBE1] = pscbe1; r[B3R_PSCBE2] = pscbe2; r[B3R_ALPHA0] = alpha0;
-  r[B3R_ALPHA1] = alpha1; r[B3R_BETA0] = beta0; r[B3R_VFBZB] = vfbzb;
+  r[B3R_ALPHA1] = alpha1; r[B3R_BETA0] = beta0;
+  r[B3R_VFBQ] = (m[P_CAPMOD] == 0.0) ? vfbcv : vfbzb;
   r[B3R_ACDE] = acde; r[B3R_LDEB] = ldeb; r[B3R_MOIN] = moin;
   r[B3R_ABULKCVF] = abulkCVfactor; r[B3R_NOFF] = noff;
   r[B3R_VOFFCV] = voffcv; r[B3R_CGBO] = cgbo; r[B3R_CGDO] = cgdo;
@@ -492,6 +494,7 @@ static void b3_card(const Bsim3Model &mm, double temp, const Options &opt,
   c[B3C_MJSWG] = m[P_MJSWG];
   c[B3C_GMIN] = opt.gmin;
   c[B3C_LIMITER] = opt.limiter ? 1.0 : 0.0;
+  c[B3C_CAPMOD] = m[P_CAPMOD];
 }
 
 int bsim3_prepare(Circuit &c, std::string &err) {

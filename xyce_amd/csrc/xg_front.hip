@@ -1,0 +1,460 @@
+// Multifrontal LU of the components too large for the task lists of
+// xg_lu.hip (the 20k-node random logic, the 550k-unknown power grid): the
+// refactor-and-solve that replaces N_LAS_AmesosSolver::solve (KLU,
+// N_LAS_AmesosSolver.C:231-470) for them.
+//
+// The plan (xg_lu_setup.cpp) gives every component a tree of dense fronts:
+// front s owns k consecutive pivot positions (its fully summed columns)
+// and has m >= k rows; its m x m block is row-major in one value buffer.
+// Per Newton iteration:
+//  1. every front is cleared and the Jacobian's entries are scattered into
+//     the fronts that own them;
+//  2. level by level from the leaves (a level's fronts are independent),
+//     one workgroup per front adds its children's contribution blocks (in
+//     child order), then factors its k fully summed columns: blocked
+//     right-looking LU, pivots chosen by magnitude among the fully summed
+//     rows only, panels in LDS, the trailing update leaving the Schur
+//     complement (the contribution block) in the bottom-right corner;
+//  3. the forward solve runs the same levels: a front gathers its
+//     right-hand side, adds its children's remainders, applies its row
+//     swaps, solves with its unit lower block and leaves the remainder for
+//     its parent; the backward solve walks the levels back down;
+//  4. one workgroup per component checks the normwise backward error
+//     against the unfactored rows, applies x += dx and forms the weighted
+//     RMS norm, as the task-list kernel does.
+// Every sum is in a fixed order, so the results do not depend on the
+// launch.  A pivot column with no nonzero candidate, or a backward error
+// above the bound, hands the component to the dense partial-pivoting
+// kernel (as the task lists do); beyond that kernel's size it stays
+// singular (stats[2] = 1) with its iterate unchanged.
+#include <algorithm>
+#include <cfloat>
+
+#include "xg_internal.h"
+
+namespace xg {
+
+namespace {
+
+constexpr int kFB = 256;           // threads per front workgroup
+constexpr int kFinishBlock = 1024;  // threads of the per-component check
+
+__device__ __forceinline__ void blk_argmax(double v, int idx, double *sv, int *si,
+                                           double &best, int &bi) {
+  // wave-level (value, smallest index) maximum, then across the waves
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(v, off);
+    const int oi = __shfl_xor(idx, off);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    sv[w] = v;
+    si[w] = idx;
+  }
+  __syncthreads();
+  best = sv[0];
+  bi = si[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+    if (sv[q] > best || (sv[q] == best && si[q] < bi)) {
+      best = sv[q];
+      bi = si[q];
+    }
+  __syncthreads();
+}
+
+template <int B>
+__device__ __forceinline__ double blk_reduce(double v, double *sv, bool is_max) {
+  const int t = threadIdx.x;
+  sv[t] = v;
+  __syncthreads();
+  for (int s = B / 2; s > 0; s >>= 1) {
+    if (t < s) sv[t] = is_max ? fmax(sv[t], sv[t + s]) : sv[t] + sv[t + s];
+    __syncthreads();
+  }
+  const double r = sv[0];
+  __syncthreads();
+  return r;
+}
+
+// Partial LU of the m x m row-major front A: its first k columns, pivots
+// among rows [j, k) for column j.  Panels of NB columns factor in the LDS
+// buffer P (rows kb..m-1, stride NB + 1); the panel's row swaps are then
+// applied to the other columns, U12 is solved column by column in
+// registers and the trailing block takes the panel's rank-NB update (each
+// element's updates in ascending pivot order, as in the column-by-column
+// algorithm).  piv[j] = the row swapped with row j.  False when a pivot
+// column has no nonzero candidate.
+template <int NB>
+__device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int *si,
+                             int *spiv, int32_t *piv) {
+  const int t = threadIdx.x;
+  constexpr int PS = NB + 1;
+  for (int kb = 0; kb < k; kb += NB) {
+    const int nb = min(NB, k - kb);
+    const int rows = m - kb;      // panel rows
+    const int cand = k - kb;      // rows eligible as pivots
+    for (int e = t; e < rows * NB; e += kFB) {
+      const int r = e / NB, c = e % NB;
+      P[r * PS + c] = c < nb ? A[(int64_t)(kb + r) * m + kb + c] : 0.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+      double v = -1.0;
+      int idx = 0x7fffffff;
+      for (int r = c + t; r < cand; r += kFB) {
+        const double a = fabs(P[r * PS + c]);
+        if (a > v) {
+          v = a;
+          idx = r;
+        }
+      }
+      double best;
+      int br;
+      blk_argmax(v, idx, sv, si, best, br);
+      if (!(best > 0.0) || !(best <= DBL_MAX)) return false;  // same on every thread
+      if (t == 0) spiv[c] = kb + br;
+      if (br != c)
+        for (int cc = t; cc < NB; cc += kFB) {
+          const double tmp = P[c * PS + cc];
+          P[c * PS + cc] = P[br * PS + cc];
+          P[br * PS + cc] = tmp;
+        }
+      __syncthreads();
+      const double pv = P[c * PS + c];
+      for (int r = c + 1 + t; r < rows; r += kFB) {
+        const double l = P[r * PS + c] / pv;
+        P[r * PS + c] = l;
+        for (int cc = c + 1; cc < nb; ++cc) P[r * PS + cc] -= l * P[c * PS + cc];
+      }
+      __syncthreads();
+    }
+    for (int e = t; e < rows * NB; e += kFB) {
+      const int r = e / NB, c = e % NB;
+      if (c < nb) A[(int64_t)(kb + r) * m + kb + c] = P[r * PS + c];
+    }
+    if (t < nb) piv[kb + t] = spiv[t];
+    // the panel's row swaps on every other column, in pivot order
+    for (int j = t; j < m; j += kFB) {
+      if (j >= kb && j < kb + nb) continue;
+      for (int c = 0; c < nb; ++c) {
+        const int pr = spiv[c];
+        if (pr != kb + c) {
+          const double tmp = A[(int64_t)(kb + c) * m + j];
+          A[(int64_t)(kb + c) * m + j] = A[(int64_t)pr * m + j];
+          A[(int64_t)pr * m + j] = tmp;
+        }
+      }
+    }
+    __syncthreads();
+    const int j0 = kb + nb, ncols = m - j0;
+    // U12 of column j (rows kb..kb+nb-1) by the unit lower panel block
+    auto solve_u = [&](int j, double *u) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) u[c] = c < nb ? A[(int64_t)(kb + c) * m + j] : 0.0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int i = c + 1; i < NB; ++i)
+          if (i < nb) u[i] -= P[i * PS + c] * u[c];
+    };
+    // rows j0+g, j0+g+G, ... of column j take the panel's updates
+    auto sweep = [&](int j, int g, int G, const double *u) {
+      for (int i = j0 + g; i < m; i += 4 * G) {
+        double x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ii = i + q * G;
+          x[q] = ii < m ? A[(int64_t)ii * m + j] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ii = i + q * G;
+          if (ii < m) {
+            const double *l = P + (ii - kb) * PS;
+            double v = x[q];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) v -= l[c] * u[c];
+            A[(int64_t)ii * m + j] = v;
+          }
+        }
+      }
+    };
+    if (ncols >= kFB) {
+      for (int j = j0 + t; j < m; j += kFB) {
+        double u[NB];
+        solve_u(j, u);
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < nb) A[(int64_t)(kb + c) * m + j] = u[c];
+        sweep(j, 0, 1, u);
+      }
+    } else if (ncols > 0) {
+      const int G = min(kFB / ncols, 16);
+      const bool mine = t < ncols * G;
+      const int j = j0 + t % ncols, g = t / ncols;
+      double u[NB];
+      if (mine) solve_u(j, u);
+      __syncthreads();
+      if (mine) {
+        if (g == 0)
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < nb) A[(int64_t)(kb + c) * m + j] = u[c];
+        sweep(j, g, G, u);
+      }
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool front_skip(const FrontArgs &f, int c) {
+  return (f.gate && *f.gate) || (f.active && !f.active[c]) || f.bad[c];
+}
+
+// Clears the fronts and the failure flags, then (second kernel) scatters
+// the Jacobian's entries into them.
+__global__ void __launch_bounds__(kFB) front_clear_kernel(FrontArgs f) {
+  if (f.gate && *f.gate) return;
+  const int64_t stride = (int64_t)gridDim.x * kFB;
+  for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < f.n_vals; i += stride)
+    f.vals[i] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < f.n_comp; i += stride)
+    f.bad[i] = 0;
+}
+
+__global__ void __launch_bounds__(kFB) front_scatter_kernel(FrontArgs f, const double *J) {
+  if (f.gate && *f.gate) return;
+  const int64_t stride = (int64_t)gridDim.x * kFB;
+  for (int64_t e = (int64_t)blockIdx.x * kFB + threadIdx.x; e < f.n_ent; e += stride)
+    f.vals[f.ent_fpos[e]] = J[f.ent_k[e]];
+}
+
+template <int NB, int ROWS>
+__global__ void __launch_bounds__(kFB) front_factor_kernel(FrontArgs f, int lv0) {
+  __shared__ double P[ROWS * (NB + 1)];
+  __shared__ double sv[kFB / 64];
+  __shared__ int si[kFB / 64];
+  __shared__ int spiv[NB];
+  const int s = f.lv[lv0 + blockIdx.x];
+  const int c = f.comp[s];
+  if (front_skip(f, c)) return;
+  const int t = threadIdx.x;
+  const int m = f.m[s], k = f.k[s];
+  double *A = f.vals + f.fo[s];
+  // the children's contribution blocks, in child order
+  for (int q = f.ch_ptr[s]; q < f.ch_ptr[s + 1]; ++q) {
+    const int ch = f.ch[q];
+    const int mc = f.m[ch], kc = f.k[ch], w = mc - kc;
+    const double *C = f.vals + f.fo[ch] + (int64_t)kc * mc + kc;
+    const int32_t *map = f.cbmap + f.ro[ch] + kc;
+    for (int64_t e = t; e < (int64_t)w * w; e += kFB) {
+      const int i = (int)(e / w), j = (int)(e % w);
+      A[(int64_t)map[i] * m + map[j]] += C[(int64_t)i * mc + j];
+    }
+    __syncthreads();
+  }
+  if (!front_factor<NB>(A, m, k, P, sv, si, spiv, f.piv + f.ro[s]))
+    if (t == 0) f.bad[c] = 1;
+}
+
+// Forward solve of one level's fronts: the front's right-hand side (its
+// fully summed rows' entries, zero below), its children's remainders,
+// its row swaps, then L y = b with the unit lower columns; y goes to the
+// position vector, the remainder below stays in wrhs for the parent.
+__global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const double *bp,
+                                                        int lv0) {
+  const int s = f.lv[lv0 + blockIdx.x];
+  const int c = f.comp[s];
+  if (front_skip(f, c)) return;
+  const int t = threadIdx.x;
+  const int m = f.m[s], k = f.k[s];
+  const double *A = f.vals + f.fo[s];
+  const int32_t *rows = f.rows + f.ro[s];
+  double *w = f.wrhs + f.ro[s];
+  for (int i = t; i < m; i += kFB) w[i] = i < k ? bp[rows[i]] : 0.0;
+  __syncthreads();
+  for (int q = f.ch_ptr[s]; q < f.ch_ptr[s + 1]; ++q) {
+    const int ch = f.ch[q];
+    const int kc = f.k[ch], wc = f.m[ch] - kc;
+    const double *r = f.wrhs + f.ro[ch] + kc;
+    const int32_t *map = f.cbmap + f.ro[ch] + kc;
+    for (int i = t; i < wc; i += kFB) w[map[i]] += r[i];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int32_t *pv = f.piv + f.ro[s];
+    for (int j = 0; j < k; ++j) {
+      const int p = pv[j];
+      if (p != j) {
+        const double tmp = w[j];
+        w[j] = w[p];
+        w[p] = tmp;
+      }
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    const double yj = w[j];
+    for (int i = j + 1 + t; i < m; i += kFB) w[i] -= A[(int64_t)i * m + j] * yj;
+    __syncthreads();
+  }
+  for (int i = t; i < k; i += kFB) f.v[rows[i]] = w[i];
+}
+
+// Backward solve of one level's fronts (levels from the root down): the
+// rows below the fully summed block are solved already (by ancestors).
+__global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
+  __shared__ double tb[kFrontMaxM];
+  const int s = f.lv[lv0 + blockIdx.x];
+  const int c = f.comp[s];
+  if (front_skip(f, c)) return;
+  const int t = threadIdx.x;
+  const int m = f.m[s], k = f.k[s];
+  const double *A = f.vals + f.fo[s];
+  const int32_t *rows = f.rows + f.ro[s];
+  for (int i = t; i < k; i += kFB) {
+    double r = f.v[rows[i]];
+    const double *a = A + (int64_t)i * m;
+    for (int j = k; j < m; ++j) r -= a[j] * f.v[rows[j]];
+    tb[i] = r;
+  }
+  __syncthreads();
+  for (int j = k - 1; j >= 0; --j) {
+    const double xj = tb[j] / A[(int64_t)j * m + j];
+    __syncthreads();
+    if (t == 0) tb[j] = xj;
+    for (int i = t; i < j; i += kFB) tb[i] -= A[(int64_t)i * m + j] * xj;
+    __syncthreads();
+  }
+  for (int i = t; i < k; i += kFB) f.v[rows[i]] = tb[i];
+}
+
+// Per front component: backward error of the solution against the
+// unfactored rows (entry-ordered J and position-ordered rhs of the permute
+// kernel), then x += dx and the weighted RMS norm of dx
+// (N_NLS_DampedNewton.C:311-335), as the task-list kernel does.
+__global__ void __launch_bounds__(kFinishBlock) front_finish_kernel(FrontArgs f,
+                                                                    SparseLuArgs a) {
+  __shared__ double sv[kFinishBlock];
+  const int c = blockIdx.x;
+  const int t = threadIdx.x;
+  if (f.gate && *f.gate) return;
+  if (!f.front_comp[c]) return;
+  if (a.active && !a.active[c]) return;  // stats written by the task kernel
+  const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
+  double *stats = a.stats + 4 * (size_t)c;
+  double rmax = 0.0;
+  for (int i = t; i < n; i += kFinishBlock) rmax = fmax(rmax, fabs(a.bp[beg + i]));
+  const double max_rhs = blk_reduce<kFinishBlock>(rmax, sv, true);
+  bool good = !f.bad[c];
+  if (good) {
+    const int eb = a.ent_off[c];
+    const double *jp = a.Jp + eb;
+    const int32_t *ecol = a.ent_col + eb, *ers = a.ent_rs + beg + c;
+    const double *v = f.v + beg;
+    double rm = 0.0, am = 0.0, xm = 0.0;
+    for (int i = t; i < n; i += kFinishBlock) {
+      double r = a.bp[beg + i], as = 0.0;
+      for (int j = ers[i]; j < ers[i + 1]; ++j) {
+        const double q = jp[j];
+        r = r - q * v[ecol[j]];
+        as = as + fabs(q);
+      }
+      rm = fmax(rm, fabs(r));
+      am = fmax(am, as);
+      xm = fmax(xm, fabs(v[i]));
+    }
+    rm = blk_reduce<kFinishBlock>(rm, sv, true);
+    am = blk_reduce<kFinishBlock>(am, sv, true);
+    xm = blk_reduce<kFinishBlock>(xm, sv, true);
+    good = rm <= a.max_berr * (am * xm + max_rhs);  // false for NaN / inf too
+  }
+  if (!good) {
+    // singular for the fronts' restricted pivoting: the dense partial-
+    // pivoting kernel launched next refactors the component when it has a
+    // dense form, and reports it singular otherwise
+    if (t == 0) {
+      stats[0] = max_rhs;
+      stats[1] = 0.0;
+      stats[2] = 1.0;
+      stats[3] = (double)n;
+      a.fallback[c] = 1;
+    }
+    return;
+  }
+  const int32_t *qcol = a.qcol + beg;
+  double acc = 0.0;
+  for (int i = t; i < n; i += kFinishBlock) {
+    const int g = qcol[i];
+    const double dx = f.v[beg + i];
+    const double w = a.reltol * fmax(fabs(a.wa[g]), fabs(a.wb[g])) + a.abstol;
+    const double q = dx / w;
+    acc = acc + q * q;
+    a.x[g] = a.x[g] + dx;
+  }
+  const double ss = blk_reduce<kFinishBlock>(acc, sv, false);
+  if (t == 0) {
+    stats[0] = max_rhs;
+    stats[1] = sqrt(ss / n);
+    stats[2] = 0.0;
+    stats[3] = (double)n;
+  }
+}
+
+template <int NB, int ROWS>
+hipError_t launch_level(const FrontArgs &f, int lv0, int nfr, hipStream_t s) {
+  hipLaunchKernelGGL((front_factor_kernel<NB, ROWS>), dim3(nfr), dim3(kFB), 0, s, f, lv0);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// The panel form for fronts of up to `max_m` rows: 0 when none fits.
+int front_form(int32_t max_m) {
+  if (max_m <= 64) return 1;
+  if (max_m <= 256) return 2;
+  if (max_m <= 1024) return 3;
+  if (max_m <= 1934) return 4;
+  if (max_m <= kFrontMaxM) return 5;
+  return 0;
+}
+
+hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t s) {
+  if (f.n_fronts == 0) return hipSuccess;
+  const unsigned gclear =
+      (unsigned)std::min<int64_t>((std::max(f.n_vals, (int64_t)f.n_comp) + kFB - 1) / kFB, 8192);
+  hipLaunchKernelGGL(front_clear_kernel, dim3(gclear), dim3(kFB), 0, s, f);
+  const unsigned gsc = (unsigned)std::min<int64_t>((f.n_ent + kFB - 1) / kFB + 1, 8192);
+  hipLaunchKernelGGL(front_scatter_kernel, dim3(gsc), dim3(kFB), 0, s, f, a.J);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int L = 0; L < f.n_levels; ++L) {
+    const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
+    switch (front_form(f.lv_maxm_host[L])) {
+      case 1: e = launch_level<16, 64>(f, lv0, nfr, s); break;
+      case 2: e = launch_level<16, 256>(f, lv0, nfr, s); break;
+      case 3: e = launch_level<16, 1024>(f, lv0, nfr, s); break;
+      case 4: e = launch_level<8, 1934>(f, lv0, nfr, s); break;
+      default: e = launch_level<4, kFrontMaxM>(f, lv0, nfr, s); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  for (int L = 0; L < f.n_levels; ++L) {
+    const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
+    hipLaunchKernelGGL(front_fwd_kernel, dim3(nfr), dim3(kFB), 0, s, f, (const double *)a.bp,
+                       lv0);
+  }
+  for (int L = f.n_levels - 1; L >= 0; --L) {
+    const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
+    hipLaunchKernelGGL(front_bwd_kernel, dim3(nfr), dim3(kFB), 0, s, f, lv0);
+  }
+  hipLaunchKernelGGL(front_finish_kernel, dim3(f.n_comp), dim3(kFinishBlock), 0, s, f, a);
+  return hipGetLastError();
+}
+
+}  // namespace xg

@@ -184,6 +184,9 @@ struct LoadPtrs {
   // keep what the last evaluation wrote)
   const int32_t *active = nullptr;
   const int32_t *comp = nullptr;
+  // optional: a device stop word; when it holds nonzero the launch does
+  // nothing (iterations after convergence inside a multi-iteration graph)
+  const int32_t *gate = nullptr;
 };
 
 __device__ __forceinline__ bool skip_slot(const LoadPtrs &p, int slot) {
@@ -200,7 +203,7 @@ hipError_t launch_mos1_eval(const Mos1Args &a, const SolverStateDev &ss,
 // +0.0 and entries without contributions set to +0.0.
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
                          const double *slots, double *partial, bool overwrite,
-                         hipStream_t s);
+                         hipStream_t s, const int32_t *gate = nullptr);
 // Newton-iteration assembly (N_TIA_OneStep.C:215-330).
 struct TiaAssembleArgs {
   int64_t n1 = 0, nnz = 0;
@@ -214,6 +217,7 @@ struct TiaAssembleArgs {
   // are skipped, as fmax skips them
   const int32_t *node_comp = nullptr;
   double *norms = nullptr;
+  const int32_t *gate = nullptr;  // stop word (LoadPtrs::gate)
 };
 
 // Dense LU solve per connected component (ground excluded).
@@ -228,6 +232,7 @@ struct DenseSolveArgs {
   const double *J, *rhs;
   double *x;                      // Newton iterate, updated in place
   const double *wa, *wb;          // weights: reltol * max(|wa|, |wb|) + abstol
+  const int32_t *gate = nullptr;  // stop word (LoadPtrs::gate)
   double reltol, abstol;
   const int32_t *active;          // n_comp flags or null (all)
   int32_t norm_only = 0;          // only stats[0] = max |rhs| per component
@@ -266,7 +271,26 @@ struct SparseLuArgs {
   double *val_work, *vec_work;    // factors / solution of global components
   double *stats;
   int32_t *fallback;              // n_comp: 1 = refactor with pivoting
+  const int32_t *front = nullptr; // n_comp: 1 = factored by dense fronts
+  const int32_t *gate = nullptr;  // stop word (LoadPtrs::gate)
 };
+
+// Dense fronts (multifrontal LU) of the components too large for task
+// lists; the plan is LuPlan's fr_* arrays (xg_setup.h).
+constexpr int kFrontMaxM = 3481;  // largest front a workgroup's panel holds
+struct FrontArgs {
+  int32_t n_fronts = 0, n_comp = 0, n_levels = 0;
+  int64_t n_vals = 0, n_ent = 0;
+  const int32_t *comp, *m, *k, *rows, *cbmap, *ch_ptr, *ch, *lv, *ent_k, *front_comp;
+  const int64_t *fo, *ro, *ent_fpos;
+  double *vals, *wrhs, *v;        // fronts, front right-hand sides, positions
+  int32_t *piv, *bad;             // row swaps (at ro), failed components
+  const int32_t *active;
+  const int32_t *gate = nullptr;  // stop word (LoadPtrs::gate)
+  const int32_t *lv_ptr_host, *lv_maxm_host;  // host arrays: the launch loop
+};
+int front_form(int32_t max_m);
+hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t s);
 
 hipError_t launch_tia_assemble(const TiaAssembleArgs &a, hipStream_t s);
 hipError_t launch_dense_solve(const DenseSolveArgs &a, hipStream_t s);
@@ -289,7 +313,7 @@ hipError_t launch_component_limited(const int32_t *flags, const int32_t *slot_co
                                     hipStream_t s);
 hipError_t launch_iteration_counts(const int32_t *flags, const int32_t *slot_comp,
                                    int32_t n, int32_t *comp_cnt, int64_t *out,
-                                   hipStream_t s);
+                                   hipStream_t s, const int32_t *gate = nullptr);
 hipError_t launch_count_flags(const int32_t *flags, int32_t n, int64_t *out,
                               hipStream_t s);
 
@@ -306,6 +330,8 @@ int handle_assemble_norms(xg_handle *h, int32_t transient, const double *qn0,
                           const double *JQ, double *rhs, double *J, double *norms);
 hipError_t handle_iteration_counts(xg_handle *h, int32_t *comp_cnt, int64_t *out128);
 hipStream_t handle_stream(const xg_handle *h);
+// The stop word the launches of the next enqueues check (nullptr: none).
+void handle_set_gate(xg_handle *h, const int32_t *gate);
 int32_t handle_n_comp(const xg_handle *h);
 int64_t handle_n_nodes(const xg_handle *h);
 void handle_sizes(const xg_handle *h, int64_t *n_state, int64_t *n_store, int64_t *nnz);

@@ -10,6 +10,7 @@ namespace {
 __global__ void __launch_bounds__(256)
     linear_eval_kernel(LinArgs a, SolverStateDev ss, LoadPtrs p, double time,
                        double bp_tol, const double *dstep) {
+  if (p.gate && *p.gate) return;
   lin::linear_eval_one(a, ss, p, time, bp_tol, dstep, blockIdx.x * 256 + threadIdx.x);
 }
 

@@ -155,7 +155,8 @@ __device__ __forceinline__ void run_level(const Lists &ls, int L, double *val,
 __global__ void __launch_bounds__(kBlock)
     permute_kernel(int32_t n_ent, int32_t n_nodes, const int32_t *ent_k,
                    const int32_t *prow, const double *J, const double *rhs,
-                   double *Jp, double *bp) {
+                   double *Jp, double *bp, const int32_t *gate) {
+  if (gate && *gate) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n_ent) Jp[i] = J[ent_k[i]];
   if (i < n_nodes) bp[i] = rhs[prow[i]];
@@ -167,6 +168,7 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
   __shared__ double sv[kBlock];
   const int c = blockIdx.x;
   const int t = threadIdx.x;
+  if (a.gate && *a.gate) return;
   if ((a.in_lds[c] != 0) != kLds) return;  // the other form
   const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
   const int64_t nv = a.voff[c + 1] - a.voff[c];
@@ -179,6 +181,10 @@ __global__ void __launch_bounds__(kBlock) sparse_lu_kernel(SparseLuArgs a) {
       stats[3] = 0.0;
       a.fallback[c] = 0;
     }
+    return;
+  }
+  if (a.front && a.front[c]) {  // the dense-front kernels solve it
+    if (t == 0) a.fallback[c] = 0;
     return;
   }
   if (!a.ok[c]) {  // structurally singular under matching: dense kernel
@@ -316,7 +322,7 @@ hipError_t launch_sparse_lu(const SparseLuArgs &a, hipStream_t s) {
   if (nperm > 0)
     hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((nperm + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, s, a.n_ent, a.n_nodes, a.ent_k, a.prow, a.J,
-                       a.rhs, a.Jp, a.bp);
+                       a.rhs, a.Jp, a.bp, a.gate);
   hipLaunchKernelGGL(sparse_lu_kernel<true>, dim3(a.n_comp), dim3(kBlock), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !a.any_global) return e;

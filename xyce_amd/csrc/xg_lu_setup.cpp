@@ -34,6 +34,8 @@
 #include <iterator>
 #include <queue>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <string>
 #include <new>
@@ -53,6 +55,12 @@ namespace {
 // entries and multiply-adds per factorisation.
 constexpr int64_t kLuMaxFill = 4 << 20;
 constexpr int64_t kLuMaxOps = 64 << 20;
+// Past those, the component is factored by dense fronts (multifrontal);
+// the fill they may hold (strictly-lower entries of L).
+constexpr int64_t kFrontMaxFill = (int64_t)1 << 28;
+// Element steps the minimum-degree order of a front component may take
+// (bounds its setup time to seconds).
+constexpr int64_t kFrontMdWork = (int64_t)4 << 30;
 
 using Graph = std::vector<std::vector<int32_t>>;
 
@@ -136,15 +144,27 @@ struct Dissector {
     int sep = lo;
     for (int l = lo; l <= hi; ++l)
       if (cnt[l] < cnt[sep]) sep = l;
-    std::vector<int32_t> a, b, s;
+    // sides: 0 below the separator level, 1 above, 2 on it.  A level
+    // vertex with no neighbour on one side joins the other (leaves hanging
+    // off the level, such as a device's input and branch nodes, need not
+    // separate anything); a move never joins the two sides, as a vertex
+    // moves only when it has no neighbour on the side it leaves open.
+    for (int32_t v : set) lev[v] = lev[v] < sep ? 0 : lev[v] > sep ? 1 : 2;
     for (int32_t v : set) {
-      if (lev[v] < sep)
-        a.push_back(v);
-      else if (lev[v] > sep)
-        b.push_back(v);
-      else
-        s.push_back(v);
+      if (lev[v] != 2) continue;
+      bool below = false, above = false;
+      for (int32_t u : g[v])
+        if (in[u] == tag) {
+          below |= lev[u] == 0;
+          above |= lev[u] == 1;
+        }
+      if (!above)
+        lev[v] = 0;
+      else if (!below)
+        lev[v] = 1;
     }
+    std::vector<int32_t> a, b, s;
+    for (int32_t v : set) (lev[v] == 0 ? a : lev[v] == 1 ? b : s).push_back(v);
     const int32_t ta = next_tag++, tb = next_tag++;
     for (int32_t v : a) in[v] = ta;
     for (int32_t v : b) in[v] = tb;
@@ -280,10 +300,14 @@ bool min_degree_order(const Graph &g, const std::vector<int32_t> &last,
 // Strictly-lower fill and elimination depth (the longest chain of pivots
 // that depend on each other through L) of an order, by the row-subtree
 // walk of the factorisation below; false when the fill exceeds the budget.
+// With `flops` given, also the factorisation's floating-point operations
+// (per pivot j with c_j entries below it: c_j divisions and 2 c_j^2 for
+// the update), and the fill may reach `cap`.
 bool order_cost(const Graph &sym, const std::vector<int32_t> &order, int64_t &fill,
-                int32_t &depth) {
+                int32_t &depth, int64_t *flops = nullptr, int64_t cap = kLuMaxFill) {
   const int32_t n = (int32_t)sym.size();
   std::vector<int32_t> pinv(n), parent(n, -1), mark(n, -1), lev(n, 0);
+  std::vector<int64_t> cc(flops ? n : 0, 0);
   for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
   fill = 0;
   depth = 0;
@@ -294,6 +318,7 @@ bool order_cost(const Graph &sym, const std::vector<int32_t> &order, int64_t &fi
       if (j >= i) continue;
       while (mark[j] != i) {
         ++fill;
+        if (flops) cc[j]++;
         lev[i] = std::max(lev[i], lev[j] + 1);
         mark[j] = i;
         if (parent[j] < 0) {
@@ -304,7 +329,11 @@ bool order_cost(const Graph &sym, const std::vector<int32_t> &order, int64_t &fi
       }
     }
     depth = std::max(depth, lev[i]);
-    if (fill > kLuMaxFill) return false;
+    if (fill > cap) return false;
+  }
+  if (flops) {
+    *flops = 0;
+    for (int64_t c : cc) *flops += c + 2 * c * c;
   }
   return true;
 }
@@ -317,16 +346,139 @@ constexpr int32_t kMinDegreeMaxN = 100000;
 // component (lu_analyse appends the components in order).  The component's
 // entries of prow / qcol / qpos / lu_pos (disjoint from every other
 // component's) are written in place.
+// Dense fronts of one component (local positions and front ids).
+struct FrontLocal {
+  std::vector<int32_t> m, k, height;   // per front
+  std::vector<int64_t> fo;             // value offsets (n_fronts + 1)
+  std::vector<int64_t> ro;             // row-list offsets (n_fronts + 1)
+  std::vector<int32_t> rows;           // front rows, ascending positions
+  std::vector<int32_t> cbmap;          // at ro[s] + t, t >= k: row's index in the parent
+  std::vector<int32_t> ch_ptr, ch;     // children, ascending
+  std::vector<int32_t> ent_k;          // CSR entries of the component
+  std::vector<int64_t> ent_fpos;       // their value positions
+  int64_t flops = 0;
+  int32_t max_m = 0;
+};
+
 struct CompLu {
   int rc = XG_OK;
   std::string err;
-  bool ok = false, in_lds = true;
+  bool ok = false, in_lds = true, fronts = false;
+  FrontLocal fr;
   int64_t fn = 0, factor_ops = 0;
   int nlev[3] = {0, 0, 0};
   std::vector<int32_t> t_dst, t_div, ops;
   std::vector<int32_t> t_end;                    // per task: end of its ops (pairs)
   std::vector<int32_t> lv_end, lv_mid, lv_long;  // per level, task indices
 };
+
+// Supernodal multifrontal plan from the elimination tree `parent` and the
+// strictly-lower column structures `ucol` (ascending) of a component in
+// pivot positions.  Supernodes: column j joins j - 1 when j is the parent
+// of j - 1 and L(:, j-1) = {j} + L(:, j) (other children of j then hang off
+// the supernode: their contribution rows lie in its rows).  Front s covers
+// columns [f, l) with rows R = [f, l) + L(:, l-1) (ascending); its dense
+// m x m block (row-major) receives the matrix entries whose smaller
+// position falls in [f, l), and the contribution blocks of its children.
+// Entry e is (ei[e], ej[e]) in positions.
+bool build_fronts(int32_t n, const std::vector<int32_t> &parent,
+                  const std::vector<std::vector<int32_t>> &ucol,
+                  const std::vector<int32_t> &ei, const std::vector<int32_t> &ej,
+                  FrontLocal &F, std::string &err) {
+  std::vector<int32_t> nchild(n, 0);
+  for (int32_t j = 0; j < n; ++j)
+    if (parent[j] >= 0) nchild[parent[j]]++;
+  std::vector<int32_t> first;  // supernode starts
+  std::vector<int32_t> sn_of(n);
+  for (int32_t j = 0; j < n; ++j) {
+    const bool join = j > 0 && parent[j - 1] == j &&
+                      ucol[j - 1].size() == ucol[j].size() + 1;
+    if (!join) first.push_back(j);
+    sn_of[j] = (int32_t)first.size() - 1;
+  }
+  const int32_t ns = (int32_t)first.size();
+  first.push_back(n);
+  F.m.resize(ns);
+  F.k.resize(ns);
+  F.height.assign(ns, 0);
+  F.fo.assign(ns + 1, 0);
+  F.ro.assign(ns + 1, 0);
+  std::vector<int32_t> spar(ns, -1);
+  for (int32_t s = 0; s < ns; ++s) {
+    const int32_t f = first[s], l = first[s + 1];
+    const int32_t k = l - f, m = k + (int32_t)ucol[l - 1].size();
+    F.k[s] = k;
+    F.m[s] = m;
+    F.max_m = std::max(F.max_m, m);
+    F.fo[s + 1] = F.fo[s] + (int64_t)m * m;
+    F.ro[s + 1] = F.ro[s] + m;
+    if (parent[l - 1] >= 0) spar[s] = sn_of[parent[l - 1]];
+    // flops of the partial factorisation: per pivot a column scaling and a
+    // rank-1 update of the trailing block
+    for (int64_t j = 0; j < k; ++j) F.flops += (m - j - 1) + 2 * (m - j - 1) * (m - j - 1);
+  }
+  F.rows.resize(F.ro[ns]);
+  F.cbmap.assign(F.ro[ns], -1);
+  for (int32_t s = 0; s < ns; ++s) {
+    int32_t *r = F.rows.data() + F.ro[s];
+    const int32_t f = first[s], l = first[s + 1];
+    for (int32_t j = f; j < l; ++j) *r++ = j;
+    for (int32_t i : ucol[l - 1]) *r++ = i;
+  }
+  auto local = [&](int32_t s, int32_t pos) -> int32_t {
+    const int32_t *b = F.rows.data() + F.ro[s], *e = b + F.m[s];
+    const int32_t *it = std::lower_bound(b, e, pos);
+    return (it != e && *it == pos) ? (int32_t)(it - b) : -1;
+  };
+  F.ch_ptr.assign(ns + 1, 0);
+  for (int32_t s = 0; s < ns; ++s)
+    if (spar[s] >= 0) F.ch_ptr[spar[s] + 1]++;
+  for (int32_t s = 0; s < ns; ++s) F.ch_ptr[s + 1] += F.ch_ptr[s];
+  F.ch.resize(F.ch_ptr[ns]);
+  {
+    std::vector<int32_t> fillp(F.ch_ptr.begin(), F.ch_ptr.end() - 1);
+    for (int32_t s = 0; s < ns; ++s)
+      if (spar[s] >= 0) F.ch[fillp[spar[s]]++] = s;
+  }
+  for (int32_t s = 0; s < ns; ++s) {
+    const int32_t p = spar[s];
+    if (p < 0) continue;
+    F.height[p] = std::max(F.height[p], F.height[s] + 1);
+    for (int32_t t = F.k[s]; t < F.m[s]; ++t) {
+      const int32_t q = local(p, F.rows[F.ro[s] + t]);
+      if (q < 0) {
+        err = "sparse LU: a contribution row is missing from its parent front";
+        return false;
+      }
+      F.cbmap[F.ro[s] + t] = q;
+    }
+  }
+  if (std::getenv("XG_LU_DEBUG")) {
+    int32_t hmax = 0;
+    for (int32_t h : F.height) hmax = std::max(hmax, h);
+    std::vector<int64_t> cnt(hmax + 1, 0), kk(hmax + 1, 0), mm(hmax + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) {
+      cnt[F.height[s]]++;
+      kk[F.height[s]] += F.k[s];
+      mm[F.height[s]] = std::max<int64_t>(mm[F.height[s]], F.m[s]);
+    }
+    for (int32_t h = 0; h <= hmax; h += std::max(1, hmax / 40))
+      std::fprintf(stderr, "  level %d: fronts %lld, columns %lld, max m %lld\n", h,
+                   (long long)cnt[h], (long long)kk[h], (long long)mm[h]);
+  }
+  F.ent_fpos.resize(ei.size());
+  for (size_t e = 0; e < ei.size(); ++e) {
+    const int32_t i = ei[e], j = ej[e];
+    const int32_t s = sn_of[std::min(i, j)];
+    const int32_t li = local(s, i), lj = local(s, j);
+    if (li < 0 || lj < 0) {
+      err = "sparse LU: a matrix entry falls outside its front";
+      return false;
+    }
+    F.ent_fpos[e] = F.fo[s] + (int64_t)li * F.m[s] + lj;
+  }
+  return true;
+}
 
 void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
                        const std::vector<std::pair<int32_t, int32_t>> &pairs, CompLu &out) {
@@ -403,6 +555,28 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
       if (ok_md && (!ok_nd || d_md < d_nd || (d_md == d_nd && f_md < f_nd))) order.swap(md);
     }
   }
+  // A component beyond the task lists' budgets is factored by dense
+  // fronts, whose cost is their floating-point work: the minimum-degree
+  // order with the fronts' budgets replaces the dissection when it needs
+  // fewer operations.
+  {
+    int64_t f0 = 0, w0 = 0;
+    int32_t d0 = 0;
+    const bool fits = order_cost(sym, order, f0, d0, &w0, kFrontMaxFill);
+    if (!fits || f0 > kLuMaxFill || (w0 - f0) / 2 > kLuMaxOps || plan.lu_fronts) {
+      std::vector<int32_t> md;
+      int64_t f1 = 0, w1 = 0;
+      int32_t d1 = 0;
+      // (its explicit elimination graph grows too large on big meshes)
+      if (n <= kMinDegreeMaxN &&
+          min_degree_order(sym, hubs, md, 2 * kFrontMaxFill, kFrontMdWork) &&
+          order_cost(sym, md, f1, d1, &w1, kFrontMaxFill) && (!fits || w1 < w0))
+        order.swap(md);
+      if (std::getenv("XG_LU_DEBUG"))
+        std::fprintf(stderr, "lu: n %d nd fill %lld flops %.3g depth %d | md fill %lld flops %.3g depth %d\n",
+                     n, (long long)f0, (double)w0, d0, (long long)f1, (double)w1, d1);
+    }
+  }
   std::vector<int32_t> pinv(n);
   for (int32_t i = 0; i < n; ++i) pinv[order[i]] = i;
   for (int32_t i = 0; i < n; ++i) {
@@ -435,16 +609,43 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
     std::sort(lrow[i].begin(), lrow[i].end());
     for (int32_t j : lrow[i]) ucol[j].push_back(i);
     fill += (int64_t)lrow[i].size();
-    too_big = fill > kLuMaxFill;
+    too_big = fill > kFrontMaxFill;
   }
-  if (!too_big) {
-    int64_t work = 0;  // multiply-adds: sum over pivots of (column count)^2
-    for (int32_t k = 0; k < n; ++k) work += (int64_t)ucol[k].size() * (int64_t)ucol[k].size();
-    too_big = work > kLuMaxOps;
-  }
-  if (too_big) {
+  if (too_big) {  // beyond the fronts' budget too: no factorisation
     out.in_lds = false;
     return;
+  }
+  {
+    int64_t work = 0;  // multiply-adds: sum over pivots of (column count)^2
+    for (int32_t k = 0; k < n; ++k) work += (int64_t)ucol[k].size() * (int64_t)ucol[k].size();
+    if (fill > kLuMaxFill || work > kLuMaxOps || plan.lu_fronts) {
+      // too large for one workgroup's task lists: dense fronts
+      std::vector<int32_t> rpos(n);
+      for (int32_t i = 0; i < n; ++i) rpos[row_of_col[order[i]]] = i;
+      std::vector<int32_t> ei, ej;
+      for (int32_t r = 0; r < n; ++r) {
+        const int32_t g = nodes[r];
+        for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
+          const int32_t col = plan.colidx[k];
+          if (col >= G) continue;
+          out.fr.ent_k.push_back(k);
+          ei.push_back(rpos[r]);
+          ej.push_back(pinv[loc(col)]);
+        }
+      }
+      if (!build_fronts(n, parent, ucol, ei, ej, out.fr, out.err)) {
+        out.rc = XG_E_ARG;
+        return;
+      }
+      if (front_form(out.fr.max_m) == 0) {  // no panel holds its largest front
+        out.fr = FrontLocal();
+        out.in_lds = false;
+        return;
+      }
+      out.fronts = true;
+      out.in_lds = false;
+      return;
+    }
   }
   // filled row-major pattern: row i = lrow[i], i, ucol[i] (all sorted)
   std::vector<int64_t> rp(n + 1, 0);
@@ -696,11 +897,49 @@ int lu_analyse(Plan &plan, std::string &err) {
     work();  // catches inside, so the pool is always joined below
     for (auto &th : pool) th.join();
   }
+  L.front_comp.assign(nc, 0);
+  L.fr_fo.assign(1, 0);
+  L.fr_ro.assign(1, 0);
+  L.fr_ch_ptr.assign(1, 0);
+  std::vector<int32_t> fr_height;
   for (int32_t c = 0; c < nc; ++c) {
     CompLu &r = res[c];
     if (r.rc != XG_OK) {
       err = r.err;
       return r.rc;
+    }
+    if (r.fronts) {
+      // dense fronts: appended with global front ids, positions and values
+      FrontLocal &F = r.fr;
+      const int32_t s0 = (int32_t)L.fr_m.size(), ns = (int32_t)F.m.size();
+      const int64_t v0 = L.fr_fo.back(), r0 = L.fr_ro.back();
+      const int32_t beg = plan.comp_start[c];
+      L.front_comp[c] = 1;
+      L.n_front_comp++;
+      L.in_lds[c] = 0;
+      for (int32_t s = 0; s < ns; ++s) {
+        L.fr_comp.push_back(c);
+        L.fr_m.push_back(F.m[s]);
+        L.fr_k.push_back(F.k[s]);
+        L.fr_fo.push_back(v0 + F.fo[s + 1]);
+        L.fr_ro.push_back(r0 + F.ro[s + 1]);
+        L.fr_ch_ptr.push_back(L.fr_ch_ptr.back() + F.ch_ptr[s + 1] - F.ch_ptr[s]);
+        fr_height.push_back(F.height[s]);
+      }
+      for (int32_t v : F.rows) L.fr_rows.push_back(beg + v);
+      L.fr_cbmap.insert(L.fr_cbmap.end(), F.cbmap.begin(), F.cbmap.end());
+      for (int32_t ch : F.ch) L.fr_ch.push_back(s0 + ch);
+      for (size_t e = 0; e < F.ent_k.size(); ++e) {
+        L.fr_ent_k.push_back(F.ent_k[e]);
+        L.fr_ent_fpos.push_back(v0 + F.ent_fpos[e]);
+      }
+      L.fr_flops += F.flops;
+      L.fr_max_m = std::max(L.fr_max_m, F.max_m);
+      L.voff[c + 1] = L.voff[c];
+      for (int ph = 0; ph < 3; ++ph)
+        L.lv_comp[(size_t)3 * c + ph + 1] = (int32_t)L.lv_ptr.size() - 1;
+      r = CompLu();
+      continue;
     }
     if (!r.ok) {
       L.n_static_fail++;
@@ -737,6 +976,22 @@ int lu_analyse(Plan &plan, std::string &err) {
     L.max_factor_levels = std::max(L.max_factor_levels, r.nlev[0]);
     L.max_solve_levels = std::max(L.max_solve_levels, r.nlev[1] + r.nlev[2]);
     r = CompLu();  // release the component's lists
+  }
+  // front levels: heights from the leaves, processed bottom-up
+  {
+    int32_t nl = 0;
+    for (int32_t h : fr_height) nl = std::max(nl, h + 1);
+    L.fr_lv_ptr.assign(nl + 1, 0);
+    for (int32_t h : fr_height) L.fr_lv_ptr[h + 1]++;
+    for (int32_t l = 0; l < nl; ++l) L.fr_lv_ptr[l + 1] += L.fr_lv_ptr[l];
+    L.fr_lv.resize(fr_height.size());
+    L.fr_lv_maxm.assign(nl, 0);
+    std::vector<int32_t> at(L.fr_lv_ptr.begin(), L.fr_lv_ptr.end() - 1);
+    for (int32_t s = 0; s < (int32_t)fr_height.size(); ++s) {
+      const int32_t h = fr_height[s];
+      L.fr_lv[at[h]++] = s;
+      L.fr_lv_maxm[h] = std::max(L.fr_lv_maxm[h], L.fr_m[s]);
+    }
   }
   // entry lists (every component, with or without a static factorisation)
   for (int32_t c = 0; c < nc; ++c) {

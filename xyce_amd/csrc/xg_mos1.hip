@@ -105,6 +105,7 @@ __device__ __forceinline__ void junction_charge(
 __global__ void __launch_bounds__(kBlock)
     mos1_eval_kernel(Mos1Args a, SolverStateDev ss, LoadPtrs p) {
   __shared__ double card_lds[kWavesPerBlock][M1C_N];
+  if (p.gate && *p.gate) return;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int slot = blockIdx.x * kBlock + tid;

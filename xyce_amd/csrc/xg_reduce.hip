@@ -46,6 +46,7 @@ struct ReduceBatch {
   ReduceDev prog[kMaxProgs];
   double *out[kMaxProgs];
   double *partial;  // scratch for chunk partial sums
+  const int32_t *gate;  // stop word: nonzero skips the launch
 };
 
 __device__ __forceinline__ int find_prog(const int32_t *start, int n, int w) {
@@ -85,6 +86,7 @@ __device__ __forceinline__ double fold(double acc, const int32_t *__restrict__ s
 // does one kind of work.
 __global__ void __launch_bounds__(kBlock)
     fold_kernel(ReduceBatch b, const double *__restrict__ slots) {
+  if (b.gate && *b.gate) return;
   int w = blockIdx.x * kBlock + threadIdx.x;
   const int n_short4 = b.s4start[b.n];
   const int n_short = b.sstart[b.n], n_chunk = b.cstart[b.n];
@@ -157,6 +159,7 @@ __global__ void __launch_bounds__(kBlock)
 // Second pass: one wavefront per multi-chunk segment.
 __global__ void __launch_bounds__(kBlock)
     combine_kernel(ReduceBatch b) {
+  if (b.gate && *b.gate) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int w = blockIdx.x * kWaves + wave;
   if (w >= b.mstart[b.n]) return;
@@ -217,8 +220,9 @@ __global__ void __launch_bounds__(256)
 // cnt zeroed beforehand).  Grid: 64 blocks.
 __global__ void __launch_bounds__(256)
     iteration_count_kernel(const int32_t *flags, const int32_t *slot_comp, int32_t n,
-                           int32_t *cnt, int64_t *out) {
+                           int32_t *cnt, int64_t *out, const int32_t *gate) {
   __shared__ int64_t c2[2][256];
+  if (gate && *gate) return;
   int64_t c0 = 0, c1 = 0;
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
@@ -277,10 +281,11 @@ hipError_t launch_component_limited(const int32_t *flags, const int32_t *slot_co
 
 hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
                          const double *slots, double *partial, bool overwrite,
-                         hipStream_t s) {
+                         hipStream_t s, const int32_t *gate) {
   if (n_out <= 0 || n_out > kMaxProgs) return hipErrorInvalidValue;
   ReduceBatch b{};
   b.n = n_out;
+  b.gate = gate;
   b.overwrite = overwrite ? 1 : 0;
   b.partial = partial;
   for (int k = 0; k < n_out; ++k) {
@@ -319,9 +324,9 @@ hipError_t launch_reduce(int n_out, const ReduceDev *progs, double *const *outs,
 // non-finite in out[64..127]); the caller sums them.
 hipError_t launch_iteration_counts(const int32_t *flags, const int32_t *slot_comp,
                                    int32_t n, int32_t *comp_cnt, int64_t *out,
-                                   hipStream_t s) {
+                                   hipStream_t s, const int32_t *gate) {
   hipLaunchKernelGGL(iteration_count_kernel, dim3(64), dim3(256), 0, s, flags, slot_comp,
-                     n, comp_cnt, out);
+                     n, comp_cnt, out, gate);
   return hipGetLastError();
 }
 

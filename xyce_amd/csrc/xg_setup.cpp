@@ -951,6 +951,7 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     pbase += (int32_t)prog->cdst.size();
   }
   plan.n_partials = pbase;
+  plan.lu_fronts = c.opt.lu_fronts;
   return lu_analyse(plan, err);
 }
 

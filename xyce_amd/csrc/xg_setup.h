@@ -26,6 +26,7 @@ struct Options {
   int limiter = 1;
   double temp = 300.15;
   double tnom = 300.15;
+  int lu_fronts = 0;  // 1: every component through dense fronts
 };
 
 // Processed MOSFET1 model (after processParams).
@@ -132,6 +133,20 @@ struct LuPlan {
   std::vector<int32_t> ops;           // pairs (a, b)
   int64_t fnnz = 0, factor_ops = 0;
   int32_t max_factor_levels = 0, max_solve_levels = 0, n_static_fail = 0;
+  // components too large for task lists: supernodal multifrontal LU on
+  // dense fronts (every front component together, global ids)
+  std::vector<int32_t> front_comp;    // n_comp: 1 = factored by fronts
+  std::vector<int32_t> fr_comp, fr_m, fr_k;  // per front
+  std::vector<int64_t> fr_fo, fr_ro;  // value / row-list offsets (n_fronts + 1)
+  std::vector<int32_t> fr_rows;       // front rows: global positions, ascending
+  std::vector<int32_t> fr_cbmap;      // at fr_ro[s] + t (t >= k): index in parent
+  std::vector<int32_t> fr_ch_ptr, fr_ch;      // children, ascending
+  std::vector<int32_t> fr_lv_ptr, fr_lv;      // fronts by height (bottom-up)
+  std::vector<int32_t> fr_lv_maxm;            // per level: largest front
+  std::vector<int32_t> fr_ent_k;              // CSR entry -> front value
+  std::vector<int64_t> fr_ent_fpos;
+  int64_t fr_flops = 0;
+  int32_t fr_max_m = 0, n_front_comp = 0;
 };
 
 // A device-ready plan (host copies of everything that gets uploaded).
@@ -159,6 +174,7 @@ struct Plan {
   ReduceProgram red_mat[2];  // dFdx, dQdx
   int32_t n_partials = 0;    // chunk partial sums over all six programs
   int32_t limiter = 1;
+  int32_t lu_fronts = 0;  // Options::lu_fronts
   double gmin = 1e-12;
   // connected components of the circuit graph without ground (independent
   // sub-circuits), for the per-component dense solve

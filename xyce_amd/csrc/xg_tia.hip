@@ -28,6 +28,7 @@ constexpr int kLdsN = 128;  // 128 x 128 doubles = 128 KB of the 160 KB LDS
 
 __global__ void __launch_bounds__(kBlock)
     assemble_kernel(TiaAssembleArgs a) {
+  if (a.gate && *a.gate) return;
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double inv_dt = a.inv_dt;
   if (a.dstep && a.transient) {  // step size and order from device memory
@@ -394,6 +395,7 @@ __global__ void __launch_bounds__(kDenseBlock)
   __shared__ int spiv[16];
   const int c = blockIdx.x;
   const int t = threadIdx.x;
+  if (a.gate && *a.gate) return;
   const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
   double *stats = a.stats + 4 * (size_t)c;
   if ((kLds && n > kLdsN) || (!kLds && n <= kLdsN)) return;  // other variant

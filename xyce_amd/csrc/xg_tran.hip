@@ -30,11 +30,18 @@ struct xg_tran {
   double t = 0.0, bp_tol = 0.0;
   int64_t n_steps = 0, newton_total = 0, step_failures = 0;
   int32_t use_graphs = 1;
-  // graphs: 0 DC iteration, 1 first-step iteration, 2 later-step iteration,
-  // 3 / 4 a whole first / later step's first load, first iteration and
-  // (when that converged) completion
-  hipGraphExec_t exec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  hipGraph_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // graphs by (variant, iterations): variant 0 DC iteration, 1 first-step
+  // iteration, 2 later-step iteration, 3 / 4 a whole first / later step's
+  // first load, its first iterations and (when they converged) completion.
+  // Iterations after convergence inside one graph skip every launch (the
+  // stop word), so a graph of K iterations costs ~K only when all are used.
+  static constexpr int kMaxK = 4;
+  hipGraphExec_t exec[5][kMaxK + 1] = {};
+  hipGraph_t graph[5][kMaxK + 1] = {};
+  // iterations the next step's graph holds: those of the last step
+  // (adapted per step; no counterpart in the reference)
+  int32_t k_step = 1;
+  int64_t host_reads = 0;  // control-word reads (host round trips)
   int32_t *ctl_host = nullptr;  // pinned copy of the control words
   std::string err;
 };
@@ -55,7 +62,10 @@ constexpr NewtonParams kTran{1e-2, 1e-6, 1e-2, 0.33, 1e-6, 20};
 
 // ctl words: [0] iterations, [1] done, [2] failure (1 singular Jacobian,
 // 2 non-finite device values, 3 no convergence), [3] unconverged count
-enum { kIter = 0, kDone = 1, kFail = 2, kLeft = 3, kCtlWords = 8 };
+// [4] stop: done or failed -- every launch of an iteration checks it
+enum { kIter = 0, kDone = 1, kFail = 2, kLeft = 3, kStop = 4, kCtlWords = 8 };
+// iterations per graph of the DC and the continuing-iteration loops
+constexpr int kDcIters = 4, kTranIters = 2;
 
 // Step start: predictor (the rounding sequence of the predictor kernel),
 // iterate = predictor, step parameters, control words and masks reset.
@@ -122,6 +132,7 @@ __global__ void __launch_bounds__(kBlock)
                       int32_t *active, int32_t *ctl) {
   __shared__ int32_t s_left, s_sing;
   const int t = threadIdx.x;
+  if (ctl[kStop]) return;  // an iteration after convergence or failure
   if (t == 0) {
     s_left = 0;
     s_sing = 0;
@@ -159,6 +170,7 @@ __global__ void __launch_bounds__(kBlock)
       ctl[kDone] = 1;
     else if (it >= prm.maxit)
       ctl[kFail] = 3;
+    if (ctl[kFail] || ctl[kDone]) ctl[kStop] = 1;
   }
 }
 
@@ -276,24 +288,36 @@ int enqueue_complete(xg_tran *T, int dc, const int32_t *ctl) {
   return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
 }
 
-// Variants 0-2: one Newton iteration; 3 / 4: a first / later step's first
-// load and assembly, its first iteration, and the completion if that
-// iteration converged (one launch for the common one-iteration step).
-int enqueue_variant(xg_tran *T, int variant) {
-  if (variant < 3) return enqueue_iteration(T, variant);
+// Variants 0-2: K Newton iterations; 3 / 4: a first / later step's first
+// load and assembly, its first K iterations, and the completion if they
+// converged (one launch for a step that converges within K).  Every launch
+// checks the stop word, so the iterations after convergence do nothing.
+int enqueue_body(xg_tran *T, int variant, int K) {
+  if (variant < 3) {
+    for (int k = 0; k < K; ++k)
+      if (int rc = enqueue_iteration(T, variant)) return rc;
+    return XG_OK;
+  }
   const int first = variant == 3;
   int rc = enqueue_first_load(T, 0, first);
   if (rc) return rc;
-  rc = enqueue_iteration(T, first ? 1 : 2);
-  if (rc) return rc;
+  for (int k = 0; k < K; ++k)
+    if ((rc = enqueue_iteration(T, first ? 1 : 2))) return rc;
   return enqueue_complete(T, 0, T->b.ctl);
 }
 
-int capture_variant(xg_tran *T, int variant) {
-  if (T->exec[variant]) return XG_OK;
+int enqueue_variant(xg_tran *T, int variant, int K) {
+  xg::handle_set_gate(T->h, T->b.ctl + xg::kStop);
+  const int rc = enqueue_body(T, variant, K);
+  xg::handle_set_gate(T->h, nullptr);
+  return rc;
+}
+
+int capture_variant(xg_tran *T, int variant, int K) {
+  if (T->exec[variant][K]) return XG_OK;
   if (hipStreamBeginCapture(T->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return tfail(T, XG_E_HIP, "graph capture begin");
-  const int rc = enqueue_variant(T, variant);
+  const int rc = enqueue_variant(T, variant, K);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(T->stream, &g);
   if (rc) {
@@ -301,20 +325,21 @@ int capture_variant(xg_tran *T, int variant) {
     return rc;
   }
   if (e != hipSuccess) return tfail(T, XG_E_HIP, "graph capture end");
-  T->graph[variant] = g;
-  if (hipGraphInstantiate(&T->exec[variant], g, nullptr, nullptr, 0) != hipSuccess)
+  T->graph[variant][K] = g;
+  if (hipGraphInstantiate(&T->exec[variant][K], g, nullptr, nullptr, 0) != hipSuccess)
     return tfail(T, XG_E_HIP, "graph instantiate");
   return XG_OK;
 }
 
-int launch_iteration(xg_tran *T, int variant) {
-  if (!T->use_graphs) return enqueue_variant(T, variant);
-  const int rc = capture_variant(T, variant);
+int launch_iteration(xg_tran *T, int variant, int K) {
+  if (!T->use_graphs) return enqueue_variant(T, variant, K);
+  const int rc = capture_variant(T, variant, K);
   if (rc) return rc;
-  return hipGraphLaunch(T->exec[variant], T->stream) == hipSuccess ? XG_OK : XG_E_HIP;
+  return hipGraphLaunch(T->exec[variant][K], T->stream) == hipSuccess ? XG_OK : XG_E_HIP;
 }
 
 int read_ctl(xg_tran *T) {
+  T->host_reads++;
   if (hipMemcpyAsync(T->ctl_host, T->b.ctl, sizeof(int32_t) * xg::kCtlWords,
                      hipMemcpyDeviceToHost, T->stream) != hipSuccess ||
       hipStreamSynchronize(T->stream) != hipSuccess)
@@ -324,10 +349,13 @@ int read_ctl(xg_tran *T) {
 
 // Newton iterations until convergence: returns 0 converged, >0 the failure
 // word, <0 an error code.  `first_variant`, if not negative, is launched
-// first instead of a plain iteration (the fused step graph).
-int newton_loop(xg_tran *T, int variant, int32_t *iters, int first_variant = -1) {
-  for (int k = 0;; ++k) {
-    int rc = launch_iteration(T, k == 0 && first_variant >= 0 ? first_variant : variant);
+// first with `first_k` iterations (the fused step graph); then graphs of
+// `k` iterations follow, the control words read once per graph.
+int newton_loop(xg_tran *T, int variant, int k, int32_t *iters, int first_variant = -1,
+                int first_k = 1) {
+  for (int j = 0;; ++j) {
+    const bool fused = j == 0 && first_variant >= 0;
+    int rc = launch_iteration(T, fused ? first_variant : variant, fused ? first_k : k);
     if (rc) return rc < 0 ? rc : XG_E_HIP;
     rc = read_ctl(T);
     if (rc) return rc;
@@ -388,13 +416,16 @@ int xg_tran_create(xg_handle *h, const xg_tran_buffers *b, double bp_tol,
   // every graph is captured here, so that the run itself makes no capture:
   // a capture must not overlap another thread's device allocations or
   // synchronous copies, and creation is where a caller can serialise those
-  for (int v = 0; v < 5; ++v) {
-    const int rc = capture_variant(T, v);
-    if (rc) {
-      xg_tran_destroy(T);
-      return rc;
+  for (int v = 0; v < 5; ++v)
+    for (int K = 1; K <= xg_tran::kMaxK; ++K) {
+      const bool need = v >= 3 || K == (v == 0 ? xg::kDcIters : xg::kTranIters);
+      if (!need) continue;
+      const int rc = capture_variant(T, v, K);
+      if (rc) {
+        xg_tran_destroy(T);
+        return rc;
+      }
     }
-  }
   *out = T;
   return XG_OK;
 }
@@ -417,7 +448,7 @@ int xg_tran_dcop(xg_tran *T) {
   int rc = enqueue_first_load(T, 1, 0);
   if (rc) return rc;
   int32_t iters = 0;
-  rc = newton_loop(T, 0, &iters);
+  rc = newton_loop(T, 0, xg::kDcIters, &iters);
   T->newton_total += iters;
   if (rc) return rc;
   rc = complete(T, 1);
@@ -463,9 +494,11 @@ int xg_tran_step(xg_tran *T, double dt) {
                      (const double *)b.xh1, (const double *)b.xh2, b.xpred, b.x, b.step,
                      t_next, T->bp_tol, 1.0 / dt, b.conv, b.active, b.ctl);
   if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "step begin");
-  // first load, first iteration and (if converged) completion in one launch
+  // first load, the first k_step iterations and (if converged) completion
+  // in one launch; k_step follows the iterations the last step took
   int32_t iters = 0;
-  int rc = newton_loop(T, first ? 1 : 2, &iters, first ? 3 : 4);
+  const int k_first = T->use_graphs ? T->k_step : 1;
+  int rc = newton_loop(T, first ? 1 : 2, xg::kTranIters, &iters, first ? 3 : 4, k_first);
   T->newton_total += iters;
   if (rc < 0) return rc;
   if (rc > 0) {  // no convergence: two half steps
@@ -475,7 +508,8 @@ int xg_tran_step(xg_tran *T, double dt) {
     if (rc) return rc;
     return xg_tran_step(T, dt / 2);
   }
-  if (iters > 1) {  // converged after the fused graph: complete now
+  T->k_step = std::max(1, std::min<int>(iters, xg_tran::kMaxK));
+  if (iters > k_first) {  // converged after the fused graph: complete now
     rc = complete(T, 0);
     if (rc) return rc;
   }
@@ -504,6 +538,15 @@ int xg_tran_info(xg_tran *T, double *t, int64_t *counters) {
   return XG_OK;
 }
 
+int xg_tran_counters(xg_tran *T, int64_t *out) {
+  if (!T || !out) return XG_E_ARG;
+  out[0] = T->host_reads;
+  out[1] = T->n_steps;
+  out[2] = T->newton_total;
+  out[3] = T->k_step;
+  return XG_OK;
+}
+
 int xg_tran_restore(xg_tran *T, double time, int64_t n_steps, int64_t newton_total,
                     int64_t step_failures) {
   if (!T || n_steps < 0) return XG_E_ARG;
@@ -522,10 +565,11 @@ int xg_tran_sync(xg_tran *T) {
 int xg_tran_destroy(xg_tran *T) {
   if (!T) return XG_OK;
   (void)hipStreamSynchronize(T->stream);
-  for (int k = 0; k < 5; ++k) {
-    if (T->exec[k]) (void)hipGraphExecDestroy(T->exec[k]);
-    if (T->graph[k]) (void)hipGraphDestroy(T->graph[k]);
-  }
+  for (int v = 0; v < 5; ++v)
+    for (int K = 0; K <= xg_tran::kMaxK; ++K) {
+      if (T->exec[v][K]) (void)hipGraphExecDestroy(T->exec[v][K]);
+      if (T->graph[v][K]) (void)hipGraphDestroy(T->graph[v][K]);
+    }
   xg_set_step_source(T->h, nullptr);
   xg_set_active_mask(T->h, nullptr);
   xg_set_stream(T->h, T->old_stream);

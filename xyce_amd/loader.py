@@ -38,7 +38,7 @@ def _iptr(t):
 
 class DeviceLoad:
     def __init__(self, nl, gmin=1e-12, limiter=True, temp=300.15, tnom=300.15,
-                 pattern=None, plan_only=False):
+                 pattern=None, plan_only=False, lu_fronts=False):
         nl.layout()
         L = lib()
         self.nl = nl
@@ -48,6 +48,8 @@ class DeviceLoad:
         try:
             check(L.xg_circuit_set_options(c, gmin, int(limiter), temp, tnom),
                   "xg_circuit_set_options")
+            if lu_fronts:
+                check(L.xg_circuit_set_lu_mode(c, 1), "xg_circuit_set_lu_mode")
             mid = {}
             for name, m in nl.models.items():
                 if m.kind == "diode":
@@ -142,6 +144,24 @@ class DeviceLoad:
         check(lib().xg_get_pattern(self.h, rp.ctypes.data_as(_lib.c_int32_p),
                                    ci.ctypes.data_as(_lib.c_int32_p)), "get pattern")
         return rp, ci[: self.nnz]
+
+    def front_plan(self):
+        """Host copy of the dense-front plan (xg_lu_front_export) as a dict
+        of numpy arrays."""
+        sz = np.zeros(6, np.int64)
+        check(lib().xg_lu_front_export(self.h, sz.ctypes.data, None, None), "front export")
+        nf, nr, nch, nlev, ne, nc = (int(v) for v in sz)
+        i32 = dict(comp=nf, m=nf, k=nf, rows=nr, cbmap=nr, ch_ptr=nf + 1, ch=nch,
+                   lv_ptr=nlev + 1, lv=nf, ent_k=ne, front_comp=nc)
+        i64 = dict(fo=nf + 1, ro=nf + 1, ent_fpos=ne)
+        out = {k: np.zeros(max(v, 1), np.int32) for k, v in i32.items()}
+        out.update({k: np.zeros(max(v, 1), np.int64) for k, v in i64.items()})
+        p32 = (ctypes.c_void_p * 11)(*[out[k].ctypes.data for k in i32])
+        p64 = (ctypes.c_void_p * 3)(*[out[k].ctypes.data for k in i64])
+        check(lib().xg_lu_front_export(self.h, sz.ctypes.data, p32, p64), "front export")
+        for k, v in list(i32.items()) + list(i64.items()):
+            out[k] = out[k][:v]
+        return out
 
     def lu_plan(self):
         """Host copy of the static-pivot sparse LU plan (xg_lu_export) as a

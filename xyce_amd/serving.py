@@ -82,7 +82,7 @@ UIC = {"ring": False, "logic": True, "ladder": False}
 
 
 def poisson_requests(rate=200.0, n=200, seed=4, mix=(0.6, 0.3, 0.1),
-                     ring_stages=(11, 101), logic_gates=(125, 400), ladder_nodes=10000):
+                     ring_stages=(11, 101), logic_gates=(125, 1250), ladder_nodes=10000):
     """A seeded Poisson stream: rings of U{ring_stages} BSIM3 stages,
     random BSIM3 logic of U{logic_gates} gates (4 FETs each), RC ladders
     of ladder_nodes nodes with one diode per ten nodes."""
