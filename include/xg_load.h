@@ -232,6 +232,25 @@ int xg_load_all(xg_handle *h, const double *x_next, double *sta_curr,
                 double *F, double *Q, double *dFdxdVp, double *dQdxdVp,
                 double *dFdx_vals, double *dQdx_vals);
 
+/* xg_load_all without any allocation: the statement values and reduction
+   partials go to a library-wide static device array of
+   XG_NOALLOC_SCRATCH_BYTES per GPU instead of the handle's own buffers.
+   No counterpart in the reference: it is for callers that cannot allocate
+   device memory (code under hipGraph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  Results are the bits of xg_load_all.
+   Limits: XG_E_ARG when the circuit's statement values and partials exceed
+   the array (about 55,000 BSIM3 instances with their interconnect);
+   XG_E_STATE with an active mask
+   set (it keeps statement values between calls) or when the current device
+   is not the handle's.  Every call on one GPU shares that GPU's array, so
+   calls that overlap on one GPU (other streams, other threads) race;
+   serialise them. */
+#define XG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int xg_load_all_noalloc(xg_handle *h, const double *x_next, double *sta_curr,
+                        const double *sto_curr, double *sta_next, double *sto_next,
+                        double *F, double *Q, double *dFdxdVp, double *dQdxdVp,
+                        double *dFdx_vals, double *dQdx_vals);
+
 /* The three per-Newton calls with host pointers, for a caller whose
    vectors and CSR value arrays stay in host memory (the reference's
    N_LAS objects): the same arguments and results as xg_update_state,

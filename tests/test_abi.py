@@ -188,3 +188,21 @@ def test_bulk_linear_adds_match_single_adds(L):
     assert b"device 3" in L.xg_last_error()
     assert L.xg_add_capacitors(c, -1, None, None, None, None) == XG_E_ARG
     L.xg_circuit_free(c)
+
+
+def test_noalloc_load_refuses_circuits_beyond_its_scratch(L):
+    """xg_load_all_noalloc checks the circuit's statement values and
+    reduction partials against its fixed scratch array before anything
+    else (a plan-only handle suffices) and names both sizes; a circuit
+    within the limit gets past that check to the device check."""
+    from xyce_amd import circuits
+    from xyce_amd._lib import XG_E_ARG, XG_E_STATE
+    from xyce_amd.loader import DeviceLoad
+    big = DeviceLoad(circuits.power_grid_bsim3(nx=4, ny=4, n_inv=31000), plan_only=True)
+    assert L.xg_load_all_noalloc(big.h, *([None] * 11)) == XG_E_ARG
+    msg = L.xg_last_error().decode()
+    assert "bytes of scratch" in msg and str(32 << 20) in msg, msg
+    big.close()
+    small = DeviceLoad(circuits.ring_oscillator_bsim3(11, seed=3), plan_only=True)
+    assert L.xg_load_all_noalloc(small.h, *([None] * 11)) == XG_E_STATE
+    small.close()

@@ -63,6 +63,7 @@ SIGNATURES = [
     ("xg_update_state_host", ctypes.c_int, [ctypes.c_void_p] * 6),
     ("xg_load_vectors_host", ctypes.c_int, [ctypes.c_void_p] * 5),
     ("xg_load_matrices_host", ctypes.c_int, [ctypes.c_void_p] * 3),
+    ("xg_load_all_noalloc", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 11),
     ("xg_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("xg_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

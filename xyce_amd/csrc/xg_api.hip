@@ -1340,6 +1340,105 @@ int xg_load_all(xg_handle *H, const double *x_next, double *sta_curr,
   return XG_OK;
 }
 
+}  // extern "C"
+
+// ---- xg_load_all_noalloc: the statement values and reduction partials of
+// the load live in one library-wide device array instead of the handle's
+// buffers, so the call allocates nothing.  Each GPU holds its own copy of
+// the array (at its own address, found with hipGetSymbolAddress on that
+// device's first call); every call on a GPU shares it, so calls that
+// overlap on one GPU race.
+namespace {
+
+constexpr int64_t kScratchDoubles = XG_NOALLOC_SCRATCH_BYTES / sizeof(double);
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (which must be
+// `device`), looked up once per device.
+int scratch_address(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(XG_E_ARG, "xg_load_all_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "xg_load_all_noalloc: hipGetSymbolAddress");
+    if (!ptr) return fail(XG_E_HIP, "xg_load_all_noalloc: null scratch address");
+    g_scratch_dev[device] = (double *)ptr;
+  }
+  *out = g_scratch_dev[device];
+  return XG_OK;
+}
+
+__global__ void zero_doubles_kernel(double *p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xg_load_all_noalloc(xg_handle *H, const double *x_next, double *sta_curr,
+                        const double *sto_curr, double *sta_next, double *sto_next,
+                        double *F, double *Q, double *dFdxdVp, double *dQdxdVp,
+                        double *dFdx, double *dQdx) {
+  if (!H) return fail(XG_E_ARG, "xg_load_all_noalloc: null handle");
+  xg::Handle &h = H->h;
+  // statement values, then the chunk partials at a 256-byte boundary
+  const int64_t n_val = h.plan.n_slot_values;
+  const int64_t pbase = (n_val + 31) / 32 * 32;
+  const int64_t need = pbase + (int64_t)h.plan.n_partials;
+  if (need > kScratchDoubles)
+    return fail(XG_E_ARG, "xg_load_all_noalloc: the circuit needs " + std::to_string(need * 8) +
+                              " bytes of scratch; the limit is " +
+                              std::to_string(kScratchDoubles * 8) + " (use xg_load_all)");
+  if (int rc = check_update_args(H, x_next, sta_curr, sto_curr, sta_next, sto_next)) return rc;
+  if (!F || !Q || !dFdxdVp || !dQdxdVp || !dFdx || !dQdx)
+    return fail(XG_E_ARG, "xg_load_all_noalloc: null pointer");
+  if (h.active)
+    return fail(XG_E_STATE,
+                "xg_load_all_noalloc: an active mask keeps statement values across calls, "
+                "which the shared scratch does not; clear it or use xg_load_all");
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return hip_fail(e, "xg_load_all_noalloc: hipGetDevice");
+  if (cur != h.device)
+    return fail(XG_E_STATE, "xg_load_all_noalloc: the handle's GPU is not the current device");
+  double *base = nullptr;
+  if (int rc = scratch_address(h.device, &base)) return rc;
+  double *vals = base, *partial = base + pbase;
+  // Statements an instance never writes (absent drain'/source' rows,
+  // padding) read as +0.0, as in the handle's buffer, which starts zeroed;
+  // the array holds the last call's values, so clear it (a fill kernel,
+  // not a memset node, so the call may be graph-captured).
+  if (n_val > 0) {
+    const int64_t blocks = std::min<int64_t>((n_val + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3((unsigned)blocks), dim3(256), 0, h.stream,
+                       vals, n_val);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "scratch clear");
+  }
+  xg::Mos1Args m1 = h.m1;
+  m1.slots = vals;
+  xg::LinArgs lin = h.lin;
+  lin.slots = vals + h.plan.lin_slot_base;
+  xg::Bsim3Dev b3 = h.b3;
+  xg::bsim3_bind(b3, vals, h.limited_all + h.plan.m1_slots);
+  if (int rc = eval_types(h, m1, b3, lin, 0, x_next, sta_curr, sto_curr, sta_next, sto_next))
+    return rc;
+  xg::ReduceDev progs[6] = {h.red_vec[0], h.red_vec[1], h.red_vec[2],
+                            h.red_vec[3], h.red_mat[0], h.red_mat[1]};
+  double *outs[6] = {F, Q, dFdxdVp, dQdxdVp, dFdx, dQdx};
+  e = xg::launch_reduce(6, progs, outs, vals, partial, h.overwrite, h.stream);
+  if (e != hipSuccess) return hip_fail(e, "reduction");
+  return XG_OK;
+}
+
 // ---- host-pointer variants: the caller's vectors stay in host memory
 // (the reference's N_LAS vectors); each call stages them through one
 // pinned buffer to device mirrors and back, and returns when the results
