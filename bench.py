@@ -128,6 +128,14 @@ def parse():
     ap.add_argument("--no-big-tran", action="store_true",
                     help="skip the large single-circuit transients (5,000-gate logic "
                          "5 ns, power grid 2 ns)")
+    ap.add_argument("--big-legs", default="logic,grid",
+                    help="large single-circuit transients to run (logic, grid)")
+    ap.add_argument("--big-span", type=float, default=1.0,
+                    help="fraction of the large transients' spans (5 ns logic, 2 ns grid) "
+                         "to simulate; 1 is the configured workload")
+    ap.add_argument("--big-fixed", action="store_true",
+                    help="large transients with fixed steps of --big-dt instead of the "
+                         "reference's step control")
     ap.add_argument("--big-dt", type=float, default=10e-12,
                     help="fixed step of the large single-circuit transients")
     ap.add_argument("--copy-calib", action="store_true",
@@ -298,8 +306,10 @@ def large_transient_leg(torch, args, which):
     the 5,000-gate random BSIM3 logic (20,000 FETs, one 20k-unknown
     component; .TRAN 5 ns from a zero start with the supply ramped over
     50 ps), "grid" = the 500 x 500 power grid with 100,000 BSIM3 inverters
-    (550,004 unknowns; DC operating point, then .TRAN 2 ns).  Fixed steps
-    of --big-dt.  Each Newton iteration is one graph: device load, ONESTEP
+    (550,004 unknowns; DC operating point, then .TRAN 2 ns).  Steps from
+    the reference's step control (local error estimate, breakpoints at the
+    PULSE corners; --big-fixed: fixed steps of --big-dt).  Each Newton
+    iteration is one graph: device load, ONESTEP
     assembly, the dense-front (multifrontal) LU of the component, update,
     norms and the convergence test.  Setup (netlist, analysis, upload,
     graph capture) is reported apart from the run."""
@@ -308,12 +318,12 @@ def large_transient_leg(torch, args, which):
     from xyce_amd.transient import Transient
     if which == "logic":
         nl = circuits.random_logic_bsim3(n_gates=args.gates, seed=2, ramp=50e-12)
-        tstop, probes, uic = 5e-9, ["g0", "g1"], True
+        tstop, probes, uic = 5e-9 * args.big_span, ["g0", "g1"], True
         name = ("random CMOS logic, %d NAND2/NOR2 gates, %d BSIM3 (seed 2), supply ramped "
                 "over 50 ps" % (args.gates, 4 * args.gates))
     else:
         nl = circuits.power_grid_bsim3()
-        tstop, probes, uic = 2e-9, ["out0", "g250_250"], False
+        tstop, probes, uic = 2e-9 * args.big_span, ["out0", "g250_250"], False
         name = "500x500 power grid + 100,000 BSIM3 inverters (seed 5)"
     progress("large transient (%s): setup" % which)
     t0 = time.perf_counter()
@@ -326,14 +336,29 @@ def large_transient_leg(torch, args, which):
              "%.3g flops per factorisation" % (which, nl.n_nodes, setup, info[1], info[2],
                                                info[5]))
     t0 = time.perf_counter()
-    t, w = tr.run(tstop, args.big_dt, probes, every=10, uic=uic)
+    last = [t0]
+
+    def report(step, n):
+        now = time.perf_counter()
+        if now - last[0] >= 20.0:
+            last[0] = now
+            progress("large transient (%s): step %d (%s), %.1f s" % (which, step, n, now - t0))
+    counts = None
+    if args.big_fixed:
+        t, w = tr.run(tstop, args.big_dt, probes, every=10, uic=uic, progress=report)
+    else:
+        t, w, counts = tr.run_adaptive(tstop, args.big_dt, probes, uic=uic, progress=report)
     torch.cuda.synchronize()
     run = time.perf_counter() - t0
     tr.close()
     progress("large transient (%s): %.2f s, %d Newton iterations" % (which, run,
                                                                      tr.newton_total))
+    steps = ("fixed steps of %.3g s" % args.big_dt if args.big_fixed else
+             "the reference's step control (TSTEP %.3g s, RELTOL 1e-3, ABSTOL 1e-6)"
+             % args.big_dt)
     return {
-        "workload": name + "; .TRAN %.3g s at %.3g s steps" % (tstop, args.big_dt),
+        "workload": name + "; .TRAN %.3g s, %s" % (tstop, steps),
+        "step_control": None if counts is None else counts,
         "unknowns": nl.n_nodes,
         "solves_per_s": 1.0 / run,
         "run_s": run,
@@ -569,8 +594,8 @@ def main():
     if not args.no_grid:
         res["power_grid"] = power_grid_leg(torch, args)
     if not args.no_big_tran:
-        res["logic_transient"] = large_transient_leg(torch, args, "logic")
-        res["grid_transient"] = large_transient_leg(torch, args, "grid")
+        for which in args.big_legs.split(","):
+            res[which + "_transient"] = large_transient_leg(torch, args, which)
     if not args.no_serve:
         res["serving"] = serving_leg(args)
     if not args.no_cpu:

@@ -378,6 +378,13 @@ int xg_lu_solve(xg_handle *h, const double *J, const double *rhs, double *x,
    [7] factor and solve tasks.  No counterpart in the reference. */
 int xg_lu_info(xg_handle *h, int64_t info[8]);
 
+/* Fronts of more than big_m rows (default 256) are factored panel by panel
+   across many workgroups (one workgroup per panel, then every column's row
+   swaps and U12 rows, then the trailing update in 64 x 64 tiles); smaller
+   ones by one workgroup each.  Both forms give the same bits.  No
+   counterpart in the reference (tuning and tests). */
+int xg_lu_front_set_big(xg_handle *h, int32_t big_m);
+
 /* Components too large for the task lists of xg_lu_info (more than 4 M
    fill entries or 64 M multiply-adds) are factored by a supernodal
    multifrontal LU on dense fronts (partial pivoting among each front's
@@ -476,7 +483,8 @@ int xg_sizes(const xg_handle *h, int32_t *n_nodes, int32_t *n_state,
 /* ---- Transient driver: the DC operating point and ONESTEP steps with the
    whole Newton loop on the device (replaces the host-side loop of
    N_NLS_DampedNewton::solve, N_NLS_DampedNewton.C:400-700, and the step
-   sequence of N_TIA_OneStep with fixed steps; no LTE step control).  The
+   sequence of N_TIA_OneStep, with fixed steps or the reference's step
+   control).  The
    iterations run as hipGraph launches (xg_lu_solve, xg_load_all,
    xg_tia_assemble, xg_rhs_norms, xg_component_limited, the non-finite
    count and the convergence test of N_NLS_DampedNewton.C:1787-1830 per
@@ -514,16 +522,54 @@ int xg_tran_create(xg_handle *h, const xg_tran_buffers *b, double bp_tol,
    history starts there.  > 0: Newton failure (1 singular Jacobian,
    2 non-finite device values, 3 no convergence). */
 int xg_tran_dcop(xg_tran *t);
+/* DC operating point with continuation: mode 0 = xg_tran_dcop; 1 = gmin
+   stepping (.options nonlin continuation=gmin, N_NLS_NOX_Interface.C:
+   894-960): natural continuation in p from 4 to -4 with a conductance
+   10^p - 10^-4 from every voltage unknown (not the sources' branch
+   currents) to ground in the residual and on the Jacobian diagonal
+   (N_NLS_NOX_AugmentLinSys_GStepping.C), first step -2, grown by
+   1 + 0.01 ((20 - iterations) / 20)^2 after a converged solve, halved after
+   a failed one (N_NLS_LOCA_StepSizeControl.C), the last step landing on -4
+   (conductance 0), at most 400 solves; 2 = Newton, then gmin stepping if
+   it fails (no counterpart in the reference).  Returns as xg_tran_dcop;
+   xg_tran_gmin_steps gives the continuation solves of the last run. */
+int xg_tran_dcop_continuation(xg_tran *t, int32_t mode);
+int xg_tran_gmin_steps(xg_tran *t, int64_t *steps);
 /* Start without an operating point (.TRAN ... UIC / NOOP): the solution,
    state and store start at zero, and one load there starts the charge
    and F history.  For circuits whose sources are all zero at t = 0 this
-   is the DC solution itself, reached without the DC Newton solve (which
-   has no continuation here, unlike the reference's gmin / source
-   stepping). */
+   is the DC solution itself, reached without the DC Newton solve. */
 int xg_tran_uic(xg_tran *t);
 /* One step of dt (order 1 for the first two steps, then 2); a step whose
    Newton solve fails is replaced by two of dt/2, recursively. */
 int xg_tran_step(xg_tran *t, double dt);
+/* Variable-step ONESTEP (.TRAN TSTEP TSTOP with step control), after
+   xg_tran_dcop or xg_tran_uic.  The scalar control of the reference's
+   N_TIA_StepErrorControl and N_TIA_OneStep: local error estimate
+   ck sqrt((|x - x_pred|_w^2 + |Q - Q_n|_w^2) / 2) with the newLte weights
+   (reltol max|x_n| + abstol; reltol |Q_i| + abstol), accepted when <= 1
+   (N_TIA_StepErrorControl.C:668-760); completeStep / rejectStep / initialize
+   (N_TIA_OneStep.C:1945-2190, 1695-1930, 1482-1640) for the next step size
+   and order (1 or 2); the given breakpoint times (PULSE corners) and tstop
+   as stop times at which the integration restarts.  Defaults of the
+   reference: reltol 1e-3, abstol 1e-6.  xg_tran_adaptive_step takes
+   attempts until one step is accepted and returns its time (tstop reached:
+   no step).  xg_tran_adaptive_info: h_est = {next step, last estimate,
+   stop time}, counts = {accepted, rejected by the error test, rejected by
+   Newton failure, next order}. */
+int xg_tran_adaptive_begin(xg_tran *t, double tstop, double tstep, const double *bps,
+                           int32_t n_bp, double reltol, double abstol);
+int xg_tran_adaptive_step(xg_tran *t, double *time);
+int xg_tran_adaptive_info(xg_tran *t, double h_est[3], int64_t counts[4]);
+/* Eviction for batches of independent circuits (no counterpart in the
+   reference, which runs one circuit): with on = 1, a step whose Newton
+   solve still fails after halving down to min_dt (default 1e-18) takes the
+   components that did not converge out of the solve -- their unknowns stay
+   at the last accepted values -- and is taken again for the others.
+   xg_tran_evicted: flags[c] = 1 for evicted components (n_comp host
+   entries, may be NULL), *n = how many. */
+int xg_tran_set_eviction(xg_tran *t, int32_t on, double min_dt);
+int xg_tran_evicted(xg_tran *t, int32_t *flags, int64_t *n);
 /* out[k] = x_curr[ids[k]] on the driver's stream (device arrays).  An id
    outside 0 .. n_nodes - 1 (ground included) reads nothing and gives NaN. */
 int xg_tran_probe(xg_tran *t, const int32_t *ids, int32_t n, double *out);

@@ -197,3 +197,46 @@ def test_power_grid_transient_on_fronts_matches_oracle():
     from xyce_amd import circuits
     _waveform_vs_oracle(lambda: circuits.power_grid_bsim3(nx=20, ny=20, n_inv=200, seed=5),
                         0.3e-9, 2e-12, ["out0", "out77", "g10_10"], 1.2, False)
+
+
+@pytest.mark.parametrize("gates,big_m", [(5000, 100000), (600, 48)])
+def test_many_workgroup_fronts_give_the_same_bits(gates, big_m):
+    """Fronts above the many-workgroup threshold (default 256 rows: the
+    5,000-gate logic's top fronts) against the one-workgroup form of the
+    same fronts (threshold raised above every front), and, on 600 gates,
+    every front above 48 rows in the many-workgroup form against the
+    default: identical solutions, bit for bit."""
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd._lib import check, lib
+    from xyce_amd.loader import DeviceLoad, _ptr
+    nl = circuits.random_logic_bsim3(n_gates=gates, seed=2).layout()
+    print("case: %d gates, threshold %d" % (gates, big_m), flush=True)
+    n1 = nl.n_nodes + 1
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.0, 1.2, n1)
+    x[-1] = 0.0
+    out = []
+    for thr in (None, big_m):
+        dev = DeviceLoad(nl, lu_fronts=True)
+        info = np.zeros(6, np.int64)
+        lib().xg_lu_front_info(dev.h, info.ctypes.data)
+        if thr is not None:
+            check(lib().xg_lu_front_set_big(dev.h, thr), "front threshold")
+        cu = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+        bufs = [cu(a) for a in (x, np.zeros(nl.n_state), np.zeros(nl.n_store),
+                                np.zeros(nl.n_state), np.zeros(nl.n_store))]
+        outs = [cu(np.zeros(n1)) for _ in range(4)] + [cu(np.zeros(dev.nnz)) for _ in range(2)]
+        dev.set_solver_state(transient=1, newtonIter=1)
+        dev.load_all(*bufs, *outs)
+        rhs, J = cu(np.zeros(n1)), cu(np.zeros(dev.nnz))
+        qn0, fprev = cu(np.zeros(n1)), cu(np.zeros(n1))
+        check(lib().xg_tia_assemble(dev.h, 1, 1, 1e-12, *[_ptr(t) for t in (
+            qn0, fprev, *outs, rhs, J)]), "assemble")
+        dev.sync()
+        xs, st = _solve(dev, J.cpu().numpy(), rhs.cpu().numpy(), torch)
+        print("largest front %d rows, singular %s" % (info[4], st[:, 2].max()), flush=True)
+        assert st[:, 2].max() == 0
+        out.append(xs)
+        dev.close()
+    assert np.array_equal(out[0], out[1])

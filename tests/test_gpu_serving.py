@@ -41,3 +41,39 @@ def test_mixed_stream_batches_and_matches_alone():
             vmax = 1.2 if r.kind != "ladder" else 1.0
             assert np.abs(w - wb).max() <= 1e-3 * vmax + 1e-6, r.kind
     assert checked == {"ring", "logic", "ladder"}
+
+
+def test_failing_request_is_evicted_without_disturbing_its_batch():
+    """A request whose Newton solve cannot converge (a 0 -> 5 V source
+    ramped in 10 ps straight across a junction diode) is evicted from its
+    batch at the smallest step; the ring batched with it completes with the
+    waveform it gets alone (to the Newton tolerance: the batch's steps were
+    halved while the failing request was still in it)."""
+    from xyce_amd import circuits
+    from xyce_amd.circuits import Netlist, Pulse
+    from xyce_amd.serving import Request, _Batch, _run_batch
+
+    bad = Netlist("diode across a fast ramp")
+    bad.add_model("DX", "diode", 1, IS=1e-14, N=1.0)
+    bad.add_vsrc("V1", "a", "0", 0.0, Pulse(0.0, 5.0, 5e-12, 10e-12, 10e-12, 1.0, 2.0))
+    bad.add_diode("D1", "a", "0", "DX")
+
+    def req(rid, nl, probes):
+        r = Request(rid, "ring", 11, 3, 0.0, 0.05e-9, 1e-12, probes)
+        r._nl = nl
+        return r
+
+    ring = lambda: circuits.ring_oscillator_bsim3(11, seed=3, ramp=50e-12)
+    alone = req(0, ring(), ["out0"])
+    _run_batch(_Batch([alone]), None)
+    good, evil = req(1, ring(), ["out0"]), req(2, bad, ["a"])
+    _run_batch(_Batch([good, evil]), None)
+    print("errors:", good.error, "|", evil.error)
+    assert evil.error is not None and evil.error.startswith("evicted")
+    assert good.error is None
+    t_a, w_a = alone.result
+    t_g, w_g = good.result
+    # the halved steps sum to the same output times up to rounding
+    assert t_a.shape == t_g.shape and np.abs(t_a - t_g).max() <= 1e-9 * 1e-12
+    print("max difference to the run alone: %.3g V" % np.abs(w_g - w_a).max())
+    assert np.abs(w_g - w_a).max() <= 1e-3 * 1.2

@@ -96,6 +96,7 @@ SIGNATURES = [
     ("xg_lu_solve", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 +
      [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_lu_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_lu_front_set_big", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("xg_lu_front_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_lu_front_export", ctypes.c_int, [ctypes.c_void_p] * 4),
     ("xg_circuit_set_lu_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
@@ -108,6 +109,15 @@ SIGNATURES = [
     ("xg_tran_dcop", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_tran_uic", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_tran_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("xg_tran_adaptive_begin", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                              ctypes.c_double]),
+    ("xg_tran_adaptive_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_tran_adaptive_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_tran_set_eviction", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double]),
+    ("xg_tran_evicted", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_tran_dcop_continuation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("xg_tran_gmin_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_tran_probe", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                      ctypes.c_void_p]),
     ("xg_tran_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -201,6 +201,37 @@ class Netlist:
         per = p.per if p.per is not None else tstop
         return [p.v1, p.v2, p.td or 0.0, tr, tf, pw, per]
 
+    def voltage_unknowns(self):
+        """Solution ids of the voltage unknowns (nodes and internal drain' /
+        source' nodes): every unknown but the voltage sources' branch
+        currents."""
+        self.layout()
+        br = set(self.vsrc_branch)
+        return [i for i in range(self.n_nodes) if i not in br]
+
+    def breakpoints(self, tstop):
+        """Sorted corner times in (0, tstop) of every PULSE source (TD, TD+TR,
+        TD+TR+PW, TD+TR+PW+TF and TD+PER of each period, PulseData::
+        getBreakPoints, N_DEV_SourceData.C:1114-1160), with the source values
+        the load uses (pulse_params with its defaults)."""
+        out = set()
+        for v in self.vsrc:
+            pp = self.pulse_params(v)
+            if pp is None:
+                continue
+            _, _, td, tr, tf, pw, per = pp
+            base = 0.0
+            while True:
+                for c in (td, td + tr, td + tr + pw, td + tr + pw + tf, td + per):
+                    if 0.0 < base + c < tstop:
+                        out.add(base + c)
+                if per <= 0.0:
+                    break
+                base += per
+                if base + td >= tstop:
+                    break
+        return sorted(out)
+
     def linear_devices(self):
         """(kind, pos, neg, branch, value, pulse, sta) in load order:
         resistors, capacitors, diodes, sources, each in netlist order.  For

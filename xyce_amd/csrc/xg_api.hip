@@ -450,6 +450,12 @@ int xg_lu_solve(xg_handle *H, const double *J, const double *rhs, double *x,
     f.gate = h.gate;
     f.lv_ptr_host = h.fr_lv_ptr.data();
     f.lv_maxm_host = h.fr_lv_maxm.data();
+    const xg::LuPlan &L = h.plan.lu;
+    f.lv_host = L.fr_lv.data();
+    f.m_host = L.fr_m.data();
+    f.k_host = L.fr_k.data();
+    f.ch_ptr_host = L.fr_ch_ptr.data();
+    f.ch_host = L.fr_ch.data();
     e = xg::launch_fronts(f, u, h.stream);
     if (e != hipSuccess) return hip_fail(e, "front LU");
   }
@@ -497,6 +503,12 @@ int xg_lu_info(xg_handle *H, int64_t *info) {
     info[6] = 0;
   }
   info[7] = (int64_t)L.t_dst.size();
+  return XG_OK;
+}
+
+int xg_lu_front_set_big(xg_handle *H, int32_t big_m) {
+  if (!H || big_m < 1) return fail(XG_E_ARG, "xg_lu_front_set_big");
+  H->h.fr.big_m = big_m;
   return XG_OK;
 }
 
@@ -626,6 +638,29 @@ int32_t handle_n_comp(const xg_handle *H) {
   return (int32_t)H->h.plan.comp_start.size() - 1;
 }
 int64_t handle_n_nodes(const xg_handle *H) { return H->h.plan.n_nodes; }
+
+void handle_voltage_diagonal(const xg_handle *H, std::vector<int32_t> &ids,
+                             std::vector<int32_t> &pos) {
+  const xg::Plan &P = H->h.plan;
+  const int32_t n = P.n_nodes, S = P.lin_slots;
+  std::vector<char> branch(n, 0);
+  for (int32_t s = 0; s < S; ++s)
+    if (P.lin_type[s] == xg::LIN_V) {
+      const int32_t b = P.lin_lid[(size_t)2 * S + s];
+      if (b >= 0 && b < n) branch[b] = 1;
+    }
+  ids.clear();
+  pos.clear();
+  for (int32_t i = 0; i < n; ++i) {
+    if (branch[i]) continue;
+    const int32_t *b = P.colidx.data() + P.rowptr[i];
+    const int32_t *e = P.colidx.data() + P.rowptr[i + 1];
+    const int32_t *it = std::lower_bound(b, e, i);
+    if (it == e || *it != i) continue;
+    ids.push_back(i);
+    pos.push_back((int32_t)(it - P.colidx.data()));
+  }
+}
 void handle_sizes(const xg_handle *H, int64_t *n_state, int64_t *n_store, int64_t *nnz) {
   *n_state = H->h.plan.n_state;
   *n_store = H->h.plan.n_store;

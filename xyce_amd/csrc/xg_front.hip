@@ -88,13 +88,20 @@ __device__ __forceinline__ double blk_reduce(double v, double *sv, bool is_max) 
 // element's updates in ascending pivot order, as in the column-by-column
 // algorithm).  piv[j] = the row swapped with row j.  False when a pivot
 // column has no nonzero candidate.
+//
+// Columns [k0, k1) only, with the row swaps applied to columns [swap_lo, cl)
+// and the updates confined to columns below cl: one panel of the
+// many-workgroup form below, which applies the swaps left of swap_lo
+// afterwards (k0 = 0, k1 = k, cl = m, swap_lo = 0 is the whole
+// factorization).
 template <int NB>
 __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int *si,
-                             int *spiv, int32_t *piv) {
+                             int *spiv, int32_t *piv, int k0, int k1, int cl,
+                             int swap_lo = 0) {
   const int t = threadIdx.x;
   constexpr int PS = NB + 1;
-  for (int kb = 0; kb < k; kb += NB) {
-    const int nb = min(NB, k - kb);
+  for (int kb = k0; kb < k1; kb += NB) {
+    const int nb = min(NB, k1 - kb);
     const int rows = m - kb;      // panel rows
     const int cand = k - kb;      // rows eligible as pivots
     for (int e = t; e < rows * NB; e += kFB) {
@@ -138,7 +145,7 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
     }
     if (t < nb) piv[kb + t] = spiv[t];
     // the panel's row swaps on every other column, in pivot order
-    for (int j = t; j < m; j += kFB) {
+    for (int j = swap_lo + t; j < cl; j += kFB) {
       if (j >= kb && j < kb + nb) continue;
       for (int c = 0; c < nb; ++c) {
         const int pr = spiv[c];
@@ -150,7 +157,7 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
       }
     }
     __syncthreads();
-    const int j0 = kb + nb, ncols = m - j0;
+    const int j0 = kb + nb, ncols = cl - j0;
     // U12 of column j (rows kb..kb+nb-1) by the unit lower panel block
     auto solve_u = [&](int j, double *u) {
 #pragma unroll
@@ -184,7 +191,7 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
       }
     };
     if (ncols >= kFB) {
-      for (int j = j0 + t; j < m; j += kFB) {
+      for (int j = j0 + t; j < cl; j += kFB) {
         double u[NB];
         solve_u(j, u);
 #pragma unroll
@@ -245,6 +252,7 @@ __global__ void __launch_bounds__(kFB) front_factor_kernel(FrontArgs f, int lv0)
   if (front_skip(f, c)) return;
   const int t = threadIdx.x;
   const int m = f.m[s], k = f.k[s];
+  if (m > f.big_m) return;  // factored by many workgroups (launch_big_front)
   double *A = f.vals + f.fo[s];
   // the children's contribution blocks, in child order
   for (int q = f.ch_ptr[s]; q < f.ch_ptr[s + 1]; ++q) {
@@ -258,8 +266,141 @@ __global__ void __launch_bounds__(kFB) front_factor_kernel(FrontArgs f, int lv0)
     }
     __syncthreads();
   }
-  if (!front_factor<NB>(A, m, k, P, sv, si, spiv, f.piv + f.ro[s]))
+  if (!front_factor<NB>(A, m, k, P, sv, si, spiv, f.piv + f.ro[s], 0, k, m))
     if (t == 0) f.bad[c] = 1;
+}
+
+// ---- many-workgroup form of one large front (rows m > big_m): per panel of
+// kBigW fully summed columns, one workgroup factors the panel (swaps on
+// the columns left of the panel's end), then the columns right of it take
+// the panel's row swaps and their U12 rows (one thread per column), then
+// the trailing block takes the rank-kBigW update in 64 x 64 tiles.  Every
+// element sees the same operations in the same order as in the
+// one-workgroup form, so both give the same bits.
+constexpr int kBigW = 32;   // panel width
+constexpr int kTile = 64;   // trailing-update tile
+
+__global__ void __launch_bounds__(kFB) front_extend_kernel(FrontArgs f, int s, int ch) {
+  if (front_skip(f, f.comp[s])) return;
+  const int m = f.m[s];
+  double *A = f.vals + f.fo[s];
+  const int mc = f.m[ch], kc = f.k[ch], w = mc - kc;
+  const double *C = f.vals + f.fo[ch] + (int64_t)kc * mc + kc;
+  const int32_t *map = f.cbmap + f.ro[ch] + kc;
+  const int64_t n = (int64_t)w * w;
+  for (int64_t e = (int64_t)blockIdx.x * kFB + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * kFB) {
+    const int i = (int)(e / w), j = (int)(e % w);
+    A[(int64_t)map[i] * m + map[j]] += C[(int64_t)i * mc + j];
+  }
+}
+
+template <int NB, int ROWS>
+__global__ void __launch_bounds__(kFB) front_panel_kernel(FrontArgs f, int s, int kb) {
+  __shared__ double P[ROWS * (NB + 1)];
+  __shared__ double sv[kFB / 64];
+  __shared__ int si[kFB / 64];
+  __shared__ int spiv[NB];
+  const int c = f.comp[s];
+  if (front_skip(f, c)) return;
+  const int m = f.m[s], k = f.k[s];
+  const int cl = min(kb + kBigW, k);
+  if (!front_factor<NB>(f.vals + f.fo[s], m, k, P, sv, si, spiv, f.piv + f.ro[s], kb, cl,
+                        cl, kb))
+    if (threadIdx.x == 0) f.bad[c] = 1;
+}
+
+// Columns j >= cl: the panel's row swaps in pivot order, then U12 = L11^-1
+// (rows kb..cl-1) with the unit lower panel block, each u[i] updated in
+// ascending pivot order.
+__global__ void __launch_bounds__(kFB) front_swap_trsm_kernel(FrontArgs f, int s, int kb) {
+  __shared__ double L11[kBigW][kBigW + 1];
+  __shared__ int32_t pv[kBigW];
+  if (front_skip(f, f.comp[s])) return;
+  const int m = f.m[s], k = f.k[s];
+  const int cl = min(kb + kBigW, k), nw = cl - kb;
+  double *A = f.vals + f.fo[s];
+  const int32_t *piv = f.piv + f.ro[s];
+  for (int e = threadIdx.x; e < kBigW * kBigW; e += kFB) {
+    const int i = e / kBigW, cc = e % kBigW;
+    L11[i][cc] = (i < nw && cc < nw) ? A[(int64_t)(kb + i) * m + kb + cc] : 0.0;
+  }
+  if (threadIdx.x < kBigW) pv[threadIdx.x] = threadIdx.x < nw ? piv[kb + threadIdx.x] : 0;
+  __syncthreads();
+  const int j = cl + blockIdx.x * kFB + threadIdx.x;
+  if (j >= m) return;
+  for (int cc = 0; cc < nw; ++cc) {
+    const int pr = pv[cc];
+    if (pr != kb + cc) {
+      const double tmp = A[(int64_t)(kb + cc) * m + j];
+      A[(int64_t)(kb + cc) * m + j] = A[(int64_t)pr * m + j];
+      A[(int64_t)pr * m + j] = tmp;
+    }
+  }
+  double u[kBigW];
+#pragma unroll
+  for (int cc = 0; cc < kBigW; ++cc) u[cc] = cc < nw ? A[(int64_t)(kb + cc) * m + j] : 0.0;
+#pragma unroll
+  for (int cc = 0; cc < kBigW; ++cc)
+#pragma unroll
+    for (int i = cc + 1; i < kBigW; ++i)
+      if (i < nw) u[i] -= L11[i][cc] * u[cc];
+#pragma unroll
+  for (int cc = 0; cc < kBigW; ++cc)
+    if (cc < nw) A[(int64_t)(kb + cc) * m + j] = u[cc];
+}
+
+// Rows and columns >= cl: A -= L21 U12, each element's terms in ascending
+// pivot order.  Grid: (tiles over columns, tiles over rows).
+__global__ void __launch_bounds__(kFB) front_update_kernel(FrontArgs f, int s, int kb) {
+  __shared__ double Lt[kTile][kBigW + 1];
+  __shared__ double Ut[kBigW][kTile + 1];
+  if (front_skip(f, f.comp[s])) return;
+  const int m = f.m[s], k = f.k[s];
+  const int cl = min(kb + kBigW, k), nw = cl - kb;
+  double *A = f.vals + f.fo[s];
+  const int r0 = cl + blockIdx.y * kTile, c0 = cl + blockIdx.x * kTile;
+  if (r0 >= m || c0 >= m) return;
+  for (int e = threadIdx.x; e < kTile * kBigW; e += kFB) {
+    const int r = e / kBigW, cc = e % kBigW;
+    Lt[r][cc] = (r0 + r < m && cc < nw) ? A[(int64_t)(r0 + r) * m + kb + cc] : 0.0;
+  }
+  for (int e = threadIdx.x; e < kBigW * kTile; e += kFB) {
+    const int cc = e / kTile, q = e % kTile;
+    Ut[cc][q] = (cc < nw && c0 + q < m) ? A[(int64_t)(kb + cc) * m + c0 + q] : 0.0;
+  }
+  __syncthreads();
+  // thread: one column of the tile, rows r, r + 4, ... (16 rows)
+  const int q = threadIdx.x % kTile, rg = threadIdx.x / kTile;
+  const int j = c0 + q;
+  if (j >= m) return;
+  for (int r = rg; r < kTile; r += kFB / kTile) {
+    const int i = r0 + r;
+    if (i >= m) break;
+    double v = A[(int64_t)i * m + j];
+    for (int cc = 0; cc < nw; ++cc) v -= Lt[r][cc] * Ut[cc][q];
+    A[(int64_t)i * m + j] = v;
+  }
+}
+
+// The row swaps of the later panels on the factored columns j < k (the L
+// part), deferred from the panels: column j takes every pivot after its
+// own panel, in pivot order, as the one-workgroup form applies them.
+__global__ void __launch_bounds__(kFB) front_left_swaps_kernel(FrontArgs f, int s) {
+  if (front_skip(f, f.comp[s])) return;
+  const int m = f.m[s], k = f.k[s];
+  const int j = blockIdx.x * kFB + threadIdx.x;
+  if (j >= k) return;
+  double *A = f.vals + f.fo[s];
+  const int32_t *piv = f.piv + f.ro[s];
+  for (int c = (j / kBigW + 1) * kBigW; c < k; ++c) {
+    const int pr = piv[c];
+    if (pr != c) {
+      const double tmp = A[(int64_t)c * m + j];
+      A[(int64_t)c * m + j] = A[(int64_t)pr * m + j];
+      A[(int64_t)pr * m + j] = tmp;
+    }
+  }
 }
 
 // Forward solve of one level's fronts: the front's right-hand side (its
@@ -412,6 +553,37 @@ hipError_t launch_level(const FrontArgs &f, int lv0, int nfr, hipStream_t s) {
   return hipGetLastError();
 }
 
+// One front of more than big_m rows by many workgroups: its children's
+// contribution blocks (one launch per child, in child order), then per
+// panel the panel, swap / U12 and trailing-update launches.
+hipError_t launch_big_front(const FrontArgs &f, int sf, hipStream_t s) {
+  const int m = f.m_host[sf], k = f.k_host[sf];
+  for (int q = f.ch_ptr_host[sf]; q < f.ch_ptr_host[sf + 1]; ++q) {
+    const int ch = f.ch_host[q];
+    const int64_t w = (int64_t)f.m_host[ch] - f.k_host[ch];
+    const unsigned g = (unsigned)std::min<int64_t>((w * w + kFB - 1) / kFB + 1, 4096);
+    hipLaunchKernelGGL(front_extend_kernel, dim3(g), dim3(kFB), 0, s, f, sf, ch);
+  }
+  for (int kb = 0; kb < k; kb += kBigW) {
+    if (m <= 1934)
+      hipLaunchKernelGGL((front_panel_kernel<8, 1934>), dim3(1), dim3(kFB), 0, s, f, sf, kb);
+    else
+      hipLaunchKernelGGL((front_panel_kernel<4, kFrontMaxM>), dim3(1), dim3(kFB), 0, s, f, sf,
+                         kb);
+    const int cl = std::min(kb + kBigW, k);
+    const int rest = m - cl;
+    if (rest <= 0) continue;
+    hipLaunchKernelGGL(front_swap_trsm_kernel, dim3((rest + kFB - 1) / kFB), dim3(kFB), 0, s,
+                       f, sf, kb);
+    const unsigned tiles = (unsigned)((rest + kTile - 1) / kTile);
+    hipLaunchKernelGGL(front_update_kernel, dim3(tiles, tiles), dim3(kFB), 0, s, f, sf, kb);
+  }
+  if (k > kBigW)
+    hipLaunchKernelGGL(front_left_swaps_kernel, dim3((k + kFB - 1) / kFB), dim3(kFB), 0, s, f,
+                       sf);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 // The panel form for fronts of up to `max_m` rows: 0 when none fits.
@@ -435,14 +607,28 @@ hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t 
   if (e != hipSuccess) return e;
   for (int L = 0; L < f.n_levels; ++L) {
     const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
-    switch (front_form(f.lv_maxm_host[L])) {
-      case 1: e = launch_level<16, 64>(f, lv0, nfr, s); break;
-      case 2: e = launch_level<16, 256>(f, lv0, nfr, s); break;
-      case 3: e = launch_level<16, 1024>(f, lv0, nfr, s); break;
-      case 4: e = launch_level<8, 1934>(f, lv0, nfr, s); break;
-      default: e = launch_level<4, kFrontMaxM>(f, lv0, nfr, s); break;
+    // the level's fronts up to big_m rows, one workgroup each
+    int32_t small_max = 0;
+    bool any_big = false;
+    for (int q = lv0; q < lv0 + nfr; ++q) {
+      const int32_t mq = f.m_host[f.lv_host[q]];
+      if (mq > f.big_m) any_big = true;
+      else small_max = std::max(small_max, mq);
     }
-    if (e != hipSuccess) return e;
+    if (small_max > 0) {
+      switch (front_form(small_max)) {
+        case 1: e = launch_level<16, 64>(f, lv0, nfr, s); break;
+        case 2: e = launch_level<16, 256>(f, lv0, nfr, s); break;
+        case 3: e = launch_level<16, 1024>(f, lv0, nfr, s); break;
+        case 4: e = launch_level<8, 1934>(f, lv0, nfr, s); break;
+        default: e = launch_level<4, kFrontMaxM>(f, lv0, nfr, s); break;
+      }
+      if (e != hipSuccess) return e;
+    }
+    if (any_big)
+      for (int q = lv0; q < lv0 + nfr; ++q)
+        if (f.m_host[f.lv_host[q]] > f.big_m)
+          if ((e = launch_big_front(f, f.lv_host[q], s)) != hipSuccess) return e;
   }
   for (int L = 0; L < f.n_levels; ++L) {
     const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;

@@ -278,6 +278,8 @@ struct SparseLuArgs {
 // Dense fronts (multifrontal LU) of the components too large for task
 // lists; the plan is LuPlan's fr_* arrays (xg_setup.h).
 constexpr int kFrontMaxM = 3481;  // largest front a workgroup's panel holds
+// Fronts of more rows are factored by many workgroups per panel.
+constexpr int32_t kFrontBigM = 256;
 struct FrontArgs {
   int32_t n_fronts = 0, n_comp = 0, n_levels = 0;
   int64_t n_vals = 0, n_ent = 0;
@@ -288,6 +290,10 @@ struct FrontArgs {
   const int32_t *active;
   const int32_t *gate = nullptr;  // stop word (LoadPtrs::gate)
   const int32_t *lv_ptr_host, *lv_maxm_host;  // host arrays: the launch loop
+  // host copies for the many-workgroup form of fronts of more than big_m
+  // rows
+  const int32_t *lv_host, *m_host, *k_host, *ch_ptr_host, *ch_host;
+  int32_t big_m = kFrontBigM;
 };
 int front_form(int32_t max_m);
 hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t s);
@@ -334,6 +340,10 @@ hipStream_t handle_stream(const xg_handle *h);
 void handle_set_gate(xg_handle *h, const int32_t *gate);
 int32_t handle_n_comp(const xg_handle *h);
 int64_t handle_n_nodes(const xg_handle *h);
+// Voltage unknowns (every unknown but the voltage sources' branch currents)
+// with a diagonal Jacobian entry, and that entry's CSR position.
+void handle_voltage_diagonal(const xg_handle *h, std::vector<int32_t> &ids,
+                             std::vector<int32_t> &pos);
 void handle_sizes(const xg_handle *h, int64_t *n_state, int64_t *n_store, int64_t *nnz);
 
 }  // namespace xg

@@ -77,7 +77,8 @@ SPANS = {"ring": (0.1e-9, 1e-12, ["out0"]),
          "ladder": (0.05e-9, 1e-12, ["n0", "n1"])}
 # Random logic starts from zero (.TRAN UIC) with its supply ramping up: its
 # feedback paths can leave the plain DC Newton solve without convergence,
-# and there is no gmin / source stepping here.
+# and the gmin-stepping operating point (Transient.dcop("gmin")) costs
+# more solves than the ramp.
 UIC = {"ring": False, "logic": True, "ladder": False}
 
 
@@ -131,7 +132,21 @@ def _run_batch(batch, sync):
     tr = Transient(merged)
     t1 = time.perf_counter()
     try:
+        # a request whose Newton solve fails at dt / 1024 leaves the batch
+        # (its sub-circuits frozen); the others run on
+        tr.set_eviction(True, dt / 1024)
         t, w = tr.run(tstop, dt, names, every=10, uic=UIC[reqs[0].kind])
+        evicted = set(tr.evicted_components())
+        gone = set()
+        if evicted:
+            import ctypes
+            from ._lib import lib
+            nc = ctypes.c_int32()
+            comp = np.zeros(max(merged.n_nodes, 1), np.int32)
+            lib().xg_components(tr.dev.h, ctypes.byref(nc), comp.ctypes.data)
+            for name, nid in merged.node_ids.items():
+                if 0 <= nid < merged.n_nodes and int(comp[nid]) in evicted:
+                    gone.add(int(name.split("_", 1)[0][1:]))
         for r in reqs:
             r.batch_info = dict(size=len(reqs), step_failures=tr.step_failures,
                                 newton=tr.newton_total, setup_s=t1 - t0,
@@ -143,9 +158,12 @@ def _run_batch(batch, sync):
     for r in reqs:
         r._nl = None
     j = 0
-    for r in reqs:
+    for k, r in enumerate(reqs):
         m = len(r.probes)
-        r.result = (t, w[j:j + m])
+        if k in gone:
+            r.error = "evicted: its Newton solve failed at the smallest step"
+        else:
+            r.result = (t, w[j:j + m])
         j += m
 
 

@@ -260,11 +260,23 @@ class Transient:
         self.t = 0.0
         self.n_steps = 0
 
-    def dcop(self):
-        """DC operating point at t = 0 (the tranop before a transient)."""
+    def dcop(self, continuation="none"):
+        """DC operating point at t = 0 (the tranop before a transient).
+        continuation="gmin": by gmin stepping (.options nonlin
+        continuation=gmin: a conductance 10^p - 10^-4 from every voltage
+        unknown to ground, p stepped from 4 to -4); "auto": Newton first,
+        gmin stepping if it fails (no counterpart in the reference, which
+        stops at a failed operating point).  Continuation needs the native
+        driver."""
+        modes = {"none": 0, "gmin": 1, "auto": 2}
+        if continuation not in modes:
+            raise ValueError("continuation must be one of %s" % sorted(modes))
         if self._tr is not None:
-            self._native(lib().xg_tran_dcop(self._tr), "DC operating point")
+            self._native(lib().xg_tran_dcop_continuation(self._tr, modes[continuation]),
+                         "DC operating point")
             return
+        if continuation != "none":
+            raise RuntimeError("DC continuation needs the native transient driver")
         self.x.zero_()
         self.sta_c.zero_()
         self.sto_c.zero_()
@@ -328,13 +340,15 @@ class Transient:
                 raise KeyError("probe %r maps to id %d outside 0..%d" % (p, i, n - 1))
         return ids
 
-    def run(self, tstop, dt, probes, every=1, uic=False):
-        """DC operating point (or, uic=True, the zero start), then steps of
-        dt to tstop.  Returns (times, waveforms[probe, time]) for the given
-        node names."""
+    def run(self, tstop, dt, probes, every=1, uic=False, dc="none", progress=None):
+        """DC operating point (or, uic=True, the zero start; dc selects the
+        operating point's continuation, see dcop), then steps of dt to
+        tstop.  Returns (times, waveforms[probe, time]) for the given node
+        names.  progress(step, n_steps), if given, is called every 50
+        steps."""
         torch = self.torch
         if self._tr is not None:
-            return self._run_native(tstop, dt, probes, every, uic)
+            return self._run_native(tstop, dt, probes, every, uic, dc, progress)
         idx = torch.tensor(self._probe_ids(probes), device=self.x.device)
         self.uic() if uic else self.dcop()
         n = int(round(tstop / dt))
@@ -351,7 +365,7 @@ class Transient:
                 k += 1
         return np.array(times), out[:, :k].cpu().numpy()
 
-    def _run_native(self, tstop, dt, probes, every, uic=False):
+    def _run_native(self, tstop, dt, probes, every, uic=False, dc="none", progress=None):
         torch = self.torch
         L = lib()
         n = int(round(tstop / dt))
@@ -361,7 +375,7 @@ class Transient:
             out = torch.zeros(n // every + 1, len(probes), dtype=torch.float64,
                               device=self.x.device)
             torch.cuda.current_stream().synchronize()
-        self.uic() if uic else self.dcop()
+        self.uic() if uic else self.dcop(dc)
         check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes), out[0].data_ptr()),
               "probe")
         times = [0.0]
@@ -371,6 +385,8 @@ class Transient:
             rc = L.xg_tran_step(self._tr, dt)
             if rc:
                 self._native(rc, "time step")
+            if progress is not None and s % 50 == 0:
+                progress(s, n)
             if s % every == 0:
                 check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes),
                                       out[k].data_ptr()), "probe")
@@ -382,6 +398,81 @@ class Transient:
         with DEVICE_LOCK:
             w = out[:k].T.cpu().numpy()
         return np.array(times), w
+
+    def run_adaptive(self, tstop, tstep, probes, uic=False, reltol=1e-3, abstol=1e-6,
+                     max_points=1_000_000, dc="none", progress=None):
+        """DC operating point (or the zero start), then the reference's
+        variable-step ONESTEP to tstop: step size and order from the local
+        error estimate, steps rejected above it, the PULSE corners as
+        breakpoints at which the integration restarts (N_TIA_StepErrorControl,
+        N_TIA_OneStep completeStep / rejectStep / initialize; .TRAN TSTEP
+        TSTOP with RELTOL / ABSTOL of the time integrator).  Returns (times,
+        waveforms[probe, accepted step], counts) where counts holds the
+        accepted steps, the steps rejected by the error test and by Newton
+        failures.  progress(steps, t), if given, is called every 50
+        accepted steps.  Needs the native driver."""
+        if self._tr is None:
+            raise RuntimeError("run_adaptive needs the native transient driver")
+        torch = self.torch
+        L = lib()
+        with DEVICE_LOCK:
+            ids = torch.tensor(self._probe_ids(probes), dtype=torch.int32,
+                               device=self.x.device)
+            buf = torch.zeros(4096, len(probes), dtype=torch.float64, device=self.x.device)
+            torch.cuda.current_stream().synchronize()
+        self.uic() if uic else self.dcop(dc)
+        bps = np.ascontiguousarray(self.nl.breakpoints(tstop), dtype=np.float64)
+        self._native(L.xg_tran_adaptive_begin(self._tr, tstop, tstep, bps.ctypes.data,
+                                              len(bps), reltol, abstol), "adaptive begin")
+        times, chunks = [0.0], []
+        check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes), buf[0].data_ptr()),
+              "probe")
+        k = 1
+        tnow = ctypes.c_double()
+        while abs(times[-1] - tstop) >= 1e-10 * tstop:
+            if len(times) >= max_points:
+                raise RuntimeError("more than %d accepted steps" % max_points)
+            rc = L.xg_tran_adaptive_step(self._tr, ctypes.byref(tnow))
+            if rc:
+                self._native(rc, "adaptive step")
+            if k == buf.shape[0]:
+                with DEVICE_LOCK:
+                    chunks.append(buf.cpu().numpy())
+                k = 0
+            check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes), buf[k].data_ptr()),
+                  "probe")
+            times.append(tnow.value)
+            k += 1
+            if progress is not None and len(times) % 50 == 0:
+                progress(len(times) - 1, tnow.value)
+        check(L.xg_tran_sync(self._tr), "sync")
+        self._sync_counters()
+        with DEVICE_LOCK:
+            chunks.append(buf[:k].cpu().numpy())
+        counts = np.zeros(4, np.int64)
+        check(L.xg_tran_adaptive_info(self._tr, None, counts.ctypes.data), "adaptive info")
+        return np.array(times), np.concatenate(chunks).T, dict(
+            accepted=int(counts[0]), rejected_error=int(counts[1]),
+            rejected_newton=int(counts[2]))
+
+    def set_eviction(self, on=True, min_dt=1e-18):
+        """Batches of independent circuits: a step whose Newton solve still
+        fails at min_dt takes the sub-circuits that did not converge out of
+        the solve (frozen at their last accepted values) instead of failing
+        the run.  Needs the native driver."""
+        if self._tr is None:
+            raise RuntimeError("eviction needs the native transient driver")
+        check(lib().xg_tran_set_eviction(self._tr, 1 if on else 0, min_dt), "eviction")
+
+    def evicted_components(self):
+        """Ids of the evicted sub-circuits (components of xg_components)."""
+        import ctypes as C
+        nc = C.c_int32()
+        check(lib().xg_components(self.dev.h, C.byref(nc), None), "components")
+        flags = np.zeros(max(nc.value, 1), np.int32)
+        n = C.c_int64()
+        check(lib().xg_tran_evicted(self._tr, flags.ctypes.data, C.byref(n)), "evicted")
+        return [int(c) for c in np.flatnonzero(flags[:nc.value])]
 
     # ---- checkpoint / restart (N_IO_RestartMgr's role for a transient)
     _CKPT = ("x_curr", "xh1", "xh2", "sta_c", "sto_c", "qn0", "fprev")
