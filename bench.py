@@ -131,8 +131,12 @@ def parse():
     ap.add_argument("--big-legs", default="logic,grid",
                     help="large single-circuit transients to run (logic, grid)")
     ap.add_argument("--big-span", type=float, default=1.0,
-                    help="fraction of the large transients' spans (5 ns logic, 2 ns grid) "
-                         "to simulate; 1 is the configured workload")
+                    help="fraction of the logic transient's 5 ns to simulate; 1 is the "
+                         "configured workload")
+    ap.add_argument("--grid-span", type=float, default=0.0005,
+                    help="fraction of the power grid's 2 ns to simulate (default 1 ps: "
+                         "its 100,000 clock phases put a PULSE corner, where the step "
+                         "control ends a step, every ~2.5 fs, so 2 ns takes ~8e5 steps)")
     ap.add_argument("--big-fixed", action="store_true",
                     help="large transients with fixed steps of --big-dt instead of the "
                          "reference's step control")
@@ -318,12 +322,14 @@ def large_transient_leg(torch, args, which):
     from xyce_amd.transient import Transient
     if which == "logic":
         nl = circuits.random_logic_bsim3(n_gates=args.gates, seed=2, ramp=50e-12)
-        tstop, probes, uic = 5e-9 * args.big_span, ["g0", "g1"], True
+        full = 5e-9
+        tstop, probes, uic = full * args.big_span, ["g0", "g1"], True
         name = ("random CMOS logic, %d NAND2/NOR2 gates, %d BSIM3 (seed 2), supply ramped "
                 "over 50 ps" % (args.gates, 4 * args.gates))
     else:
         nl = circuits.power_grid_bsim3()
-        tstop, probes, uic = 2e-9 * args.big_span, ["out0", "g250_250"], False
+        full = 2e-9
+        tstop, probes, uic = full * args.grid_span, ["out0", "g250_250"], False
         name = "500x500 power grid + 100,000 BSIM3 inverters (seed 5)"
     progress("large transient (%s): setup" % which)
     t0 = time.perf_counter()
@@ -370,6 +376,10 @@ def large_transient_leg(torch, args, which):
         "fronts": {"count": int(info[1]), "levels": int(info[2]), "values": int(info[3]),
                    "largest": int(info[4]), "flops_per_factorisation": int(info[5])},
         "final_probe_V": [float(v) for v in w[:, -1]],
+        # the PULSE corners the step control must land on: in the span run
+        # and in the configured .TRAN
+        "breakpoints": {"in_span": len(nl.breakpoints(tstop)),
+                        "in_configured_tran": len(nl.breakpoints(full))},
     }
 
 

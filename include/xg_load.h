@@ -409,12 +409,15 @@ int xg_lu_front_export(const xg_handle *h, int64_t sizes[6], int32_t *const *i32
                        int64_t *const *i64);
 
 /* Linear-solve form of a circuit, set before xg_finalize: XG_LU_AUTO
-   (default) factors components by task lists within their budgets and by
-   dense fronts beyond them; XG_LU_FRONTS factors every component by dense
-   fronts.  No counterpart in the reference (a solver choice, like its
+   (default) factors a component by task lists when they fit in one
+   workgroup's LDS and within their budgets, else by dense fronts;
+   XG_LU_FRONTS factors every component by dense fronts; XG_LU_TASKS uses
+   task lists wherever their budgets allow (from global memory beyond the
+   LDS).  No counterpart in the reference (a solver choice, like its
    linear-solver options). */
 #define XG_LU_AUTO 0
 #define XG_LU_FRONTS 1
+#define XG_LU_TASKS 2
 int xg_circuit_set_lu_mode(xg_circuit *c, int32_t mode);
 
 /* Host copy of the sparse LU plan, for inspection: sizes[0..5] = n_nodes,

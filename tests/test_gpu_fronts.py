@@ -56,7 +56,7 @@ def test_fronts_solve_like_lapack_and_the_task_lists(name):
     nl = _cases()[name].layout()
     print("case:", name, flush=True)
     fr = DeviceLoad(nl, lu_fronts=True)
-    tk = DeviceLoad(nl)
+    tk = DeviceLoad(nl, lu_mode="tasks")
     info = np.zeros(6, np.int64)
     from xyce_amd._lib import lib
     lib().xg_lu_front_info(fr.h, info.ctypes.data)
@@ -76,6 +76,9 @@ def test_fronts_solve_like_lapack_and_the_task_lists(name):
         if np.linalg.matrix_rank(A) < n:
             continue
         assert np.all(st[:, 2] == 0), st[:, 2]
+        if trial == 0:
+            # a dominant diagonal: the fronts solve it, no pivoting fallback
+            assert fr.lu_info()["fallbacks"] == 0
         want = np.linalg.solve(A, rhs[:n])
         berr = np.abs(A @ x[:n] - rhs[:n]).max() / (np.abs(A).sum(1).max() *
                                                     np.abs(x[:n]).max() + np.abs(rhs[:n]).max())
@@ -240,3 +243,52 @@ def test_many_workgroup_fronts_give_the_same_bits(gates, big_m):
         out.append(xs)
         dev.close()
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("big_m", [256, 48])
+def test_fronts_pivot_on_random_values_without_fallback(big_m):
+    """Random normal values on a 700-gate logic pattern (2,800 unknowns,
+    beyond the dense fallback's 2048): the fronts must pivot at most
+    columns, and the solve must stand on its own -- not singular, backward
+    error at rounding level, equal to SuperLU -- in both the one-workgroup
+    and the many-workgroup forms, with the same bits in both."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+    from xyce_amd import circuits
+    from xyce_amd._lib import check, lib
+    from xyce_amd.loader import DeviceLoad
+    nl = circuits.random_logic_bsim3(n_gates=700, seed=9).layout()
+    fr = DeviceLoad(nl, lu_fronts=True)
+    check(lib().xg_lu_front_set_big(fr.h, big_m), "set big")
+    rp, ci = fr.pattern()
+    n = fr.n_rows - 1
+    rng = np.random.default_rng(5)
+    vals = rng.standard_normal(len(ci))
+    rhs = rng.standard_normal(n + 1)
+    rhs[n] = 0.0
+    x, st = _solve(fr, vals, rhs, torch)
+    comp = np.zeros(n, np.int32)
+    nc = ctypes.c_int32()
+    check(lib().xg_components(fr.h, ctypes.byref(nc), comp.ctypes.data), "components")
+    big = int(np.argmax(np.bincount(comp)))
+    assert np.bincount(comp)[big] > 2048
+    assert st[big, 2] == 0, st[big]
+    rows_of = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    m = (rows_of < n) & (ci < n)
+    A = sp.csr_matrix((vals[m], (rows_of[m], ci[m])), shape=(n, n))
+    idx = np.flatnonzero(comp == big)
+    Ab = A[idx][:, idx]
+    xb = x[idx]
+    r = Ab @ xb - rhs[idx]
+    berr = np.abs(r).max() / (abs(Ab).sum(1).max() * np.abs(xb).max() + np.abs(rhs[idx]).max())
+    print("big_m %d: backward error %.3g" % (big_m, berr), flush=True)
+    assert berr < 1e-12
+    ref = spla.spsolve(Ab.tocsc(), rhs[idx])
+    assert np.abs(xb - ref).max() <= 1e-6 * np.abs(ref).max()
+    # the one-workgroup form gives the same bits
+    check(lib().xg_lu_front_set_big(fr.h, 1 << 30), "set big")
+    x1, _ = _solve(fr, vals, rhs, torch)
+    assert x1.tobytes() == x.tobytes()
+    fr.close()
+

@@ -284,7 +284,8 @@ def test_sparse_lu_against_numpy(mode):
 
 def test_sparse_lu_global_memory_form():
     """A component too large for LDS (an RC ladder of 6000 nodes) takes
-    the global-memory form of the sparse LU; checked against a sparse
+    the global-memory form of the task-list LU when task lists are asked
+    for (automatically it takes dense fronts); checked against a sparse
     direct solve."""
     import ctypes
     import scipy.sparse as sp
@@ -299,7 +300,8 @@ def test_sparse_lu_global_memory_form():
         nl.add_res("R%d" % k, "n%d" % k, "n%d" % (k + 1), 100.0)
         nl.add_cap("C%d" % k, "n%d" % (k + 1), "0", 1e-15)
     nl.layout()
-    dev = DeviceLoad(nl)
+    assert DeviceLoad(nl, plan_only=True).lu_info()["global_components"] == 0
+    dev = DeviceLoad(nl, lu_mode="tasks")
     info = dev.lu_info()
     assert info["global_components"] == 1
     rp, ci = dev.pattern()

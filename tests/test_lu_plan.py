@@ -141,7 +141,7 @@ CASES = [
 def test_plan_solves_like_lapack(case):
     nl = CASES[case]()
     nl.layout()
-    dev = DeviceLoad(nl, plan_only=True)
+    dev = DeviceLoad(nl, plan_only=True, lu_mode="tasks")
     rp, ci = dev.pattern()
     n = dev.n_rows - 1
     rng = np.random.default_rng(case)
@@ -168,7 +168,7 @@ def test_levels_stay_shallow():
     dozen dependent levels (the natural order would need ~300)."""
     nl = circuits.ring_oscillator_bsim3(301, seed=3)
     nl.layout()
-    info = DeviceLoad(nl, plan_only=True).lu_info()
+    info = DeviceLoad(nl, plan_only=True, lu_mode="tasks").lu_info()
     assert info["max_factor_levels"] <= 48
     assert info["fnnz"] < 12 * 303
 
@@ -177,13 +177,13 @@ def test_same_plan_alone_and_in_batch():
     parts = circuits.monte_carlo_rings(3, seed=5, stages=(11, 31))
     batch = circuits.merge(parts)
     batch.layout()
-    B = DeviceLoad(batch, plan_only=True)
+    B = DeviceLoad(batch, plan_only=True, lu_mode="tasks")
     PB = B.lu_plan()
     _, cs = _comp_starts(B)
     for k in range(3):
         p = circuits.monte_carlo_rings(3, seed=5, stages=(11, 31))[k]
         p.layout()
-        A = DeviceLoad(p, plan_only=True)
+        A = DeviceLoad(p, plan_only=True, lu_mode="tasks")
         PA = A.lu_plan()
         la = PA["lv_comp"]
         lb = PB["lv_comp"][3 * k:3 * k + 4]
@@ -217,7 +217,7 @@ def test_real_jacobians(which):
     r.run(0.1e-9, 1e-12, ["out0"], every=50)
     nl = mk()
     nl.layout()
-    dev = DeviceLoad(nl, plan_only=True)
+    dev = DeviceLoad(nl, plan_only=True, lu_mode="tasks")
     n = dev.n_rows - 1
     idx = np.arange(n)
     passed = 0
@@ -233,6 +233,26 @@ def test_real_jacobians(which):
     assert passed >= 0.8 * len(mats[::5])
 
 
+def test_automatic_form_keeps_task_lists_in_lds():
+    """By default a component takes task lists only when they fit in one
+    workgroup's LDS; a 400-gate random-logic request (its task lists would
+    need ~1 MB) is factored by dense fronts, a 301-stage ring by task
+    lists."""
+    nl = circuits.random_logic_bsim3(n_gates=400, seed=7)
+    nl.layout()
+    dev = DeviceLoad(nl, plan_only=True)
+    info = dev.lu_info()
+    fp = dev.front_plan()
+    # the request's one large component is on fronts; only its few small
+    # components (sources with their own nodes) keep task lists
+    assert info["fnnz"] < 1000
+    assert int(fp["front_comp"].sum()) == 1 and int(fp["m"].max()) > 100
+    ring = circuits.ring_oscillator_bsim3(301, seed=3)
+    ring.layout()
+    info = DeviceLoad(ring, plan_only=True).lu_info()
+    assert info["fnnz"] > 0 and info["global_components"] == 0
+
+
 def test_random_logic_takes_the_shorter_chain():
     """Random logic is expander-like: the minimum-degree order, chosen where
     its elimination chain is shorter than the dissection's (about a third
@@ -240,7 +260,7 @@ def test_random_logic_takes_the_shorter_chain():
     400-gate request's factorisation under 400 levels and 40k entries."""
     nl = circuits.random_logic_bsim3(n_gates=400, seed=7)
     nl.layout()
-    info = DeviceLoad(nl, plan_only=True).lu_info()
+    info = DeviceLoad(nl, plan_only=True, lu_mode="tasks").lu_info()
     print(info)
     assert info["static_fail"] == 0
     assert info["max_factor_levels"] < 400

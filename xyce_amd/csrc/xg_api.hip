@@ -48,6 +48,10 @@ struct Handle {
   SparseLuArgs lu{};
   FrontArgs fr{};
   std::vector<int32_t> fr_lv_ptr, fr_lv_maxm;  // host copies for the launches
+  // fronts of more than fr.big_m rows by level (host, offsets, device)
+  std::vector<int32_t> fr_big, fr_big_ptr;
+  int32_t *fr_big_dev = nullptr;
+  std::vector<int64_t> fr_xr_off;  // receiving rows per front (host copy)
   int32_t *d_rowptr = nullptr, *d_colidx = nullptr;
   int32_t *slot_comp = nullptr;
   int32_t *node_comp = nullptr;  // node -> component (ground excluded)
@@ -140,9 +144,9 @@ int xg_circuit_new(xg_circuit **c, int32_t n_nodes, int32_t n_state,
 void xg_circuit_free(xg_circuit *c) { delete c; }
 
 int xg_circuit_set_lu_mode(xg_circuit *c, int32_t mode) {
-  if (!c || (mode != XG_LU_AUTO && mode != XG_LU_FRONTS))
+  if (!c || (mode != XG_LU_AUTO && mode != XG_LU_FRONTS && mode != XG_LU_TASKS))
     return fail(XG_E_ARG, "xg_circuit_set_lu_mode: bad argument");
-  c->c.opt.lu_fronts = mode == XG_LU_FRONTS ? 1 : 0;
+  c->c.opt.lu_mode = mode;
   return XG_OK;
 }
 
@@ -456,6 +460,10 @@ int xg_lu_solve(xg_handle *H, const double *J, const double *rhs, double *x,
     f.k_host = L.fr_k.data();
     f.ch_ptr_host = L.fr_ch_ptr.data();
     f.ch_host = L.fr_ch.data();
+    f.big = h.fr_big_dev;
+    f.big_host = h.fr_big.data();
+    f.big_ptr_host = h.fr_big_ptr.data();
+    f.xr_off_host = h.fr_xr_off.data();
     e = xg::launch_fronts(f, u, h.stream);
     if (e != hipSuccess) return hip_fail(e, "front LU");
   }
@@ -506,9 +514,28 @@ int xg_lu_info(xg_handle *H, int64_t *info) {
   return XG_OK;
 }
 
+// The fronts of more than big_m rows grouped by level, on the host and (once
+// uploaded) on the device.
+static int front_big_lists(xg::Handle &h) {
+  const xg::LuPlan &L = h.plan.lu;
+  h.fr_big.clear();
+  h.fr_big_ptr.assign(1, 0);
+  for (size_t l = 0; l + 1 < L.fr_lv_ptr.size(); ++l) {
+    for (int32_t q = L.fr_lv_ptr[l]; q < L.fr_lv_ptr[l + 1]; ++q)
+      if (L.fr_m[L.fr_lv[q]] > h.fr.big_m) h.fr_big.push_back(L.fr_lv[q]);
+    h.fr_big_ptr.push_back((int32_t)h.fr_big.size());
+  }
+  if (h.fr_big_dev && !h.fr_big.empty() &&
+      hipMemcpy(h.fr_big_dev, h.fr_big.data(), sizeof(int32_t) * h.fr_big.size(),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return XG_E_HIP;
+  return XG_OK;
+}
+
 int xg_lu_front_set_big(xg_handle *H, int32_t big_m) {
   if (!H || big_m < 1) return fail(XG_E_ARG, "xg_lu_front_set_big");
   H->h.fr.big_m = big_m;
+  if (front_big_lists(H->h)) return fail(XG_E_HIP, "xg_lu_front_set_big: upload");
   return XG_OK;
 }
 
@@ -893,6 +920,92 @@ static bool upload_handle(xg_handle *H) {
       chk(f.v = h.alloc<double>((size_t)(P.n_nodes > 0 ? P.n_nodes : 1)));
       chk(f.piv = h.alloc<int32_t>((size_t)L.fr_ro.back()));
       chk(f.bad = h.alloc<int32_t>((size_t)(u.n_comp > 0 ? u.n_comp : 1)));
+      chk(h.fr_big_dev = h.alloc<int32_t>((size_t)f.n_fronts));
+      if (ok && front_big_lists(h)) ok = false;
+      // the children's contribution rows grouped by the row they land on
+      // (children in order within each row), for the many-workgroup form
+      {
+        const int32_t nf = f.n_fronts;
+        std::vector<int64_t> xr_off(nf + 1, 0), ext_ptr(1, 0);
+        std::vector<int32_t> xr_row, ext_ch, ext_i, cnt, at;
+        for (int32_t s = 0; s < nf; ++s) {
+          const int32_t m = L.fr_m[s];
+          cnt.assign(m, 0);
+          for (int32_t c = L.fr_ch_ptr[s]; c < L.fr_ch_ptr[s + 1]; ++c) {
+            const int32_t ch = L.fr_ch[c], kc = L.fr_k[ch], w = L.fr_m[ch] - kc;
+            for (int32_t i = 0; i < w; ++i) cnt[L.fr_cbmap[L.fr_ro[ch] + kc + i]]++;
+          }
+          at.assign(m, 0);
+          int64_t run = ext_ptr.back();
+          const int64_t base = run;
+          for (int32_t r = 0; r < m; ++r) {
+            if (!cnt[r]) continue;
+            xr_row.push_back(r);
+            at[r] = (int32_t)(run - base);
+            run += cnt[r];
+            ext_ptr.push_back(run);
+          }
+          ext_ch.resize((size_t)run);
+          ext_i.resize((size_t)run);
+          for (int32_t c = L.fr_ch_ptr[s]; c < L.fr_ch_ptr[s + 1]; ++c) {
+            const int32_t ch = L.fr_ch[c], kc = L.fr_k[ch], w = L.fr_m[ch] - kc;
+            for (int32_t i = 0; i < w; ++i) {
+              const int64_t e = base + at[L.fr_cbmap[L.fr_ro[ch] + kc + i]]++;
+              ext_ch[(size_t)e] = ch;
+              ext_i[(size_t)e] = i;
+            }
+          }
+          xr_off[s + 1] = (int64_t)xr_row.size();
+        }
+        h.fr_xr_off = xr_off;
+        chk(f.xr_off = h.upload(xr_off));
+        chk(f.xr_row = h.upload(xr_row));
+        chk(f.ext_ptr = h.upload(ext_ptr));
+        chk(f.ext_ch = h.upload(ext_ch));
+        chk(f.ext_i = h.upload(ext_i));
+      }
+      // rows of more than kLongRow entries of the front components
+      {
+        std::vector<int32_t> lptr(1, 0), lrows;
+        for (int32_t c = 0; c < u.n_comp; ++c) {
+          if (L.front_comp[c]) {
+            const int32_t beg = P.comp_start[c], n = P.comp_start[c + 1] - beg;
+            const int32_t *rs = L.ent_rs.data() + beg + c;
+            for (int32_t i = 0; i < n; ++i)
+              if (rs[i + 1] - rs[i] > xg::kLongRow) lrows.push_back(i);
+          }
+          lptr.push_back((int32_t)lrows.size());
+        }
+        chk(f.long_ptr = h.upload(lptr));
+        chk(f.long_rows = h.upload(lrows));
+      }
+      // chunked finish of the largest front components
+      {
+        std::vector<int32_t> split(u.n_comp > 0 ? u.n_comp : 1, 0), ch_comp, ch_beg, comps,
+            cptr(1, 0);
+        for (int32_t c = 0; c < u.n_comp; ++c) {
+          const int32_t n = P.comp_start[c + 1] - P.comp_start[c];
+          if (!L.front_comp[c] || n <= xg::kFinishSplit) continue;
+          split[c] = 1;
+          comps.push_back(c);
+          for (int32_t r = 0; r < n; r += xg::kFinishChunk) {
+            ch_comp.push_back(c);
+            ch_beg.push_back(r);
+          }
+          cptr.push_back((int32_t)ch_comp.size());
+        }
+        if (!comps.empty()) {
+          f.n_fin_chunks = (int32_t)ch_comp.size();
+          f.n_fin_comps = (int32_t)comps.size();
+          chk(f.fin_split = h.upload(split));
+          chk(f.fin_chunk_comp = h.upload(ch_comp));
+          chk(f.fin_chunk_beg = h.upload(ch_beg));
+          chk(f.fin_comps = h.upload(comps));
+          chk(f.fin_cptr = h.upload(cptr));
+          chk(f.fin_part = h.alloc<double>((size_t)4 * ch_comp.size()));
+          chk(f.fin_good = h.alloc<int32_t>((size_t)u.n_comp));
+        }
+      }
       u.front = f.front_comp;
     }
     if (ok && hipMemset(u.fallback, 0, sizeof(int32_t) * (u.n_comp > 0 ? u.n_comp : 1)) !=

@@ -39,9 +39,11 @@ namespace {
 constexpr int kFB = 256;           // threads per front workgroup
 constexpr int kFinishBlock = 1024;  // threads of the per-component check
 
+// (value, smallest index) maximum over the block: per wavefront, then
+// across the B / 64 wavefronts.
+template <int B>
 __device__ __forceinline__ void blk_argmax(double v, int idx, double *sv, int *si,
                                            double &best, int &bi) {
-  // wave-level (value, smallest index) maximum, then across the waves
   for (int off = 32; off >= 1; off >>= 1) {
     const double ov = __shfl_xor(v, off);
     const int oi = __shfl_xor(idx, off);
@@ -58,7 +60,7 @@ __device__ __forceinline__ void blk_argmax(double v, int idx, double *sv, int *s
   __syncthreads();
   best = sv[0];
   bi = si[0];
-  for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+  for (int q = 1; q < B / 64; ++q)
     if (sv[q] > best || (sv[q] == best && si[q] < bi)) {
       best = sv[q];
       bi = si[q];
@@ -94,7 +96,7 @@ __device__ __forceinline__ double blk_reduce(double v, double *sv, bool is_max) 
 // many-workgroup form below, which applies the swaps left of swap_lo
 // afterwards (k0 = 0, k1 = k, cl = m, swap_lo = 0 is the whole
 // factorization).
-template <int NB>
+template <int NB, int BS = kFB>
 __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int *si,
                              int *spiv, int32_t *piv, int k0, int k1, int cl,
                              int swap_lo = 0) {
@@ -104,7 +106,7 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
     const int nb = min(NB, k1 - kb);
     const int rows = m - kb;      // panel rows
     const int cand = k - kb;      // rows eligible as pivots
-    for (int e = t; e < rows * NB; e += kFB) {
+    for (int e = t; e < rows * NB; e += BS) {
       const int r = e / NB, c = e % NB;
       P[r * PS + c] = c < nb ? A[(int64_t)(kb + r) * m + kb + c] : 0.0;
     }
@@ -112,7 +114,7 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
     for (int c = 0; c < nb; ++c) {
       double v = -1.0;
       int idx = 0x7fffffff;
-      for (int r = c + t; r < cand; r += kFB) {
+      for (int r = c + t; r < cand; r += BS) {
         const double a = fabs(P[r * PS + c]);
         if (a > v) {
           v = a;
@@ -121,31 +123,31 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
       }
       double best;
       int br;
-      blk_argmax(v, idx, sv, si, best, br);
+      blk_argmax<BS>(v, idx, sv, si, best, br);
       if (!(best > 0.0) || !(best <= DBL_MAX)) return false;  // same on every thread
       if (t == 0) spiv[c] = kb + br;
       if (br != c)
-        for (int cc = t; cc < NB; cc += kFB) {
+        for (int cc = t; cc < NB; cc += BS) {
           const double tmp = P[c * PS + cc];
           P[c * PS + cc] = P[br * PS + cc];
           P[br * PS + cc] = tmp;
         }
       __syncthreads();
       const double pv = P[c * PS + c];
-      for (int r = c + 1 + t; r < rows; r += kFB) {
+      for (int r = c + 1 + t; r < rows; r += BS) {
         const double l = P[r * PS + c] / pv;
         P[r * PS + c] = l;
         for (int cc = c + 1; cc < nb; ++cc) P[r * PS + cc] -= l * P[c * PS + cc];
       }
       __syncthreads();
     }
-    for (int e = t; e < rows * NB; e += kFB) {
+    for (int e = t; e < rows * NB; e += BS) {
       const int r = e / NB, c = e % NB;
       if (c < nb) A[(int64_t)(kb + r) * m + kb + c] = P[r * PS + c];
     }
     if (t < nb) piv[kb + t] = spiv[t];
     // the panel's row swaps on every other column, in pivot order
-    for (int j = swap_lo + t; j < cl; j += kFB) {
+    for (int j = swap_lo + t; j < cl; j += BS) {
       if (j >= kb && j < kb + nb) continue;
       for (int c = 0; c < nb; ++c) {
         const int pr = spiv[c];
@@ -190,8 +192,8 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
         }
       }
     };
-    if (ncols >= kFB) {
-      for (int j = j0 + t; j < cl; j += kFB) {
+    if (ncols >= BS) {
+      for (int j = j0 + t; j < cl; j += BS) {
         double u[NB];
         solve_u(j, u);
 #pragma unroll
@@ -200,7 +202,7 @@ __device__ bool front_factor(double *A, int m, int k, double *P, double *sv, int
         sweep(j, 0, 1, u);
       }
     } else if (ncols > 0) {
-      const int G = min(kFB / ncols, 16);
+      const int G = min(BS / ncols, 16);
       const bool mine = t < ncols * G;
       const int j = j0 + t % ncols, g = t / ncols;
       double u[NB];
@@ -241,11 +243,11 @@ __global__ void __launch_bounds__(kFB) front_scatter_kernel(FrontArgs f, const d
     f.vals[f.ent_fpos[e]] = J[f.ent_k[e]];
 }
 
-template <int NB, int ROWS>
-__global__ void __launch_bounds__(kFB) front_factor_kernel(FrontArgs f, int lv0) {
+template <int NB, int ROWS, int BS>
+__global__ void __launch_bounds__(BS) front_factor_kernel(FrontArgs f, int lv0) {
   __shared__ double P[ROWS * (NB + 1)];
-  __shared__ double sv[kFB / 64];
-  __shared__ int si[kFB / 64];
+  __shared__ double sv[BS / 64];
+  __shared__ int si[BS / 64];
   __shared__ int spiv[NB];
   const int s = f.lv[lv0 + blockIdx.x];
   const int c = f.comp[s];
@@ -260,50 +262,61 @@ __global__ void __launch_bounds__(kFB) front_factor_kernel(FrontArgs f, int lv0)
     const int mc = f.m[ch], kc = f.k[ch], w = mc - kc;
     const double *C = f.vals + f.fo[ch] + (int64_t)kc * mc + kc;
     const int32_t *map = f.cbmap + f.ro[ch] + kc;
-    for (int64_t e = t; e < (int64_t)w * w; e += kFB) {
+    for (int64_t e = t; e < (int64_t)w * w; e += BS) {
       const int i = (int)(e / w), j = (int)(e % w);
       A[(int64_t)map[i] * m + map[j]] += C[(int64_t)i * mc + j];
     }
     __syncthreads();
   }
-  if (!front_factor<NB>(A, m, k, P, sv, si, spiv, f.piv + f.ro[s], 0, k, m))
+  if (!front_factor<NB, BS>(A, m, k, P, sv, si, spiv, f.piv + f.ro[s], 0, k, m))
     if (t == 0) f.bad[c] = 1;
 }
 
-// ---- many-workgroup form of one large front (rows m > big_m): per panel of
-// kBigW fully summed columns, one workgroup factors the panel (swaps on
-// the columns left of the panel's end), then the columns right of it take
-// the panel's row swaps and their U12 rows (one thread per column), then
-// the trailing block takes the rank-kBigW update in 64 x 64 tiles.  Every
-// element sees the same operations in the same order as in the
-// one-workgroup form, so both give the same bits.
+// ---- many-workgroup form of the large fronts (rows m > big_m), all of a
+// level's at once (blockIdx.y or .z = the front among the level's large
+// ones, f.big[lb + that]): per panel of kBigW fully summed columns, one
+// workgroup per front factors the panel (swaps on the columns left of the
+// panel's end), then the columns right of it take the panel's row swaps
+// and their U12 rows (one thread per column), then the trailing block
+// takes the rank-kBigW update in 64 x 64 tiles.  A front whose columns
+// are all factored leaves the later panel launches at once.  Every element
+// sees the same operations in the same order as in the one-workgroup
+// form, so both give the same bits.
 constexpr int kBigW = 32;   // panel width
 constexpr int kTile = 64;   // trailing-update tile
 
-__global__ void __launch_bounds__(kFB) front_extend_kernel(FrontArgs f, int s, int ch) {
+// The children's contribution blocks, one workgroup per receiving row of
+// a front (blockIdx.x = the row among the front's receiving rows): the
+// child rows that land on it, in child order, each added across its
+// columns, so every element takes its children's terms in child order.
+__global__ void __launch_bounds__(kFB) front_extend_kernel(FrontArgs f, int lb) {
+  const int s = f.big[lb + blockIdx.y];
   if (front_skip(f, f.comp[s])) return;
+  const int64_t q = f.xr_off[s] + blockIdx.x;
+  if (q >= f.xr_off[s + 1]) return;
   const int m = f.m[s];
-  double *A = f.vals + f.fo[s];
-  const int mc = f.m[ch], kc = f.k[ch], w = mc - kc;
-  const double *C = f.vals + f.fo[ch] + (int64_t)kc * mc + kc;
-  const int32_t *map = f.cbmap + f.ro[ch] + kc;
-  const int64_t n = (int64_t)w * w;
-  for (int64_t e = (int64_t)blockIdx.x * kFB + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * kFB) {
-    const int i = (int)(e / w), j = (int)(e % w);
-    A[(int64_t)map[i] * m + map[j]] += C[(int64_t)i * mc + j];
+  double *Ar = f.vals + f.fo[s] + (int64_t)f.xr_row[q] * m;
+  for (int64_t e = f.ext_ptr[q]; e < f.ext_ptr[q + 1]; ++e) {
+    const int ch = f.ext_ch[e], i = f.ext_i[e];
+    const int mc = f.m[ch], kc = f.k[ch], w = mc - kc;
+    const double *C = f.vals + f.fo[ch] + (int64_t)(kc + i) * mc + kc;
+    const int32_t *map = f.cbmap + f.ro[ch] + kc;
+    for (int j = threadIdx.x; j < w; j += kFB) Ar[map[j]] += C[j];
+    __syncthreads();
   }
 }
 
 template <int NB, int ROWS>
-__global__ void __launch_bounds__(kFB) front_panel_kernel(FrontArgs f, int s, int kb) {
+__global__ void __launch_bounds__(kFB) front_panel_kernel(FrontArgs f, int lb, int kb) {
   __shared__ double P[ROWS * (NB + 1)];
   __shared__ double sv[kFB / 64];
   __shared__ int si[kFB / 64];
   __shared__ int spiv[NB];
+  const int s = f.big[lb + blockIdx.x];
   const int c = f.comp[s];
   if (front_skip(f, c)) return;
   const int m = f.m[s], k = f.k[s];
+  if (kb >= k) return;
   const int cl = min(kb + kBigW, k);
   if (!front_factor<NB>(f.vals + f.fo[s], m, k, P, sv, si, spiv, f.piv + f.ro[s], kb, cl,
                         cl, kb))
@@ -313,12 +326,15 @@ __global__ void __launch_bounds__(kFB) front_panel_kernel(FrontArgs f, int s, in
 // Columns j >= cl: the panel's row swaps in pivot order, then U12 = L11^-1
 // (rows kb..cl-1) with the unit lower panel block, each u[i] updated in
 // ascending pivot order.
-__global__ void __launch_bounds__(kFB) front_swap_trsm_kernel(FrontArgs f, int s, int kb) {
+__global__ void __launch_bounds__(kFB) front_swap_trsm_kernel(FrontArgs f, int lb, int kb) {
   __shared__ double L11[kBigW][kBigW + 1];
   __shared__ int32_t pv[kBigW];
+  const int s = f.big[lb + blockIdx.y];
   if (front_skip(f, f.comp[s])) return;
   const int m = f.m[s], k = f.k[s];
+  if (kb >= k) return;
   const int cl = min(kb + kBigW, k), nw = cl - kb;
+  if (cl + (int)(blockIdx.x * kFB) >= m) return;
   double *A = f.vals + f.fo[s];
   const int32_t *piv = f.piv + f.ro[s];
   for (int e = threadIdx.x; e < kBigW * kBigW; e += kFB) {
@@ -351,12 +367,14 @@ __global__ void __launch_bounds__(kFB) front_swap_trsm_kernel(FrontArgs f, int s
 }
 
 // Rows and columns >= cl: A -= L21 U12, each element's terms in ascending
-// pivot order.  Grid: (tiles over columns, tiles over rows).
-__global__ void __launch_bounds__(kFB) front_update_kernel(FrontArgs f, int s, int kb) {
+// pivot order.  Grid: (tiles over columns, tiles over rows, fronts).
+__global__ void __launch_bounds__(kFB) front_update_kernel(FrontArgs f, int lb, int kb) {
   __shared__ double Lt[kTile][kBigW + 1];
   __shared__ double Ut[kBigW][kTile + 1];
+  const int s = f.big[lb + blockIdx.z];
   if (front_skip(f, f.comp[s])) return;
   const int m = f.m[s], k = f.k[s];
+  if (kb >= k) return;
   const int cl = min(kb + kBigW, k), nw = cl - kb;
   double *A = f.vals + f.fo[s];
   const int r0 = cl + blockIdx.y * kTile, c0 = cl + blockIdx.x * kTile;
@@ -386,7 +404,8 @@ __global__ void __launch_bounds__(kFB) front_update_kernel(FrontArgs f, int s, i
 // The row swaps of the later panels on the factored columns j < k (the L
 // part), deferred from the panels: column j takes every pivot after its
 // own panel, in pivot order, as the one-workgroup form applies them.
-__global__ void __launch_bounds__(kFB) front_left_swaps_kernel(FrontArgs f, int s) {
+__global__ void __launch_bounds__(kFB) front_left_swaps_kernel(FrontArgs f, int lb) {
+  const int s = f.big[lb + blockIdx.y];
   if (front_skip(f, f.comp[s])) return;
   const int m = f.m[s], k = f.k[s];
   const int j = blockIdx.x * kFB + threadIdx.x;
@@ -407,8 +426,14 @@ __global__ void __launch_bounds__(kFB) front_left_swaps_kernel(FrontArgs f, int 
 // fully summed rows' entries, zero below), its children's remainders,
 // its row swaps, then L y = b with the unit lower columns; y goes to the
 // position vector, the remainder below stays in wrhs for the parent.
+// Blocked by 64 columns: the diagonal block on one wavefront (the
+// solution broadcast by lane shuffles), then every row below takes the
+// block's 64 terms.  Each element takes its terms in ascending column
+// order, as the column-by-column algorithm gives them.
+template <int ROWS>
 __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const double *bp,
                                                         int lv0) {
+  __shared__ double ws[ROWS];
   const int s = f.lv[lv0 + blockIdx.x];
   const int c = f.comp[s];
   if (front_skip(f, c)) return;
@@ -417,14 +442,14 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
   const double *A = f.vals + f.fo[s];
   const int32_t *rows = f.rows + f.ro[s];
   double *w = f.wrhs + f.ro[s];
-  for (int i = t; i < m; i += kFB) w[i] = i < k ? bp[rows[i]] : 0.0;
+  for (int i = t; i < m; i += kFB) ws[i] = i < k ? bp[rows[i]] : 0.0;
   __syncthreads();
   for (int q = f.ch_ptr[s]; q < f.ch_ptr[s + 1]; ++q) {
     const int ch = f.ch[q];
     const int kc = f.k[ch], wc = f.m[ch] - kc;
     const double *r = f.wrhs + f.ro[ch] + kc;
     const int32_t *map = f.cbmap + f.ro[ch] + kc;
-    for (int i = t; i < wc; i += kFB) w[map[i]] += r[i];
+    for (int i = t; i < wc; i += kFB) ws[map[i]] += r[i];
     __syncthreads();
   }
   if (t == 0) {
@@ -432,25 +457,48 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
     for (int j = 0; j < k; ++j) {
       const int p = pv[j];
       if (p != j) {
-        const double tmp = w[j];
-        w[j] = w[p];
-        w[p] = tmp;
+        const double tmp = ws[j];
+        ws[j] = ws[p];
+        ws[p] = tmp;
       }
     }
   }
   __syncthreads();
-  for (int j = 0; j < k; ++j) {
-    const double yj = w[j];
-    for (int i = j + 1 + t; i < m; i += kFB) w[i] -= A[(int64_t)i * m + j] * yj;
+  for (int jb = 0; jb < k; jb += 64) {
+    const int je = min(jb + 64, k);
+    if (t < 64) {
+      const int i = jb + t;
+      double wi = i < je ? ws[i] : 0.0;
+      for (int j = jb; j < je; ++j) {
+        const double yj = __shfl(wi, j - jb);
+        if (i > j && i < je) wi -= A[(int64_t)i * m + j] * yj;
+      }
+      if (i < je) ws[i] = wi;
+    }
+    __syncthreads();
+    for (int i = je + t; i < m; i += kFB) {
+      double r = ws[i];
+      const double *a = A + (int64_t)i * m;
+      for (int j = jb; j < je; ++j) r -= a[j] * ws[j];
+      ws[i] = r;
+    }
     __syncthreads();
   }
-  for (int i = t; i < k; i += kFB) f.v[rows[i]] = w[i];
+  for (int i = t; i < m; i += kFB) {
+    w[i] = ws[i];
+    if (i < k) f.v[rows[i]] = ws[i];
+  }
 }
 
 // Backward solve of one level's fronts (levels from the root down): the
 // rows below the fully summed block are solved already (by ancestors).
+// Blocked by 64 columns from the last: the diagonal block on one
+// wavefront, then every row above takes the block's terms; each element
+// takes its terms in descending column order, as the column-by-column
+// algorithm gives them.
+template <int ROWS>
 __global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
-  __shared__ double tb[kFrontMaxM];
+  __shared__ double tb[ROWS];
   const int s = f.lv[lv0 + blockIdx.x];
   const int c = f.comp[s];
   if (front_skip(f, c)) return;
@@ -465,14 +513,59 @@ __global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
     tb[i] = r;
   }
   __syncthreads();
-  for (int j = k - 1; j >= 0; --j) {
-    const double xj = tb[j] / A[(int64_t)j * m + j];
+  for (int jb = ((k - 1) / 64) * 64; jb >= 0 && k > 0; jb -= 64) {
+    const int je = min(jb + 64, k);
+    if (t < 64) {
+      const int i = jb + t;
+      double ti = i < je ? tb[i] : 0.0;
+      for (int j = je - 1; j >= jb; --j) {
+        double xj = 0.0;
+        if (i == j) xj = ti / A[(int64_t)j * m + j];
+        xj = __shfl(xj, j - jb);
+        if (i == j)
+          ti = xj;
+        else if (i < j)
+          ti -= A[(int64_t)i * m + j] * xj;
+      }
+      if (i < je) tb[i] = ti;
+    }
     __syncthreads();
-    if (t == 0) tb[j] = xj;
-    for (int i = t; i < j; i += kFB) tb[i] -= A[(int64_t)i * m + j] * xj;
+    for (int i = t; i < jb; i += kFB) {
+      double r = tb[i];
+      const double *a = A + (int64_t)i * m;
+      for (int j = je - 1; j >= jb; --j) r -= a[j] * tb[j];
+      tb[i] = r;
+    }
     __syncthreads();
   }
   for (int i = t; i < k; i += kFB) f.v[rows[i]] = tb[i];
+}
+
+// The residual and |J| row-sum maxima of a component's long rows in
+// [r0, r1) (rows of more than kLongRow entries, listed per component):
+// each row by the whole block, strided partial sums then a fixed tree.
+template <int B>
+__device__ void long_rows_check(const FrontArgs &f, const SparseLuArgs &a, int c, int r0,
+                                int r1, double *sv, double &rm, double &am) {
+  const int beg = a.cstart[c];
+  const int eb = a.ent_off[c];
+  const double *jp = a.Jp + eb;
+  const int32_t *ecol = a.ent_col + eb, *ers = a.ent_rs + beg + c;
+  const double *v = f.v + beg;
+  for (int q = f.long_ptr[c]; q < f.long_ptr[c + 1]; ++q) {
+    const int i = f.long_rows[q];
+    if (i < r0 || i >= r1) continue;
+    double s = 0.0, as = 0.0;
+    for (int j = ers[i] + threadIdx.x; j < ers[i + 1]; j += B) {
+      const double qv = jp[j];
+      s = s + qv * v[ecol[j]];
+      as = as + fabs(qv);
+    }
+    s = blk_reduce<B>(s, sv, false);
+    as = blk_reduce<B>(as, sv, false);
+    rm = fmax(rm, fabs(a.bp[beg + i] - s));
+    am = fmax(am, as);
+  }
 }
 
 // Per front component: backward error of the solution against the
@@ -486,6 +579,7 @@ __global__ void __launch_bounds__(kFinishBlock) front_finish_kernel(FrontArgs f,
   const int t = threadIdx.x;
   if (f.gate && *f.gate) return;
   if (!f.front_comp[c]) return;
+  if (f.fin_split && f.fin_split[c]) return;  // the chunked kernels below
   if (a.active && !a.active[c]) return;  // stats written by the task kernel
   const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
   double *stats = a.stats + 4 * (size_t)c;
@@ -500,6 +594,8 @@ __global__ void __launch_bounds__(kFinishBlock) front_finish_kernel(FrontArgs f,
     const double *v = f.v + beg;
     double rm = 0.0, am = 0.0, xm = 0.0;
     for (int i = t; i < n; i += kFinishBlock) {
+      xm = fmax(xm, fabs(v[i]));
+      if (ers[i + 1] - ers[i] > kLongRow) continue;  // below, by the block
       double r = a.bp[beg + i], as = 0.0;
       for (int j = ers[i]; j < ers[i + 1]; ++j) {
         const double q = jp[j];
@@ -508,8 +604,8 @@ __global__ void __launch_bounds__(kFinishBlock) front_finish_kernel(FrontArgs f,
       }
       rm = fmax(rm, fabs(r));
       am = fmax(am, as);
-      xm = fmax(xm, fabs(v[i]));
     }
+    long_rows_check<kFinishBlock>(f, a, c, 0, n, sv, rm, am);
     rm = blk_reduce<kFinishBlock>(rm, sv, true);
     am = blk_reduce<kFinishBlock>(am, sv, true);
     xm = blk_reduce<kFinishBlock>(xm, sv, true);
@@ -547,40 +643,172 @@ __global__ void __launch_bounds__(kFinishBlock) front_finish_kernel(FrontArgs f,
   }
 }
 
+// ---- the same finish for components of more than kFinishSplit unknowns,
+// in chunks of kFinishChunk rows: (1) per chunk the maxima of |rhs|, of
+// the residual, of the row sums of |J| and of |dx|; (2) per component
+// their maxima (exact in any order) and the verdict; (3) per chunk
+// x += dx and the chunk's sum of squared weighted steps; (4) per component
+// those sums in chunk order, the weighted RMS norm.
+__global__ void __launch_bounds__(kFB) finish_part_kernel(FrontArgs f, SparseLuArgs a) {
+  __shared__ double sv[kFB];
+  if (f.gate && *f.gate) return;
+  const int ch = blockIdx.x, t = threadIdx.x;
+  const int c = f.fin_chunk_comp[ch];
+  if (a.active && !a.active[c]) return;
+  const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
+  const int r0 = f.fin_chunk_beg[ch], r1 = min(n, r0 + kFinishChunk);
+  double rmax = 0.0, rm = 0.0, am = 0.0, xm = 0.0;
+  const bool bad = f.bad[c] != 0;
+  const int eb = a.ent_off[c];
+  const double *jp = a.Jp + eb;
+  const int32_t *ecol = a.ent_col + eb, *ers = a.ent_rs + beg + c;
+  const double *v = f.v + beg;
+  for (int i = r0 + t; i < r1; i += kFB) {
+    const double b = a.bp[beg + i];
+    rmax = fmax(rmax, fabs(b));
+    if (!bad) {
+      xm = fmax(xm, fabs(v[i]));
+      if (ers[i + 1] - ers[i] > kLongRow) continue;  // below, by the block
+      double r = b, as = 0.0;
+      for (int j = ers[i]; j < ers[i + 1]; ++j) {
+        const double q = jp[j];
+        r = r - q * v[ecol[j]];
+        as = as + fabs(q);
+      }
+      rm = fmax(rm, fabs(r));
+      am = fmax(am, as);
+    }
+  }
+  if (!bad) long_rows_check<kFB>(f, a, c, r0, r1, sv, rm, am);
+  double *p = f.fin_part + 4 * (size_t)ch;
+  rmax = blk_reduce<kFB>(rmax, sv, true);
+  rm = blk_reduce<kFB>(rm, sv, true);
+  am = blk_reduce<kFB>(am, sv, true);
+  xm = blk_reduce<kFB>(xm, sv, true);
+  if (t == 0) {
+    p[0] = rmax;
+    p[1] = rm;
+    p[2] = am;
+    p[3] = xm;
+  }
+}
+
+__global__ void __launch_bounds__(kFB) finish_decide_kernel(FrontArgs f, SparseLuArgs a) {
+  __shared__ double sv[kFB];
+  if (f.gate && *f.gate) return;
+  const int c = f.fin_comps[blockIdx.x], t = threadIdx.x;
+  if (a.active && !a.active[c]) return;
+  const int n = a.cstart[c + 1] - a.cstart[c];
+  double m4[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int ch = f.fin_cptr[blockIdx.x] + t; ch < f.fin_cptr[blockIdx.x + 1]; ch += kFB)
+    for (int q = 0; q < 4; ++q) m4[q] = fmax(m4[q], f.fin_part[4 * (size_t)ch + q]);
+  for (int q = 0; q < 4; ++q) m4[q] = blk_reduce<kFB>(m4[q], sv, true);
+  const bool good = !f.bad[c] && m4[1] <= a.max_berr * (m4[2] * m4[3] + m4[0]);
+  if (t == 0) {
+    f.fin_good[c] = good ? 1 : 0;
+    double *stats = a.stats + 4 * (size_t)c;
+    stats[0] = m4[0];
+    if (!good) {
+      stats[1] = 0.0;
+      stats[2] = 1.0;
+      stats[3] = (double)n;
+      a.fallback[c] = 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kFB) finish_update_kernel(FrontArgs f, SparseLuArgs a) {
+  __shared__ double sv[kFB];
+  if (f.gate && *f.gate) return;
+  const int ch = blockIdx.x, t = threadIdx.x;
+  const int c = f.fin_chunk_comp[ch];
+  if (a.active && !a.active[c]) return;
+  if (!f.fin_good[c]) return;
+  const int beg = a.cstart[c], n = a.cstart[c + 1] - beg;
+  const int r0 = f.fin_chunk_beg[ch], r1 = min(n, r0 + kFinishChunk);
+  const int32_t *qcol = a.qcol + beg;
+  double acc = 0.0;
+  for (int i = r0 + t; i < r1; i += kFB) {
+    const int g = qcol[i];
+    const double dx = f.v[beg + i];
+    const double w = a.reltol * fmax(fabs(a.wa[g]), fabs(a.wb[g])) + a.abstol;
+    const double q = dx / w;
+    acc = acc + q * q;
+    a.x[g] = a.x[g] + dx;
+  }
+  acc = blk_reduce<kFB>(acc, sv, false);
+  if (t == 0) f.fin_part[4 * (size_t)ch] = acc;
+}
+
+__global__ void __launch_bounds__(64) finish_sum_kernel(FrontArgs f, SparseLuArgs a) {
+  if (f.gate && *f.gate) return;
+  const int c = f.fin_comps[blockIdx.x];
+  if (a.active && !a.active[c]) return;
+  if (!f.fin_good[c] || threadIdx.x != 0) return;
+  const int n = a.cstart[c + 1] - a.cstart[c];
+  double ss = 0.0;
+  for (int ch = f.fin_cptr[blockIdx.x]; ch < f.fin_cptr[blockIdx.x + 1]; ++ch)
+    ss = ss + f.fin_part[4 * (size_t)ch];
+  double *stats = a.stats + 4 * (size_t)c;
+  stats[1] = sqrt(ss / n);
+  stats[2] = 0.0;
+  stats[3] = (double)n;
+}
+
+// One workgroup per front of the level: one wavefront for fronts of up
+// to 64 rows (cheap barriers, four times the workgroups per CU), four for
+// larger ones.  The block size changes which thread does an operation,
+// never an element's operations or their order.
 template <int NB, int ROWS>
 hipError_t launch_level(const FrontArgs &f, int lv0, int nfr, hipStream_t s) {
-  hipLaunchKernelGGL((front_factor_kernel<NB, ROWS>), dim3(nfr), dim3(kFB), 0, s, f, lv0);
+  if (ROWS <= 64)
+    hipLaunchKernelGGL((front_factor_kernel<NB, ROWS, 64>), dim3(nfr), dim3(64), 0, s, f, lv0);
+  else
+    hipLaunchKernelGGL((front_factor_kernel<NB, ROWS, kFB>), dim3(nfr), dim3(kFB), 0, s, f,
+                       lv0);
   return hipGetLastError();
 }
 
-// One front of more than big_m rows by many workgroups: its children's
-// contribution blocks (one launch per child, in child order), then per
-// panel the panel, swap / U12 and trailing-update launches.
-hipError_t launch_big_front(const FrontArgs &f, int sf, hipStream_t s) {
-  const int m = f.m_host[sf], k = f.k_host[sf];
-  for (int q = f.ch_ptr_host[sf]; q < f.ch_ptr_host[sf + 1]; ++q) {
-    const int ch = f.ch_host[q];
-    const int64_t w = (int64_t)f.m_host[ch] - f.k_host[ch];
-    const unsigned g = (unsigned)std::min<int64_t>((w * w + kFB - 1) / kFB + 1, 4096);
-    hipLaunchKernelGGL(front_extend_kernel, dim3(g), dim3(kFB), 0, s, f, sf, ch);
+// A level's fronts of more than big_m rows (f.big[lb, lb + nb)) by many
+// workgroups: their children's contribution blocks (one launch, a
+// workgroup per receiving row), then per panel position the panel,
+// swap / U12 and trailing-update launches over all of them, then the
+// deferred left swaps.
+hipError_t launch_big_fronts(const FrontArgs &f, int lb, int nb, hipStream_t s) {
+  int max_m = 0, max_k = 0;
+  int64_t max_rows = 0;
+  for (int q = lb; q < lb + nb; ++q) {
+    const int sf = f.big_host[q];
+    max_m = std::max(max_m, f.m_host[sf]);
+    max_k = std::max(max_k, f.k_host[sf]);
+    max_rows = std::max(max_rows, f.xr_off_host[sf + 1] - f.xr_off_host[sf]);
   }
-  for (int kb = 0; kb < k; kb += kBigW) {
-    if (m <= 1934)
-      hipLaunchKernelGGL((front_panel_kernel<8, 1934>), dim3(1), dim3(kFB), 0, s, f, sf, kb);
+  if (max_rows > 0)
+    hipLaunchKernelGGL(front_extend_kernel, dim3((unsigned)max_rows, nb), dim3(kFB), 0, s, f,
+                       lb);
+  for (int kb = 0; kb < max_k; kb += kBigW) {
+    if (max_m <= 1934)
+      hipLaunchKernelGGL((front_panel_kernel<8, 1934>), dim3(nb), dim3(kFB), 0, s, f, lb, kb);
     else
-      hipLaunchKernelGGL((front_panel_kernel<4, kFrontMaxM>), dim3(1), dim3(kFB), 0, s, f, sf,
+      hipLaunchKernelGGL((front_panel_kernel<4, kFrontMaxM>), dim3(nb), dim3(kFB), 0, s, f, lb,
                          kb);
-    const int cl = std::min(kb + kBigW, k);
-    const int rest = m - cl;
+    // the largest trailing block at this panel position
+    int rest = 0;
+    for (int q = lb; q < lb + nb; ++q) {
+      const int sf = f.big_host[q];
+      if (kb < f.k_host[sf])
+        rest = std::max(rest, f.m_host[sf] - std::min(kb + kBigW, f.k_host[sf]));
+    }
     if (rest <= 0) continue;
-    hipLaunchKernelGGL(front_swap_trsm_kernel, dim3((rest + kFB - 1) / kFB), dim3(kFB), 0, s,
-                       f, sf, kb);
+    hipLaunchKernelGGL(front_swap_trsm_kernel, dim3((rest + kFB - 1) / kFB, nb), dim3(kFB), 0,
+                       s, f, lb, kb);
     const unsigned tiles = (unsigned)((rest + kTile - 1) / kTile);
-    hipLaunchKernelGGL(front_update_kernel, dim3(tiles, tiles), dim3(kFB), 0, s, f, sf, kb);
+    hipLaunchKernelGGL(front_update_kernel, dim3(tiles, tiles, nb), dim3(kFB), 0, s, f, lb,
+                       kb);
   }
-  if (k > kBigW)
-    hipLaunchKernelGGL(front_left_swaps_kernel, dim3((k + kFB - 1) / kFB), dim3(kFB), 0, s, f,
-                       sf);
+  if (max_k > kBigW)
+    hipLaunchKernelGGL(front_left_swaps_kernel, dim3((max_k + kFB - 1) / kFB, nb), dim3(kFB),
+                       0, s, f, lb);
   return hipGetLastError();
 }
 
@@ -625,21 +853,43 @@ hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t 
       }
       if (e != hipSuccess) return e;
     }
+    // the level's larger fronts, at most 65535 per launch (grid y / z)
     if (any_big)
-      for (int q = lv0; q < lv0 + nfr; ++q)
-        if (f.m_host[f.lv_host[q]] > f.big_m)
-          if ((e = launch_big_front(f, f.lv_host[q], s)) != hipSuccess) return e;
+      for (int lb = f.big_ptr_host[L]; lb < f.big_ptr_host[L + 1]; lb += 65535)
+        if ((e = launch_big_fronts(f, lb, std::min(65535, f.big_ptr_host[L + 1] - lb), s)) !=
+            hipSuccess)
+          return e;
   }
+  // the solves' LDS holds a front's rows: the level's largest front picks
+  // the instance
   for (int L = 0; L < f.n_levels; ++L) {
     const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
-    hipLaunchKernelGGL(front_fwd_kernel, dim3(nfr), dim3(kFB), 0, s, f, (const double *)a.bp,
-                       lv0);
+    const int mm = f.lv_maxm_host[L];
+    const double *bp = a.bp;
+    if (mm <= 64)
+      hipLaunchKernelGGL(front_fwd_kernel<64>, dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
+    else if (mm <= 512)
+      hipLaunchKernelGGL(front_fwd_kernel<512>, dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
+    else
+      hipLaunchKernelGGL(front_fwd_kernel<kFrontMaxM>, dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
   }
   for (int L = f.n_levels - 1; L >= 0; --L) {
     const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
-    hipLaunchKernelGGL(front_bwd_kernel, dim3(nfr), dim3(kFB), 0, s, f, lv0);
+    const int mm = f.lv_maxm_host[L];
+    if (mm <= 64)
+      hipLaunchKernelGGL(front_bwd_kernel<64>, dim3(nfr), dim3(kFB), 0, s, f, lv0);
+    else if (mm <= 512)
+      hipLaunchKernelGGL(front_bwd_kernel<512>, dim3(nfr), dim3(kFB), 0, s, f, lv0);
+    else
+      hipLaunchKernelGGL(front_bwd_kernel<kFrontMaxM>, dim3(nfr), dim3(kFB), 0, s, f, lv0);
   }
   hipLaunchKernelGGL(front_finish_kernel, dim3(f.n_comp), dim3(kFinishBlock), 0, s, f, a);
+  if (f.n_fin_chunks > 0) {
+    hipLaunchKernelGGL(finish_part_kernel, dim3(f.n_fin_chunks), dim3(kFB), 0, s, f, a);
+    hipLaunchKernelGGL(finish_decide_kernel, dim3(f.n_fin_comps), dim3(kFB), 0, s, f, a);
+    hipLaunchKernelGGL(finish_update_kernel, dim3(f.n_fin_chunks), dim3(kFB), 0, s, f, a);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(f.n_fin_comps), dim3(64), 0, s, f, a);
+  }
   return hipGetLastError();
 }
 

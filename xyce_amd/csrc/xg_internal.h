@@ -294,7 +294,29 @@ struct FrontArgs {
   // rows
   const int32_t *lv_host, *m_host, *k_host, *ch_ptr_host, *ch_host;
   int32_t big_m = kFrontBigM;
+  // those fronts grouped by level (device), with the host's level offsets
+  const int32_t *big = nullptr, *big_host = nullptr, *big_ptr_host = nullptr;
+  // components of more than kFinishSplit unknowns finish in chunks of
+  // kFinishChunk rows: per component split[c] (1 = chunked), the chunks'
+  // component and first row, the chunked components and their chunk
+  // ranges, per-chunk partials (4 doubles) and the per-component verdict
+  // children's contribution rows by target: front s's rows that receive
+  // contributions are xr_row[xr_off[s], xr_off[s + 1]); target q takes
+  // child rows (ext_ch, ext_i)[ext_ptr[q], ext_ptr[q + 1]) in child order
+  const int64_t *xr_off = nullptr, *ext_ptr = nullptr, *xr_off_host = nullptr;
+  const int32_t *xr_row = nullptr, *ext_ch = nullptr, *ext_i = nullptr;
+  // rows of more than kLongRow Jacobian entries (supply rails) per front
+  // component, checked by a whole workgroup: long_rows[long_ptr[c], ..)
+  const int32_t *long_ptr = nullptr, *long_rows = nullptr;
+  int32_t n_fin_chunks = 0, n_fin_comps = 0;
+  const int32_t *fin_split = nullptr, *fin_chunk_comp = nullptr, *fin_chunk_beg = nullptr;
+  const int32_t *fin_comps = nullptr, *fin_cptr = nullptr;
+  double *fin_part = nullptr;
+  int32_t *fin_good = nullptr;
 };
+constexpr int32_t kFinishSplit = 8192;
+constexpr int32_t kLongRow = 256;
+constexpr int32_t kFinishChunk = 2048;
 int front_form(int32_t max_m);
 hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t s);
 

@@ -342,6 +342,13 @@ bool order_cost(const Graph &sym, const std::vector<int32_t> &order, int64_t &fi
 // factors in fewer dependent steps than the nested dissection).
 constexpr int32_t kMinDegreeMaxN = 100000;
 
+// The fronts' cost of one more dependent elimination level, in
+// floating-point operations: a level is about three launches (factor,
+// forward, backward; ~20 us), while the fronts factor at ~50 GFLOP/s.  An
+// order with a long elimination chain (the minimum-degree order of a
+// ladder is the chain itself) loses to a shallower one with more work.
+constexpr int64_t kFrontLevelFlops = 1000000;
+
 // One component's static factorisation, with task indices local to the
 // component (lu_analyse appends the components in order).  The component's
 // entries of prow / qcol / qpos / lu_pos (disjoint from every other
@@ -486,6 +493,16 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
   const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
   const int32_t *nodes = plan.comp_nodes.data() + beg;
   auto loc = [&](int32_t g) { return plan.comp_local[g]; };
+  // dense fronts by choice: always (XG_LU_FRONTS), or (XG_LU_AUTO) when the
+  // task lists cannot fit in LDS -- their global-memory form is slower than
+  // the fronts.  With `fill` strictly-lower entries the filled pattern has
+  // n + 2 fill values, the task lists at least n + fill tasks and 2 fill
+  // solve terms, a lower bound of their LDS bytes.
+  auto fronts_for = [&](int64_t fill) {
+    if (plan.lu_mode == XG_LU_FRONTS) return true;
+    if (plan.lu_mode != XG_LU_AUTO) return false;
+    return sparse_lu_lds_bytes(n, n + 2 * fill, n + fill, 2 * fill, 0) > sparse_lu_lds_budget();
+  };
   // local structure of the component's rows
   Graph rows(n);
   for (int32_t r = 0; r < n; ++r) {
@@ -563,14 +580,15 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
     int64_t f0 = 0, w0 = 0;
     int32_t d0 = 0;
     const bool fits = order_cost(sym, order, f0, d0, &w0, kFrontMaxFill);
-    if (!fits || f0 > kLuMaxFill || (w0 - f0) / 2 > kLuMaxOps || plan.lu_fronts) {
+    if (!fits || f0 > kLuMaxFill || (w0 - f0) / 2 > kLuMaxOps || fronts_for(f0)) {
       std::vector<int32_t> md;
       int64_t f1 = 0, w1 = 0;
       int32_t d1 = 0;
       // (its explicit elimination graph grows too large on big meshes)
       if (n <= kMinDegreeMaxN &&
           min_degree_order(sym, hubs, md, 2 * kFrontMaxFill, kFrontMdWork) &&
-          order_cost(sym, md, f1, d1, &w1, kFrontMaxFill) && (!fits || w1 < w0))
+          order_cost(sym, md, f1, d1, &w1, kFrontMaxFill) &&
+          (!fits || w1 + kFrontLevelFlops * d1 < w0 + kFrontLevelFlops * d0))
         order.swap(md);
       if (std::getenv("XG_LU_DEBUG"))
         std::fprintf(stderr, "lu: n %d nd fill %lld flops %.3g depth %d | md fill %lld flops %.3g depth %d\n",
@@ -618,7 +636,7 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
   {
     int64_t work = 0;  // multiply-adds: sum over pivots of (column count)^2
     for (int32_t k = 0; k < n; ++k) work += (int64_t)ucol[k].size() * (int64_t)ucol[k].size();
-    if (fill > kLuMaxFill || work > kLuMaxOps || plan.lu_fronts) {
+    if (fill > kLuMaxFill || work > kLuMaxOps || fronts_for(fill)) {
       // too large for one workgroup's task lists: dense fronts
       std::vector<int32_t> rpos(n);
       for (int32_t i = 0; i < n; ++i) rpos[row_of_col[order[i]]] = i;
@@ -692,123 +710,154 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
   }
   out.fn = fn;
   out.ok = true;
-  // ---- factor tasks: values in the order diag, column of L, row of U
+  // ---- factor tasks: values in the order diag, column of L, row of U.
+  // Tasks are kept flat: task k's (a, b) operand pairs are
+  // ops[2 * beg[k], 2 * beg[k + 1]).
   std::vector<int32_t> lev(fn, -1);
-  struct Task {
-    int32_t dst, div, lev;
-    std::vector<int32_t> ops;
-  };
-  std::vector<Task> tasks;
-  // terms L(i, m) U(m, j) over m in lrow[i] and lrow[j] (m < min(i, j))
-  auto intersect = [&](int32_t i, int32_t j, std::vector<int32_t> &ops, int &lv) {
-    const auto &a = lrow[i], &b = lrow[j];
-    size_t x = 0, y = 0;
-    while (x < a.size() && y < b.size()) {
-      if (a[x] < b[y]) {
-        ++x;
-      } else if (b[y] < a[x]) {
-        ++y;
-      } else {
-        const int32_t pa = (int32_t)(rp[i] + (int64_t)x), pb = uof[lo[j] + y];
-        ops.push_back(pa);
-        ops.push_back(pb);
-        lv = std::max(lv, std::max(lev[pa], lev[pb]) + 1);
-        ++x;
-        ++y;
-      }
+  struct Tasks {
+    std::vector<int32_t> dst, div, lev, ops;
+    std::vector<int64_t> beg{0};
+    void open(int32_t d, int32_t v, int32_t l) {
+      dst.push_back(d);
+      div.push_back(v);
+      lev.push_back(l);
     }
+    void close() { beg.push_back((int64_t)ops.size() / 2); }
+    void drop() {  // the open task (no operands)
+      dst.pop_back();
+      div.pop_back();
+      lev.pop_back();
+    }
+    size_t size() const { return dst.size(); }
+  };
+  Tasks tasks, bwd;
+  // terms L(i, m) U(m, j) over m in lrow[i] and lrow[j] (m < min(i, j)),
+  // ascending m.  The pivot p's row is scattered into at_p (at_p[m] = the
+  // index of m in lrow[p]), so each task walks only its other row.
+  std::vector<int32_t> at_p(n, -1);
+  int32_t lv = 0;
+  auto term = [&](int32_t pa, int32_t pb) {
+    tasks.ops.push_back(pa);
+    tasks.ops.push_back(pb);
+    lv = std::max(lv, std::max(lev[pa], lev[pb]) + 1);
   };
   for (int32_t p = 0; p < n; ++p) {
-    const int32_t dp = (int32_t)(rp[p] + (int64_t)lrow[p].size());
-    {
-      Task t{dp, -2, 0, {}};
-      intersect(p, p, t.ops, t.lev);
-      lev[t.dst] = t.lev;
-      tasks.push_back(std::move(t));
-    }
+    const auto &rowp = lrow[p];
+    for (size_t x = 0; x < rowp.size(); ++x) at_p[rowp[x]] = (int32_t)x;
+    const int32_t dp = (int32_t)(rp[p] + (int64_t)rowp.size());
+    lv = 0;
+    for (size_t x = 0; x < rowp.size(); ++x)
+      term((int32_t)(rp[p] + (int64_t)x), uof[lo[p] + x]);
+    tasks.open(dp, -2, lv);
+    tasks.close();
+    lev[dp] = lv;
     for (int32_t i : ucol[p]) {  // L(i, p), i > p
-      Task t{pos(i, p), dp, lev[dp] + 1, {}};
-      intersect(i, p, t.ops, t.lev);
-      lev[t.dst] = t.lev;
-      tasks.push_back(std::move(t));
+      const int32_t d = pos(i, p);
+      lv = lev[dp] + 1;
+      const auto &a = lrow[i];
+      for (size_t x = 0; x < a.size() && a[x] < p; ++x) {
+        const int32_t y = at_p[a[x]];
+        if (y >= 0) term((int32_t)(rp[i] + (int64_t)x), uof[lo[p] + y]);
+      }
+      tasks.open(d, dp, lv);
+      tasks.close();
+      lev[d] = lv;
     }
     for (size_t z = 0; z < ucol[p].size(); ++z) {  // U(p, j), j > p
       const int32_t j = ucol[p][z];
-      Task t{dp + 1 + (int32_t)z, -1, 0, {}};
-      intersect(p, j, t.ops, t.lev);
-      if (t.ops.empty()) continue;  // the scattered value is final
-      lev[t.dst] = t.lev;
-      tasks.push_back(std::move(t));
+      const int64_t b0 = (int64_t)tasks.ops.size();
+      lv = 0;
+      const auto &b = lrow[j];
+      for (size_t y = 0; y < b.size() && b[y] < p; ++y) {
+        const int32_t x = at_p[b[y]];
+        if (x >= 0) term((int32_t)(rp[p] + (int64_t)x), uof[lo[j] + y]);
+      }
+      if ((int64_t)tasks.ops.size() == b0) continue;  // the scattered value is final
+      const int32_t d = dp + 1 + (int32_t)z;
+      tasks.open(d, -1, lv);
+      tasks.close();
+      lev[d] = lv;
     }
+    for (int32_t m : rowp) at_p[m] = -1;
   }
   // ---- forward solve (L y = b, unit diagonal) and backward (U x = y).
   // A forward row needs its L entries and the earlier y it multiplies, not
   // the whole factorisation: it joins the factor phase at the level after
   // both, so the forward solve adds no levels of its own.
-  std::vector<Task> fwd, bwd;
   {
     std::vector<int32_t> lf(n, -1), lb(n, -1);
     for (int32_t i = 0; i < n; ++i) {
       if (lrow[i].empty()) continue;
-      Task t{i, kLuFwdTask, 0, {}};
+      int32_t l = 0;
       for (size_t x = 0; x < lrow[i].size(); ++x) {
         const int32_t k = lrow[i][x];
         const int32_t pl = (int32_t)(rp[i] + (int64_t)x);
-        t.ops.push_back(pl);
-        t.ops.push_back(k);
-        t.lev = std::max(t.lev, std::max(lev[pl], lf[k]) + 1);
+        tasks.ops.push_back(pl);
+        tasks.ops.push_back(k);
+        l = std::max(l, std::max(lev[pl], lf[k]) + 1);
       }
-      lf[i] = t.lev;
-      tasks.push_back(std::move(t));
+      tasks.open(i, kLuFwdTask, l);
+      tasks.close();
+      lf[i] = l;
     }
     for (int32_t i = n - 1; i >= 0; --i) {
       const int32_t di = (int32_t)(rp[i] + (int64_t)lrow[i].size());
-      Task t{i, di, 0, {}};
+      int32_t l = 0;
       for (size_t z = 0; z < ucol[i].size(); ++z) {
         const int32_t j = ucol[i][z];
-        t.ops.push_back(di + 1 + (int32_t)z);
-        t.ops.push_back(j);
-        t.lev = std::max(t.lev, lb[j] + 1);
+        bwd.ops.push_back(di + 1 + (int32_t)z);
+        bwd.ops.push_back(j);
+        l = std::max(l, lb[j] + 1);
       }
-      lb[i] = t.lev;
-      bwd.push_back(std::move(t));
+      bwd.open(i, di, l);
+      bwd.close();
+      lb[i] = l;
     }
   }
   // ---- the three phases, tasks bucketed by level (short tasks first, then
-  // mid, then long within a level)
-  auto emit = [&](std::vector<Task> &ts, int ph) {
+  // mid, then long within a level; stable within each class)
+  auto emit = [&](Tasks &ts, int ph) {
+    const int32_t nt = (int32_t)ts.size();
     int nlev = 0;
-    for (auto &t : ts) nlev = std::max(nlev, t.lev + 1);
-    std::vector<std::vector<int32_t>> by(nlev);
-    for (size_t k = 0; k < ts.size(); ++k) by[ts[k].lev].push_back((int32_t)k);
-    for (auto &lv : by) {
-      auto cls = [&](int32_t k) {
-        const size_t m = ts[k].ops.size() / 2;
-        return m <= (size_t)kLuShortOps ? 0 : m <= (size_t)kLuLongOps ? 1 : 2;
-      };
-      std::stable_sort(lv.begin(), lv.end(),
-                       [&](int32_t x, int32_t y) { return cls(x) < cls(y); });
-      int32_t first_mid = -1, first_long = -1;
-      for (int32_t k : lv) {
-        if (first_mid < 0 && cls(k) >= 1) first_mid = (int32_t)out.t_dst.size();
-        if (first_long < 0 && cls(k) == 2) first_long = (int32_t)out.t_dst.size();
-        const Task &t = ts[k];
-        out.t_dst.push_back(t.dst);
-        out.t_div.push_back(t.div);
-        out.ops.insert(out.ops.end(), t.ops.begin(), t.ops.end());
+    for (int32_t l : ts.lev) nlev = std::max(nlev, l + 1);
+    auto cls = [&](int32_t k) {
+      const int64_t m = ts.beg[k + 1] - ts.beg[k];
+      return m <= kLuShortOps ? 0 : m <= kLuLongOps ? 1 : 2;
+    };
+    // counting sort by (level, class), stable in task order
+    std::vector<int64_t> cnt((size_t)3 * nlev + 1, 0);
+    for (int32_t k = 0; k < nt; ++k) cnt[(size_t)3 * ts.lev[k] + cls(k) + 1]++;
+    for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+    std::vector<int32_t> sorted(nt);
+    {
+      std::vector<int64_t> at(cnt.begin(), cnt.end() - 1);
+      for (int32_t k = 0; k < nt; ++k) sorted[at[(size_t)3 * ts.lev[k] + cls(k)]++] = k;
+    }
+    out.t_dst.reserve(out.t_dst.size() + nt);
+    out.t_div.reserve(out.t_div.size() + nt);
+    out.t_end.reserve(out.t_end.size() + nt);
+    out.ops.reserve(out.ops.size() + ts.ops.size());
+    const int32_t base = (int32_t)out.t_dst.size();
+    for (int l = 0; l < nlev; ++l) {
+      for (int64_t q = cnt[(size_t)3 * l]; q < cnt[(size_t)3 * l + 3]; ++q) {
+        const int32_t k = sorted[q];
+        out.t_dst.push_back(ts.dst[k]);
+        out.t_div.push_back(ts.div[k]);
+        out.ops.insert(out.ops.end(), ts.ops.begin() + 2 * ts.beg[k],
+                       ts.ops.begin() + 2 * ts.beg[k + 1]);
         out.t_end.push_back((int32_t)(out.ops.size() / 2));
-        if (ph == 0) out.factor_ops += (int64_t)t.ops.size() / 2;
+        if (ph == 0) out.factor_ops += ts.beg[k + 1] - ts.beg[k];
       }
-      const int32_t end = (int32_t)out.t_dst.size();
-      out.lv_long.push_back(first_long < 0 ? end : first_long);
-      out.lv_mid.push_back(first_mid < 0 ? out.lv_long.back() : first_mid);
-      out.lv_end.push_back(end);
+      out.lv_mid.push_back(base + (int32_t)cnt[(size_t)3 * l + 1]);
+      out.lv_long.push_back(base + (int32_t)cnt[(size_t)3 * l + 2]);
+      out.lv_end.push_back(base + (int32_t)cnt[(size_t)3 * l + 3]);
     }
     out.nlev[ph] = nlev;
-    std::vector<Task>().swap(ts);
+    ts = Tasks();
   };
+  Tasks none;
   emit(tasks, 0);
-  emit(fwd, 1);
+  emit(none, 1);
   emit(bwd, 2);
   out.in_lds = sparse_lu_lds_bytes(n, fn, (int64_t)out.t_dst.size(),
                                    (int64_t)out.ops.size() / 2,
@@ -916,7 +965,9 @@ int lu_analyse(Plan &plan, std::string &err) {
       const int32_t beg = plan.comp_start[c];
       L.front_comp[c] = 1;
       L.n_front_comp++;
-      L.in_lds[c] = 0;
+      // no task lists: the LDS pass of the task kernel (launched always)
+      // hands the component to the front kernels
+      L.in_lds[c] = 1;
       for (int32_t s = 0; s < ns; ++s) {
         L.fr_comp.push_back(c);
         L.fr_m.push_back(F.m[s]);

@@ -12,6 +12,9 @@
 #include "xg_setup.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <cctype>
 #include <cmath>
@@ -528,6 +531,7 @@ static void finish_program(const std::vector<Contrib> &cs, int32_t n_out,
 }
 
 int build_plan(Circuit &c, Plan &plan, std::string &err) {
+  const auto t_start = std::chrono::steady_clock::now();
   const int32_t n = c.n_nodes;
   const int32_t G = n;  // ground slot / row / column
   auto L = [G](int32_t g) { return g < 0 ? G : g; };
@@ -951,8 +955,16 @@ int build_plan(Circuit &c, Plan &plan, std::string &err) {
     pbase += (int32_t)prog->cdst.size();
   }
   plan.n_partials = pbase;
-  plan.lu_fronts = c.opt.lu_fronts;
-  return lu_analyse(plan, err);
+  plan.lu_mode = c.opt.lu_mode;
+  const auto t_lu = std::chrono::steady_clock::now();
+  const int rc = lu_analyse(plan, err);
+  if (std::getenv("XG_LU_DEBUG")) {
+    const auto t_end = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "setup: plan %.3f s, LU analysis %.3f s\n",
+                 std::chrono::duration<double>(t_lu - t_start).count(),
+                 std::chrono::duration<double>(t_end - t_lu).count());
+  }
+  return rc;
 }
 
 }  // namespace xg

@@ -26,7 +26,7 @@ struct Options {
   int limiter = 1;
   double temp = 300.15;
   double tnom = 300.15;
-  int lu_fronts = 0;  // 1: every component through dense fronts
+  int lu_mode = 0;  // XG_LU_AUTO, XG_LU_FRONTS or XG_LU_TASKS
 };
 
 // Processed MOSFET1 model (after processParams).
@@ -174,7 +174,7 @@ struct Plan {
   ReduceProgram red_mat[2];  // dFdx, dQdx
   int32_t n_partials = 0;    // chunk partial sums over all six programs
   int32_t limiter = 1;
-  int32_t lu_fronts = 0;  // Options::lu_fronts
+  int32_t lu_mode = 0;  // Options::lu_mode
   double gmin = 1e-12;
   // connected components of the circuit graph without ground (independent
   // sub-circuits), for the per-component dense solve

@@ -36,9 +36,12 @@ def _iptr(t):
     return _ptr(t, "torch.int32")
 
 
+LU_MODES = {"auto": 0, "fronts": 1, "tasks": 2}
+
+
 class DeviceLoad:
     def __init__(self, nl, gmin=1e-12, limiter=True, temp=300.15, tnom=300.15,
-                 pattern=None, plan_only=False, lu_fronts=False):
+                 pattern=None, plan_only=False, lu_fronts=False, lu_mode="auto"):
         nl.layout()
         L = lib()
         self.nl = nl
@@ -48,8 +51,12 @@ class DeviceLoad:
         try:
             check(L.xg_circuit_set_options(c, gmin, int(limiter), temp, tnom),
                   "xg_circuit_set_options")
-            if lu_fronts:
-                check(L.xg_circuit_set_lu_mode(c, 1), "xg_circuit_set_lu_mode")
+            # linear-solve form: "auto" (task lists where they fit in LDS,
+            # dense fronts elsewhere), "fronts" (every component) or
+            # "tasks" (task lists wherever their budgets allow)
+            mode = LU_MODES["fronts" if lu_fronts else lu_mode]
+            if mode:
+                check(L.xg_circuit_set_lu_mode(c, mode), "xg_circuit_set_lu_mode")
             mid = {}
             for name, m in nl.models.items():
                 if m.kind == "diode":

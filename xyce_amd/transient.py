@@ -66,12 +66,12 @@ _FAIL_WORDS = {1: "singular Jacobian", 2: "non-finite device values", 3: "no con
 
 class Transient:
     def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda",
-                 solver="lu", driver=None, graphs=True, lu_fronts=False):
+                 solver="lu", driver=None, graphs=True, lu_fronts=False, lu_mode="auto"):
         # the device load's host analysis runs unlocked; its upload, the
         # buffers and the driver's graph captures under DEVICE_LOCK.
-        # lu_fronts: every component through dense fronts (else only those
-        # beyond the task lists' budgets)
-        dev = DeviceLoad(nl, gmin=gmin, limiter=limiter, lu_fronts=lu_fronts)
+        # lu_fronts: every component through dense fronts; lu_mode: the
+        # linear-solve form otherwise (loader.LU_MODES)
+        dev = DeviceLoad(nl, gmin=gmin, limiter=limiter, lu_fronts=lu_fronts, lu_mode=lu_mode)
         with DEVICE_LOCK:
             self._init(nl, dev, bp_tol, device, solver, driver, graphs, limiter)
 
