@@ -40,7 +40,13 @@ HBM_PEAK_GBS = 8000.0
 # and WRITE_SIZE counter passes scaled by a plain-copy calibration
 # (profiles/bsim3_device_load_random_logic_5000g_hbm_traffic.md); valid for
 # the default --gates only.
-B3_EVAL_TRAFFIC_BYTES = 29.66e6
+B3_EVAL_TRAFFIC_BYTES = 29.70e6
+# FP64 operations per BSIM3 eval launch (the same kernel and workload), from
+# the SQ_INSTS_VALU_{ADD,MUL,TRANS,FMA}_F64 counters (per-wave instruction
+# counts; 64 lanes each, an FMA two operations): 64 (333575 + 345814 +
+# 76987 + 2 x 556793)
+B3_EVAL_FP64_FLOPS = 64 * (333575 + 345814 + 76987 + 2 * 556793)
+FP64_PEAK_TFLOPS = 78.6
 # Eval waves up to which the library's automatic choice is the LDS-staged
 # split kernel (one eval wave per 64 instances; two eval waves per workgroup
 # above 256), else the HBM-record kernel.
@@ -123,6 +129,8 @@ def parse():
                     help="first requests of the same stream served one circuit per "
                          "launch (default: all of them)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-host-loads", action="store_true",
+                    help="skip the host-pointer (PCIe-staged) load timing")
     ap.add_argument("--no-grid", action="store_true",
                     help="skip the power-grid (100k inverter) load timing")
     ap.add_argument("--no-big-tran", action="store_true",
@@ -298,8 +306,8 @@ def power_grid_leg(torch, args):
         "device_eval_ms": eval_ms,
         "eval_roofline_frac": B3_BYTES_PER_EVAL * n_b3 / (eval_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "setup_s": setup,
-        "linear_solve": "not timed: one 550k-unknown component exceeds the static "
-                        "sparse-LU fill budget",
+        "linear_solve": "dense-front (multifrontal) LU of the one 550k-unknown "
+                        "component: timed per Newton iteration in grid_transient",
     }
     dev.close()
     return res
@@ -381,6 +389,46 @@ def large_transient_leg(torch, args, which):
         "breakpoints": {"in_span": len(nl.breakpoints(tstop)),
                         "in_configured_tran": len(nl.breakpoints(full))},
     }
+
+
+def host_loads_leg(nl, x, sta, sto, args):
+    """The drop-in host-pointer mode (xg_update_state_host,
+    xg_load_vectors_host, xg_load_matrices_host) on the same circuit and
+    inputs: the caller's vectors and CSR value arrays stay in host memory
+    (numpy), as the reference's N_LAS objects do, and every call stages its
+    inputs and results over PCIe.  One Newton iteration's loads per step;
+    the step time includes the transfers."""
+    import ctypes
+    from xyce_amd._lib import check, lib
+    from xyce_amd.loader import DeviceLoad
+    L = lib()
+    progress("host-pointer loads")
+    dev = DeviceLoad(nl)
+    dev.set_solver_state(transient=1, newtonIter=1)
+    dev.set_output_mode(True)
+    n1 = nl.n_nodes + 1
+    hs = [x.copy(), sta.copy(), sto.copy(), np.zeros(nl.n_state), sto.copy()]
+    ho = [np.zeros(n1) for _ in range(4)] + [np.zeros(dev.nnz) for _ in range(2)]
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+
+    def step():
+        check(L.xg_update_state_host(dev.h, *[p(a) for a in hs]), "update host")
+        check(L.xg_load_vectors_host(dev.h, *[p(a) for a in ho[:4]]), "vectors host")
+        check(L.xg_load_matrices_host(dev.h, *[p(a) for a in ho[4:]]), "matrices host")
+    for _ in range(max(args.warmup, 3)):
+        step()
+    k = max(10, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        step()
+    ms = (time.perf_counter() - t0) * 1e3 / k
+    moved = 8 * (n1 + 2 * nl.n_state + 3 * nl.n_store + 4 * n1 + 2 * dev.nnz)
+    dev.close()
+    return {"ms_per_newton_iteration": ms,
+            "evals_per_s": len(nl.bsim3) / (ms * 1e-3),
+            "host_bytes_per_iteration": moved,
+            "note": "host (numpy) arrays in and out of every call; PCIe-inclusive, "
+                    "never the headline value"}
 
 
 def serving_leg(args):
@@ -588,7 +636,15 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": B3_EVAL_TRAFFIC_BYTES if args.gates == 5000 else None,
                      "kernel": b3_kernel_name(n_b3, tune["variant"]), "kernel_ms": eval_ms,
-                     "bytes_per_eval": B3_BYTES_PER_EVAL},
+                     "bytes_per_eval": B3_BYTES_PER_EVAL,
+                     # the same launch's FP64 VALU rate: counter operations
+                     # over the measured duration, against the vector peak
+                     "fp64": None if args.gates != 5000 else {
+                         "achieved": B3_EVAL_FP64_FLOPS / (eval_ms * 1e-3) / 1e12,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": B3_EVAL_FP64_FLOPS / (eval_ms * 1e-3) / 1e12
+                         / FP64_PEAK_TFLOPS,
+                         "flops_per_launch": B3_EVAL_FP64_FLOPS}},
         "autotune": tune,
         "wall_s_timed_region": wall,
         "launch": launch,
@@ -599,6 +655,8 @@ def main():
         "reduction_ms": reduce_ms,
     }
     progress("device load: %.4f ms per step" % ms_step)
+    if not args.no_host_loads:
+        res["host_loads"] = host_loads_leg(nl, x, sta, sto, args)
     if not args.no_mc:
         res["batched_transient"] = batched_transient_leg(torch, args)
     if not args.no_grid:

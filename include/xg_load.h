@@ -118,6 +118,25 @@ int xg_add_bsim3_model(xg_circuit *c, int32_t dtype, int32_t n,
                        const char *const *keys, const double *vals,
                        int32_t *model_id);
 
+/* count MOSFET1 / BSIM3 instances in one call, as count single calls in
+   array order would: instance i takes model model_ids[i], the n_keys
+   parameters keys with the values vals[i * n_keys ...], lids[6 i ...],
+   sta_base[i] and sto_base[i], and its id goes to inst_ids[i].  An error stops at the failing instance
+   (those before it are added), as the single calls would.  For setups of
+   many instances (served batches), where one call per instance costs more
+   than the setup itself.  No counterpart in the reference, which
+   constructs instances one by one. */
+int xg_add_mos1_instances(xg_circuit *c, const int32_t *model_ids, int32_t count,
+                          int32_t n_keys,
+                          const char *const *keys, const double *vals, const int32_t *lids,
+                          const int32_t *sta_base, const int32_t *sto_base,
+                          int32_t *inst_ids);
+int xg_add_bsim3_instances(xg_circuit *c, const int32_t *model_ids, int32_t count,
+                           int32_t n_keys,
+                           const char *const *keys, const double *vals, const int32_t *lids,
+                           const int32_t *sta_base, const int32_t *sto_base,
+                           int32_t *inst_ids);
+
 /* BSIM3 instance (N_DEV_MOSFET_B3.C:1845-2785, updateTemperature and the
    size-dependent parameters N_DEV_MOSFET_B3.C:3611-4518).  lids =
    {d, g, s, b, d', s'}; 12 state slots at sta_base and 5 store slots at
@@ -377,6 +396,19 @@ int xg_lu_solve(xg_handle *h, const double *J, const double *rhs, double *x,
    pivoting by the last xg_lu_solve (synchronises; -1 without a device),
    [7] factor and solve tasks.  No counterpart in the reference. */
 int xg_lu_info(xg_handle *h, int64_t info[8]);
+
+/* Analysed components by topology, shared by every circuit of the process:
+   a component whose structure (rows with component-local columns in CSR
+   order, source pairings, linear-solve form) was analysed before takes the
+   stored ordering, fill and task lists or fronts, mapped onto its own
+   nodes and entries -- the same plan the analysis gives.  Serves batches
+   that repeat sub-circuits (parameter sweeps, served requests of one
+   topology).  xg_lu_cache_stats: out = {hits, misses, bytes held};
+   xg_lu_cache_set_limit: byte limit (default 256 MiB, 0 turns it off),
+   empties the cache and zeroes the counts.  Thread-safe.  No counterpart
+   in the reference, which analyses its one circuit once. */
+int xg_lu_cache_stats(int64_t out[3]);
+int xg_lu_cache_set_limit(int64_t bytes);
 
 /* Fronts of more than big_m rows (default 256) are factored panel by panel
    across many workgroups (one workgroup per panel, then every column's row

@@ -206,3 +206,59 @@ def test_noalloc_load_refuses_circuits_beyond_its_scratch(L):
     small = DeviceLoad(circuits.ring_oscillator_bsim3(11, seed=3), plan_only=True)
     assert L.xg_load_all_noalloc(small.h, *([None] * 11)) == XG_E_STATE
     small.close()
+
+
+def test_bulk_instance_adds_match_single_adds(L):
+    """xg_add_mos1_instances / xg_add_bsim3_instances (what DeviceLoad uses)
+    give the plan of the same instances added one call each: pattern,
+    offsets, per-instance fields and the LU plan."""
+    from xyce_amd import circuits
+    from xyce_amd.loader import DeviceLoad
+
+    def one_by_one(single):
+        def bulk(c, models, count, n_keys, keys, vals, lids, sta, sto, ids):
+            dv = ctypes.c_double * max(n_keys, 1)
+            for i in range(count):
+                model = ctypes.c_int32.from_address(models + 4 * i).value
+                v = dv.from_address(vals + 8 * i * n_keys) if n_keys else dv()
+                li = (ctypes.c_int32 * 6).from_address(lids + 24 * i)
+                s = ctypes.c_int32.from_address(sta + 4 * i).value
+                t = ctypes.c_int32.from_address(sto + 4 * i).value
+                out = ctypes.c_int32.from_address(ids + 4 * i)
+                rc = single(c, model, n_keys, keys, v, li, s, t, ctypes.byref(out))
+                if rc:
+                    return rc
+            return 0
+        return bulk
+
+    def plan(nl):
+        d = DeviceLoad(nl, plan_only=True)
+        out = [d.pattern(), d.lu_plan()]
+        if d.n_mos1:
+            out += [d.mos1_offsets(), d.mos1_fields()]
+        if d.n_bsim3:
+            out += [d.bsim3_offsets(), d.bsim3_fields()]
+        d.close()
+        return out
+
+    def flat(x):
+        if isinstance(x, dict):
+            return [np.asarray(x[k]) for k in sorted(x)]
+        if isinstance(x, (list, tuple)):
+            return [np.asarray(a) for a in x]
+        return [np.asarray(x)]
+
+    for mk in (lambda: circuits.mos1_variety(seed=2), lambda: circuits.bsim3_variety(seed=4),
+               lambda: circuits.random_logic_bsim3(n_gates=40, seed=3)):
+        a = plan(mk())
+        saved = (L.xg_add_mos1_instances, L.xg_add_bsim3_instances)
+        try:
+            L.xg_add_mos1_instances = one_by_one(L.xg_add_mos1_instance)
+            L.xg_add_bsim3_instances = one_by_one(L.xg_add_bsim3_instance)
+            b = plan(mk())
+        finally:
+            L.xg_add_mos1_instances, L.xg_add_bsim3_instances = saved
+        assert len(a) == len(b)
+        for x, y in zip(a, b):
+            for p, q in zip(flat(x), flat(y)):
+                assert np.array_equal(p, q)

@@ -265,3 +265,46 @@ def test_random_logic_takes_the_shorter_chain():
     assert info["static_fail"] == 0
     assert info["max_factor_levels"] < 400
     assert info["fnnz"] < 40000
+
+
+def _cache_stats():
+    from xyce_amd._lib import check, lib
+    st = np.zeros(3, np.int64)
+    check(lib().xg_lu_cache_stats(st.ctypes.data), "cache stats")
+    return st
+
+
+def test_plan_cache_reuses_analysed_components():
+    """A component whose structure was analysed before (a repeated ring in
+    the same batch, or the same circuit again) takes the stored analysis:
+    counted as a hit, and the plan -- orderings, positions, task lists,
+    fronts -- is the one a fresh analysis gives."""
+    from xyce_amd._lib import check, lib
+    L = lib()
+    mk = lambda: circuits.merge([circuits.ring_oscillator_bsim3(s, seed=3)
+                                 for s in (11, 31, 11, 31, 51)])
+    logic = lambda: circuits.random_logic_bsim3(n_gates=400, seed=7)
+    try:
+        check(L.xg_lu_cache_set_limit(0), "cache off")
+        ref = DeviceLoad(mk(), plan_only=True, lu_mode="tasks").lu_plan()
+        ref_fr = DeviceLoad(logic(), plan_only=True).front_plan()
+        check(L.xg_lu_cache_set_limit(64 << 20), "cache on")
+        a = DeviceLoad(mk(), plan_only=True, lu_mode="tasks").lu_plan()
+        h1, m1, _ = _cache_stats()
+        assert h1 >= 2 and m1 >= 3, (h1, m1)   # the second 11- and 31-stage rings
+        b = DeviceLoad(mk(), plan_only=True, lu_mode="tasks").lu_plan()
+        h2, m2, held = _cache_stats()
+        assert m2 == m1 and h2 == h1 + h1 + m1 and held > 0
+        for plan in (a, b):
+            for k in ref:
+                assert np.array_equal(ref[k], plan[k]), k
+        f1 = DeviceLoad(logic(), plan_only=True).front_plan()
+        h3, m3, _ = _cache_stats()
+        f2 = DeviceLoad(logic(), plan_only=True).front_plan()
+        h4, m4, _ = _cache_stats()
+        assert m4 == m3 and h4 > h3
+        for plan in (f1, f2):
+            for k in ref_fr:
+                assert np.array_equal(ref_fr[k], plan[k]), k
+    finally:
+        check(L.xg_lu_cache_set_limit(256 << 20), "cache default")

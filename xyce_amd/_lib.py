@@ -42,6 +42,8 @@ SIGNATURES = [
     ("xg_add_mos1_instance", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, c_double_p, c_int32_p, ctypes.c_int32, ctypes.c_int32, c_int32_p]),
     ("xg_add_bsim3_model", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, c_double_p, c_int32_p]),
     ("xg_add_bsim3_instance", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, c_double_p, c_int32_p, ctypes.c_int32, ctypes.c_int32, c_int32_p]),
+    ("xg_add_mos1_instances", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_add_bsim3_instances", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_char_pp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_add_foreign_stamp", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_int32_p, ctypes.c_int32, c_int32_p, c_int32_p]),
     ("xg_circuit_set_pattern", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p]),
     ("xg_finalize", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
@@ -100,6 +102,8 @@ SIGNATURES = [
     ("xg_lu_front_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_lu_front_export", ctypes.c_int, [ctypes.c_void_p] * 4),
     ("xg_circuit_set_lu_mode", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("xg_lu_cache_stats", ctypes.c_int, [ctypes.c_void_p]),
+    ("xg_lu_cache_set_limit", ctypes.c_int, [ctypes.c_int64]),
     ("xg_lu_export", ctypes.c_int, [ctypes.c_void_p] * 4),
     ("xg_set_step_source", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("xg_set_active_mask", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

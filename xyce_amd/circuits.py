@@ -621,25 +621,25 @@ def merge(parts, prefixes=None):
     launches, one LU per part."""
     prefixes = prefixes or ["s%d_" % k for k in range(len(parts))]
     nl = Netlist("batch of %d circuits" % len(parts))
+    ids = nl.node_ids
     for src, pre in zip(parts, prefixes):
-        ren = lambda n, pre=pre: n if n in GROUND_NAMES else pre + n
         # keep each part's node order, so every sub-circuit has the same
-        # unknown ordering (and so the same pivots) as when it runs alone
+        # unknown ordering (and so the same pivots) as when it runs alone;
+        # every device node of a part is one of its nodes or ground
+        ren = {g: g for g in GROUND_NAMES}
         for name in sorted(src.node_ids, key=src.node_ids.get):
-            nl.node(ren(name))
+            ren[name] = pre + name
+            ids.setdefault(pre + name, len(ids))
         for name, m in src.models.items():
             nl.add_model(pre + name, m.kind, m.dtype, **m.params)
-        for m in src.mos:
-            nl.add_mos(pre + m.name, ren(m.d), ren(m.g), ren(m.s), ren(m.b), pre + m.model,
-                       **m.params)
-        for name, a, b, r in src.res:
-            nl.add_res(pre + name, ren(a), ren(b), r)
-        for name, a, b, c in src.caps:
-            nl.add_cap(pre + name, ren(a), ren(b), c)
-        for name, a, b, model, params in src.diodes:
-            nl.add_diode(pre + name, ren(a), ren(b), pre + model, **params)
-        for v in src.vsrc:
-            nl.add_vsrc(pre + v.name, ren(v.p), ren(v.n), v.dc, v.pulse)
+        nl.mos.extend(Mosfet(pre + m.name, ren[m.d], ren[m.g], ren[m.s], ren[m.b],
+                             pre + m.model, dict(m.params)) for m in src.mos)
+        nl.res.extend((pre + name, ren[a], ren[b], float(r)) for name, a, b, r in src.res)
+        nl.caps.extend((pre + name, ren[a], ren[b], float(c)) for name, a, b, c in src.caps)
+        nl.diodes.extend((pre + name, ren[a], ren[b], pre + model, dict(params))
+                         for name, a, b, model, params in src.diodes)
+        nl.vsrc.extend(Vsource(pre + v.name, ren[v.p], ren[v.n], float(v.dc), v.pulse)
+                       for v in src.vsrc)
     return nl
 
 

@@ -252,6 +252,40 @@ int xg_add_bsim3_instance(xg_circuit *c, int32_t model_id, int32_t n,
   return rc ? fail(rc, err) : XG_OK;
 }
 
+// n instances per call, as that many single calls in array order
+typedef int (*AddOne)(xg_circuit *, int32_t, int32_t, const char *const *, const double *,
+                      const int32_t *, int32_t, int32_t, int32_t *);
+static int add_instances(AddOne one, xg_circuit *c, const int32_t *model_ids, int32_t count,
+                         int32_t n_keys, const char *const *keys, const double *vals,
+                         const int32_t *lids, const int32_t *sta_base, const int32_t *sto_base,
+                         int32_t *inst_ids, const char *what) {
+  if (!c || count < 0 || n_keys < 0 ||
+      (count > 0 && (!model_ids || !lids || !sta_base || !sto_base || !inst_ids ||
+                     (n_keys > 0 && (!keys || !vals)))))
+    return fail(XG_E_ARG, std::string(what) + ": bad argument");
+  for (int32_t i = 0; i < count; ++i)
+    if (int rc = one(c, model_ids[i], n_keys, keys, vals + (size_t)i * n_keys,
+                     lids + 6 * (size_t)i, sta_base[i], sto_base[i], inst_ids + i))
+      return rc;
+  return XG_OK;
+}
+
+int xg_add_mos1_instances(xg_circuit *c, const int32_t *model_ids, int32_t count, int32_t n_keys,
+                          const char *const *keys, const double *vals, const int32_t *lids,
+                          const int32_t *sta_base, const int32_t *sto_base,
+                          int32_t *inst_ids) {
+  return add_instances(xg_add_mos1_instance, c, model_ids, count, n_keys, keys, vals, lids,
+                       sta_base, sto_base, inst_ids, "xg_add_mos1_instances");
+}
+
+int xg_add_bsim3_instances(xg_circuit *c, const int32_t *model_ids, int32_t count, int32_t n_keys,
+                           const char *const *keys, const double *vals, const int32_t *lids,
+                           const int32_t *sta_base, const int32_t *sto_base,
+                           int32_t *inst_ids) {
+  return add_instances(xg_add_bsim3_instance, c, model_ids, count, n_keys, keys, vals, lids,
+                       sta_base, sto_base, inst_ids, "xg_add_bsim3_instances");
+}
+
 static int add_lin(xg_circuit *c, int32_t type, int32_t pos, int32_t neg,
                    int32_t br, double val, const double *pulse, int32_t sta,
                    const char *who) {
@@ -529,6 +563,18 @@ static int front_big_lists(xg::Handle &h) {
       hipMemcpy(h.fr_big_dev, h.fr_big.data(), sizeof(int32_t) * h.fr_big.size(),
                 hipMemcpyHostToDevice) != hipSuccess)
     return XG_E_HIP;
+  return XG_OK;
+}
+
+int xg_lu_cache_stats(int64_t *out) {
+  if (!out) return fail(XG_E_ARG, "xg_lu_cache_stats: bad argument");
+  xg::lu_cache_stats(out);
+  return XG_OK;
+}
+
+int xg_lu_cache_set_limit(int64_t bytes) {
+  if (bytes < 0) return fail(XG_E_ARG, "xg_lu_cache_set_limit: negative limit");
+  xg::lu_cache_set_limit(bytes);
   return XG_OK;
 }
 

@@ -31,6 +31,10 @@
 #include <algorithm>
 #include <atomic>
 #include <functional>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
 #include <iterator>
 #include <queue>
 #include <cmath>
@@ -487,8 +491,9 @@ bool build_fronts(int32_t n, const std::vector<int32_t> &parent,
   return true;
 }
 
-void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
-                       const std::vector<std::pair<int32_t, int32_t>> &pairs, CompLu &out) {
+void analyse_component_uncached(const Plan &plan, LuPlan &L, int32_t c,
+                                const std::vector<std::pair<int32_t, int32_t>> &pairs,
+                                CompLu &out) {
   const int32_t G = plan.n_nodes;
   const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
   const int32_t *nodes = plan.comp_nodes.data() + beg;
@@ -869,6 +874,161 @@ void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
   }
 }
 
+// ---- analysed components by topology.  A component's analysis depends
+// only on its local structure (rows in CSR order with component-local
+// columns, source pairings, the solve form), so a component whose
+// structure was analysed before -- the same sub-circuit in an earlier
+// batch or in this one -- takes the stored result, mapped onto its own
+// nodes and CSR entries.  Keyed by a 64-bit hash of that structure, the
+// whole structure compared on a match; least recently used entries leave
+// beyond the byte limit.  No counterpart in the reference, which analyses
+// one circuit once.
+struct CacheEntry {
+  std::vector<int32_t> key;
+  std::vector<int32_t> qcol, prow;  // local unknown / equation per position
+  std::vector<int32_t> lu_pos;      // per entry in key order (-1: none)
+  CompLu res;                       // fronts: ent_k are entry ordinals
+  size_t bytes = 0;
+};
+
+struct PlanCache {
+  std::mutex mu;
+  std::list<std::shared_ptr<CacheEntry>> lru;  // most recent first
+  std::unordered_multimap<uint64_t, std::list<std::shared_ptr<CacheEntry>>::iterator> map;
+  size_t bytes = 0, limit = (size_t)256 << 20;
+  int64_t hits = 0, misses = 0;
+};
+
+PlanCache &plan_cache() {
+  static PlanCache c;
+  return c;
+}
+
+uint64_t fnv64(const std::vector<int32_t> &v) {
+  uint64_t h = 1469598103934665603ull;
+  for (int32_t x : v) {
+    h ^= (uint32_t)x;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+template <class T>
+size_t vbytes(const std::vector<T> &v) { return v.size() * sizeof(T); }
+
+size_t entry_bytes(const CacheEntry &e) {
+  const CompLu &r = e.res;
+  const FrontLocal &f = r.fr;
+  return sizeof(CacheEntry) + vbytes(e.key) + vbytes(e.qcol) + vbytes(e.prow) +
+         vbytes(e.lu_pos) + vbytes(r.t_dst) + vbytes(r.t_div) + vbytes(r.ops) + vbytes(r.t_end) +
+         vbytes(r.lv_end) + vbytes(r.lv_mid) + vbytes(r.lv_long) + vbytes(f.m) + vbytes(f.k) +
+         vbytes(f.height) + vbytes(f.fo) + vbytes(f.ro) + vbytes(f.rows) + vbytes(f.cbmap) +
+         vbytes(f.ch_ptr) + vbytes(f.ch) + vbytes(f.ent_k) + vbytes(f.ent_fpos);
+}
+
+// The structure key of component c, and its CSR entries in key order.
+void component_key(const Plan &plan, int32_t c,
+                   const std::vector<std::pair<int32_t, int32_t>> &pairs,
+                   std::vector<int32_t> &key, std::vector<int32_t> &ents) {
+  const int32_t G = plan.n_nodes;
+  const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
+  const int32_t *nodes = plan.comp_nodes.data() + beg;
+  key.assign({1, n, plan.lu_mode, (int32_t)pairs.size()});
+  ents.clear();
+  for (auto &p : pairs) {
+    key.push_back(plan.comp_local[p.first]);
+    key.push_back(plan.comp_local[p.second]);
+  }
+  for (int32_t r = 0; r < n; ++r) {
+    const int32_t g = nodes[r];
+    const size_t at = key.size();
+    key.push_back(0);
+    for (int32_t k = plan.rowptr[g]; k < plan.rowptr[g + 1]; ++k) {
+      const int32_t col = plan.colidx[k];
+      if (col >= G) continue;
+      key.push_back(plan.comp_local[col]);
+      ents.push_back(k);
+    }
+    key[at] = (int32_t)(key.size() - at - 1);
+  }
+}
+
+void analyse_component(const Plan &plan, LuPlan &L, int32_t c,
+                       const std::vector<std::pair<int32_t, int32_t>> &pairs, CompLu &out) {
+  PlanCache &pc = plan_cache();
+  {
+    std::lock_guard<std::mutex> g(pc.mu);
+    if (pc.limit == 0) {
+      pc.misses++;
+      analyse_component_uncached(plan, L, c, pairs, out);
+      return;
+    }
+  }
+  const int32_t beg = plan.comp_start[c], n = plan.comp_start[c + 1] - beg;
+  const int32_t *nodes = plan.comp_nodes.data() + beg;
+  std::vector<int32_t> key, ents;
+  component_key(plan, c, pairs, key, ents);
+  const uint64_t h = fnv64(key);
+  std::shared_ptr<CacheEntry> hit;
+  {
+    std::lock_guard<std::mutex> g(pc.mu);
+    auto range = pc.map.equal_range(h);
+    for (auto it = range.first; it != range.second; ++it)
+      if ((*it->second)->key == key) {
+        hit = *it->second;
+        pc.lru.splice(pc.lru.begin(), pc.lru, it->second);  // most recent
+        break;
+      }
+    if (hit) pc.hits++;
+    else pc.misses++;
+  }
+  if (hit) {
+    for (int32_t i = 0; i < n; ++i) {
+      L.qcol[beg + i] = nodes[hit->qcol[i]];
+      L.qpos[nodes[hit->qcol[i]]] = i;
+      L.prow[beg + i] = nodes[hit->prow[i]];
+    }
+    for (size_t e = 0; e < ents.size(); ++e) L.lu_pos[ents[e]] = hit->lu_pos[e];
+    out = hit->res;
+    for (auto &k : out.fr.ent_k) k = ents[k];
+    return;
+  }
+  analyse_component_uncached(plan, L, c, pairs, out);
+  if (out.rc != XG_OK) return;
+  auto e = std::make_shared<CacheEntry>();
+  e->qcol.resize(n);
+  e->prow.resize(n);
+  for (int32_t i = 0; i < n; ++i) {
+    e->qcol[i] = plan.comp_local[L.qcol[beg + i]];
+    e->prow[i] = plan.comp_local[L.prow[beg + i]];
+  }
+  e->lu_pos.resize(ents.size());
+  for (size_t q = 0; q < ents.size(); ++q) e->lu_pos[q] = L.lu_pos[ents[q]];
+  e->res = out;
+  // the fronts list the entries in this same order: store their ordinals
+  if (e->res.fr.ent_k.size() != ents.size() && !e->res.fr.ent_k.empty()) return;
+  for (size_t q = 0; q < e->res.fr.ent_k.size(); ++q) e->res.fr.ent_k[q] = (int32_t)q;
+  e->key = std::move(key);
+  e->bytes = entry_bytes(*e);
+  std::lock_guard<std::mutex> g(pc.mu);
+  if (e->bytes > pc.limit / 4) return;  // too large to keep
+  pc.lru.push_front(e);
+  pc.map.emplace(h, pc.lru.begin());
+  pc.bytes += e->bytes;
+  while (pc.bytes > pc.limit && !pc.lru.empty()) {
+    auto last = std::prev(pc.lru.end());
+    const uint64_t hk = fnv64((*last)->key);
+    auto range = pc.map.equal_range(hk);
+    for (auto it = range.first; it != range.second; ++it)
+      if (it->second == last) {
+        pc.map.erase(it);
+        break;
+      }
+    pc.bytes -= (*last)->bytes;
+    pc.lru.erase(last);
+  }
+}
+
 }  // namespace
 
 int lu_analyse(Plan &plan, std::string &err) {
@@ -914,28 +1074,42 @@ int lu_analyse(Plan &plan, std::string &err) {
     return plan.comp_start[a + 1] - plan.comp_start[a] >
            plan.comp_start[b + 1] - plan.comp_start[b];
   });
-  std::atomic<int32_t> next{0};
+  // components of one structure: the first of each (largest first) is
+  // analysed in a first pass, the others in a second pass, where they take
+  // its stored analysis -- a repeated sub-circuit of a batch is analysed
+  // once
+  std::vector<int32_t> first, again;
+  {
+    std::vector<int32_t> key, ents;
+    std::unordered_map<uint64_t, int32_t> seen;
+    for (int32_t c : by_size) {
+      component_key(plan, c, comp_pairs[c], key, ents);
+      (seen.emplace(fnv64(key), c).second ? first : again).push_back(c);
+    }
+  }
   // an allocation failure inside a component's analysis is that
   // component's result (XG_E_NOMEM), never an exception leaving a thread
-  auto work = [&]() {
-    for (int32_t k; (k = next.fetch_add(1)) < nc;) {
-      const int32_t c = by_size[k];
-      try {
-        analyse_component(plan, L, c, comp_pairs[c], res[c]);
-      } catch (const std::bad_alloc &) {
-        res[c] = CompLu();
-        res[c].rc = XG_E_NOMEM;
-        res[c].err = "sparse LU analysis: out of host memory";
-      } catch (const std::exception &e) {
-        res[c] = CompLu();
-        res[c].rc = XG_E_ARG;
-        res[c].err = std::string("sparse LU analysis: ") + e.what();
+  auto run_pass = [&](const std::vector<int32_t> &list) {
+    std::atomic<int32_t> next{0};
+    const int32_t cnt = (int32_t)list.size();
+    auto work = [&]() {
+      for (int32_t k; (k = next.fetch_add(1)) < cnt;) {
+        const int32_t c = list[k];
+        try {
+          analyse_component(plan, L, c, comp_pairs[c], res[c]);
+        } catch (const std::bad_alloc &) {
+          res[c] = CompLu();
+          res[c].rc = XG_E_NOMEM;
+          res[c].err = "sparse LU analysis: out of host memory";
+        } catch (const std::exception &e) {
+          res[c] = CompLu();
+          res[c].rc = XG_E_ARG;
+          res[c].err = std::string("sparse LU analysis: ") + e.what();
+        }
       }
-    }
-  };
-  const int32_t n_threads =
-      std::max(1, std::min<int32_t>({nc, 16, (int32_t)std::thread::hardware_concurrency()}));
-  {
+    };
+    const int32_t n_threads = std::max(
+        1, std::min<int32_t>({cnt, 16, (int32_t)std::thread::hardware_concurrency()}));
     std::vector<std::thread> pool;
     pool.reserve(n_threads);
     try {
@@ -945,7 +1119,9 @@ int lu_analyse(Plan &plan, std::string &err) {
     }
     work();  // catches inside, so the pool is always joined below
     for (auto &th : pool) th.join();
-  }
+  };
+  run_pass(first);
+  run_pass(again);
   L.front_comp.assign(nc, 0);
   L.fr_fo.assign(1, 0);
   L.fr_ro.assign(1, 0);
@@ -1068,6 +1244,24 @@ int lu_analyse(Plan &plan, std::string &err) {
     L.ent_off.push_back((int32_t)L.ent_k.size());
   }
   return XG_OK;
+}
+
+void lu_cache_stats(int64_t out[3]) {
+  PlanCache &pc = plan_cache();
+  std::lock_guard<std::mutex> g(pc.mu);
+  out[0] = pc.hits;
+  out[1] = pc.misses;
+  out[2] = (int64_t)pc.bytes;
+}
+
+void lu_cache_set_limit(int64_t bytes) {
+  PlanCache &pc = plan_cache();
+  std::lock_guard<std::mutex> g(pc.mu);
+  pc.limit = bytes > 0 ? (size_t)bytes : 0;
+  pc.lru.clear();
+  pc.map.clear();
+  pc.bytes = 0;
+  pc.hits = pc.misses = 0;
 }
 
 }  // namespace xg

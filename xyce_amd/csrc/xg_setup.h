@@ -204,6 +204,10 @@ int build_plan(Circuit &c, Plan &plan, std::string &err);
 int diode_setup(const ParamSet &model, const ParamSet &inst, const Options &opt,
                 double *fields, std::string &err);
 int lu_analyse(Plan &plan, std::string &err);
+// analysed components by topology (process-wide): {hits, misses, bytes};
+// a new byte limit (0 = off) empties it and zeroes the counts
+void lu_cache_stats(int64_t out[3]);
+void lu_cache_set_limit(int64_t bytes);
 
 // BSIM3 hooks (xg_bsim3_setup.cpp).
 int bsim3_add_model(Circuit &c, int dtype, const ParamSet &p, int32_t *id,

@@ -66,20 +66,31 @@ class DeviceLoad:
                 fn = L.xg_add_mos1_model if m.kind == "mos1" else L.xg_add_bsim3_model
                 check(fn(c, m.dtype, n, k, v, ctypes.byref(out)), "add model " + name)
                 mid[name] = out.value
-            for rec in nl.mos1:
-                n, k, v = keys_vals(rec["params"])
-                lids = (ctypes.c_int32 * 6)(*rec["lids"])
-                out = ctypes.c_int32()
-                check(L.xg_add_mos1_instance(c, mid[rec["model"]], n, k, v, lids,
-                                             rec["sta"], rec["sto"], ctypes.byref(out)),
-                      "add MOSFET1 " + rec["name"])
-            for rec in nl.bsim3:
-                n, k, v = keys_vals(rec["params"])
-                lids = (ctypes.c_int32 * 6)(*rec["lids"])
-                out = ctypes.c_int32()
-                check(L.xg_add_bsim3_instance(c, mid[rec["model"]], n, k, v, lids,
-                                              rec["sta"], rec["sto"], ctypes.byref(out)),
-                      "add BSIM3 " + rec["name"])
+            # instances in runs of one parameter-name list, one bulk call
+            # per run (the same as one call per instance, in order)
+            for recs, fn, what in ((nl.mos1, L.xg_add_mos1_instances, "MOSFET1"),
+                                   (nl.bsim3, L.xg_add_bsim3_instances, "BSIM3")):
+                i = 0
+                while i < len(recs):
+                    r0 = recs[i]
+                    names = tuple(r0["params"])
+                    j = i + 1
+                    while j < len(recs) and tuple(recs[j]["params"]) == names:
+                        j += 1
+                    run = recs[i:j]
+                    n, k, _ = keys_vals(r0["params"])
+                    vals = np.array([list(r["params"].values()) for r in run],
+                                    dtype=np.float64).reshape(len(run), n)
+                    models = np.array([mid[r["model"]] for r in run], dtype=np.int32)
+                    lids = np.array([r["lids"] for r in run], dtype=np.int32)
+                    sta = np.array([r["sta"] for r in run], dtype=np.int32)
+                    sto = np.array([r["sto"] for r in run], dtype=np.int32)
+                    ids = np.zeros(len(run), np.int32)
+                    check(fn(c, models.ctypes.data, len(run), n, k, vals.ctypes.data,
+                             lids.ctypes.data, sta.ctypes.data, sto.ctypes.data,
+                             ids.ctypes.data),
+                          "add %s %s..%s" % (what, r0["name"], run[-1]["name"]))
+                    i = j
             # resistors and capacitors in two calls (each type keeps its order)
             rs, cs = [], []
             for kind, p, q, br, val, pulse, st in nl.linear_devices():
