@@ -439,9 +439,16 @@ def serving_leg(args):
     (micro-batches of one kind every 5 ms, up to 64 requests, 3 batches in
     flight).  Plain: the same stream (its first --serve-plain requests), one
     circuit per launch, one at a time.  Latency = completion - arrival,
-    including the library setup and upload of each batch."""
+    including the library setup and upload of each batch.  Each form starts
+    with an empty analysed-plan cache (the process-wide reuse of analysed
+    sub-circuits), so neither reuses the other's analyses."""
+    from xyce_amd._lib import check, lib
     from xyce_amd.serving import poisson_requests, serve, summary
+
+    def fresh_cache():
+        check(lib().xg_lu_cache_set_limit(256 << 20), "plan cache reset")
     progress("serving: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
+    fresh_cache()
     reqs = poisson_requests(rate=args.serve_rate, n=args.serve_requests, seed=4,
                             logic_gates=(125, 1250))
     b = summary(serve(reqs, max_batch=64, workers=3, window=0.005))
@@ -454,6 +461,7 @@ def serving_leg(args):
     if args.serve_plain > 0:
         plain = poisson_requests(rate=args.serve_rate, n=args.serve_requests, seed=4,
                                  logic_gates=(125, 1250))[:args.serve_plain]
+        fresh_cache()
         p = summary(serve(plain, max_batch=1, workers=1, window=0.001))
         progress("serving: plain %.1f solves/s, p99 %.3f s" % (p["solves_per_s"],
                                                               p["latency_p99_s"]))

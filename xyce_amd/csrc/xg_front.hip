@@ -498,7 +498,7 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
 // algorithm gives them.
 template <int ROWS>
 __global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
-  __shared__ double tb[ROWS];
+  __shared__ double tb[ROWS], xs[ROWS];
   const int s = f.lv[lv0 + blockIdx.x];
   const int c = f.comp[s];
   if (front_skip(f, c)) return;
@@ -506,11 +506,17 @@ __global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
   const int m = f.m[s], k = f.k[s];
   const double *A = f.vals + f.fo[s];
   const int32_t *rows = f.rows + f.ro[s];
-  for (int i = t; i < k; i += kFB) {
-    double r = f.v[rows[i]];
+  // the right-hand side less the solved rows below: a wavefront per row,
+  // its lanes strided over the row, summed by a fixed shuffle tree
+  for (int j = k + t; j < m; j += kFB) xs[j - k] = f.v[rows[j]];
+  __syncthreads();
+  const int w = t >> 6, lane = t & 63;
+  for (int i = w; i < k; i += kFB / 64) {
     const double *a = A + (int64_t)i * m;
-    for (int j = k; j < m; ++j) r -= a[j] * f.v[rows[j]];
-    tb[i] = r;
+    double p = 0.0;
+    for (int j = k + lane; j < m; j += 64) p += a[j] * xs[j - k];
+    for (int off = 32; off >= 1; off >>= 1) p += __shfl_xor(p, off);
+    if (lane == 0) tb[i] = f.v[rows[i]] - p;
   }
   __syncthreads();
   for (int jb = ((k - 1) / 64) * 64; jb >= 0 && k > 0; jb -= 64) {
