@@ -272,6 +272,62 @@ __global__ void __launch_bounds__(BS) front_factor_kernel(FrontArgs f, int lv0) 
     if (t == 0) f.bad[c] = 1;
 }
 
+// Fronts of up to kTinyM rows, one thread each (a level of the tree's
+// bottom holds up to ~10^5 of them): the children's contribution blocks in
+// child order, then the right-looking LU with whole-row swaps, pivots among
+// the fully summed rows (largest magnitude, smallest index on ties).  Every
+// element takes the same operations in the same order as in the blocked
+// workgroup form, so both give the same bits.
+constexpr int kTinyM = 16;
+
+__global__ void __launch_bounds__(64) front_tiny_kernel(FrontArgs f, int lv0, int nfr) {
+  const int q = blockIdx.x * 64 + threadIdx.x;
+  if (q >= nfr) return;
+  const int s = f.lv[lv0 + q];
+  const int c = f.comp[s];
+  if (front_skip(f, c)) return;
+  const int m = f.m[s], k = f.k[s];
+  if (m > f.big_m) return;
+  double *A = f.vals + f.fo[s];
+  for (int qq = f.ch_ptr[s]; qq < f.ch_ptr[s + 1]; ++qq) {
+    const int ch = f.ch[qq];
+    const int mc = f.m[ch], kc = f.k[ch], w = mc - kc;
+    const double *C = f.vals + f.fo[ch] + (int64_t)kc * mc + kc;
+    const int32_t *map = f.cbmap + f.ro[ch] + kc;
+    for (int i = 0; i < w; ++i)
+      for (int j = 0; j < w; ++j) A[map[i] * m + map[j]] += C[i * mc + j];
+  }
+  int32_t *piv = f.piv + f.ro[s];
+  for (int col = 0; col < k; ++col) {
+    double best = -1.0;
+    int br = col;
+    for (int r = col; r < k; ++r) {
+      const double a = fabs(A[r * m + col]);
+      if (a > best) {
+        best = a;
+        br = r;
+      }
+    }
+    if (!(best > 0.0) || !(best <= DBL_MAX)) {
+      f.bad[c] = 1;
+      return;
+    }
+    piv[col] = br;
+    if (br != col)
+      for (int j = 0; j < m; ++j) {
+        const double tmp = A[col * m + j];
+        A[col * m + j] = A[br * m + j];
+        A[br * m + j] = tmp;
+      }
+    const double pv = A[col * m + col];
+    for (int r = col + 1; r < m; ++r) {
+      const double l = A[r * m + col] / pv;
+      A[r * m + col] = l;
+      for (int j = col + 1; j < m; ++j) A[r * m + j] -= l * A[col * m + j];
+    }
+  }
+}
+
 // ---- many-workgroup form of the large fronts (rows m > big_m), all of a
 // level's at once (blockIdx.y or .z = the front among the level's large
 // ones, f.big[lb + that]): per panel of kBigW fully summed columns, one
@@ -452,14 +508,23 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
     for (int i = t; i < wc; i += kFB) ws[map[i]] += r[i];
     __syncthreads();
   }
-  if (t == 0) {
-    const int32_t *pv = f.piv + f.ro[s];
-    for (int j = 0; j < k; ++j) {
-      const int p = pv[j];
-      if (p != j) {
-        const double tmp = ws[j];
-        ws[j] = ws[p];
-        ws[p] = tmp;
+  // the row swaps in pivot order: one wavefront finds the pivots that
+  // move a row (64 at a time) and its first lane applies just those
+  const int32_t *pv = f.piv + f.ro[s];
+  if (t < 64) {
+    for (int j0 = 0; j0 < k; j0 += 64) {
+      const int j = j0 + t;
+      const int p = j < k ? pv[j] : j;
+      unsigned long long mv = __ballot(p != j);
+      while (mv) {
+        const int b = __ffsll(mv) - 1;
+        const int pb = __shfl(p, b);
+        if (t == 0) {
+          const double tmp = ws[j0 + b];
+          ws[j0 + b] = ws[pb];
+          ws[pb] = tmp;
+        }
+        mv &= mv - 1;
       }
     }
   }
@@ -849,7 +914,11 @@ hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t 
       if (mq > f.big_m) any_big = true;
       else small_max = std::max(small_max, mq);
     }
-    if (small_max > 0) {
+    if (small_max > 0 && small_max <= kTinyM) {
+      hipLaunchKernelGGL(front_tiny_kernel, dim3((nfr + 63) / 64), dim3(64), 0, s, f, lv0, nfr);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    } else if (small_max > 0) {
       switch (front_form(small_max)) {
         case 1: e = launch_level<16, 64>(f, lv0, nfr, s); break;
         case 2: e = launch_level<16, 256>(f, lv0, nfr, s); break;
