@@ -410,7 +410,7 @@ int xg_lu_info(xg_handle *h, int64_t info[8]);
 int xg_lu_cache_stats(int64_t out[3]);
 int xg_lu_cache_set_limit(int64_t bytes);
 
-/* Fronts of more than big_m rows (default 256) are factored panel by panel
+/* Fronts of more than big_m rows (default 64) are factored panel by panel
    across many workgroups (one workgroup per panel, then every column's row
    swaps and U12 rows, then the trailing update in 64 x 64 tiles); smaller
    ones by one workgroup each.  Both forms give the same bits.  No

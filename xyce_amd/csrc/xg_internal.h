@@ -279,7 +279,7 @@ struct SparseLuArgs {
 // lists; the plan is LuPlan's fr_* arrays (xg_setup.h).
 constexpr int kFrontMaxM = 3481;  // largest front a workgroup's panel holds
 // Fronts of more rows are factored by many workgroups per panel.
-constexpr int32_t kFrontBigM = 256;
+constexpr int32_t kFrontBigM = 64;
 struct FrontArgs {
   int32_t n_fronts = 0, n_comp = 0, n_levels = 0;
   int64_t n_vals = 0, n_ent = 0;
