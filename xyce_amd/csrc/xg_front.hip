@@ -339,6 +339,7 @@ __global__ void __launch_bounds__(64) front_tiny_kernel(FrontArgs f, int lv0, in
 // sees the same operations in the same order as in the one-workgroup
 // form, so both give the same bits.
 constexpr int kBigW = 32;   // panel width
+constexpr int kPanelBlock = 512;  // threads of a panel workgroup (256: 5.8 ms, 512: 4.2 ms, 1024: 5.9 ms of panels per logic factorisation)
 constexpr int kTile = 64;   // trailing-update tile
 
 // The children's contribution blocks, one workgroup per receiving row of
@@ -362,11 +363,11 @@ __global__ void __launch_bounds__(kFB) front_extend_kernel(FrontArgs f, int lb) 
   }
 }
 
-template <int NB, int ROWS>
-__global__ void __launch_bounds__(kFB) front_panel_kernel(FrontArgs f, int lb, int kb) {
+template <int NB, int ROWS, int BS>
+__global__ void __launch_bounds__(BS) front_panel_kernel(FrontArgs f, int lb, int kb) {
   __shared__ double P[ROWS * (NB + 1)];
-  __shared__ double sv[kFB / 64];
-  __shared__ int si[kFB / 64];
+  __shared__ double sv[BS / 64];
+  __shared__ int si[BS / 64];
   __shared__ int spiv[NB];
   const int s = f.big[lb + blockIdx.x];
   const int c = f.comp[s];
@@ -374,15 +375,20 @@ __global__ void __launch_bounds__(kFB) front_panel_kernel(FrontArgs f, int lb, i
   const int m = f.m[s], k = f.k[s];
   if (kb >= k) return;
   const int cl = min(kb + kBigW, k);
-  if (!front_factor<NB>(f.vals + f.fo[s], m, k, P, sv, si, spiv, f.piv + f.ro[s], kb, cl,
-                        cl, kb))
+  if (!front_factor<NB, BS>(f.vals + f.fo[s], m, k, P, sv, si, spiv, f.piv + f.ro[s], kb, cl,
+                            cl, kb))
     if (threadIdx.x == 0) f.bad[c] = 1;
 }
 
 // Columns j >= cl: the panel's row swaps in pivot order, then U12 = L11^-1
 // (rows kb..cl-1) with the unit lower panel block, each u[i] updated in
 // ascending pivot order.
-__global__ void __launch_bounds__(kFB) front_swap_trsm_kernel(FrontArgs f, int lb, int kb) {
+// One wavefront per 64 columns: the work is a few dependent loads and a
+// 32-term solve per column, so more, smaller workgroups spread it over
+// more CUs.
+constexpr int kSwapBlock = 64;
+
+__global__ void __launch_bounds__(kSwapBlock) front_swap_trsm_kernel(FrontArgs f, int lb, int kb) {
   __shared__ double L11[kBigW][kBigW + 1];
   __shared__ int32_t pv[kBigW];
   const int s = f.big[lb + blockIdx.y];
@@ -390,16 +396,16 @@ __global__ void __launch_bounds__(kFB) front_swap_trsm_kernel(FrontArgs f, int l
   const int m = f.m[s], k = f.k[s];
   if (kb >= k) return;
   const int cl = min(kb + kBigW, k), nw = cl - kb;
-  if (cl + (int)(blockIdx.x * kFB) >= m) return;
+  if (cl + (int)(blockIdx.x * kSwapBlock) >= m) return;
   double *A = f.vals + f.fo[s];
   const int32_t *piv = f.piv + f.ro[s];
-  for (int e = threadIdx.x; e < kBigW * kBigW; e += kFB) {
+  for (int e = threadIdx.x; e < kBigW * kBigW; e += kSwapBlock) {
     const int i = e / kBigW, cc = e % kBigW;
     L11[i][cc] = (i < nw && cc < nw) ? A[(int64_t)(kb + i) * m + kb + cc] : 0.0;
   }
   if (threadIdx.x < kBigW) pv[threadIdx.x] = threadIdx.x < nw ? piv[kb + threadIdx.x] : 0;
   __syncthreads();
-  const int j = cl + blockIdx.x * kFB + threadIdx.x;
+  const int j = cl + blockIdx.x * kSwapBlock + threadIdx.x;
   if (j >= m) return;
   for (int cc = 0; cc < nw; ++cc) {
     const int pr = pv[cc];
@@ -859,9 +865,11 @@ hipError_t launch_big_fronts(const FrontArgs &f, int lb, int nb, hipStream_t s) 
                        lb);
   for (int kb = 0; kb < max_k; kb += kBigW) {
     if (max_m <= 1934)
-      hipLaunchKernelGGL((front_panel_kernel<8, 1934>), dim3(nb), dim3(kFB), 0, s, f, lb, kb);
+      hipLaunchKernelGGL((front_panel_kernel<8, 1934, kPanelBlock>), dim3(nb), dim3(kPanelBlock),
+                         0, s, f, lb, kb);
     else
-      hipLaunchKernelGGL((front_panel_kernel<4, kFrontMaxM>), dim3(nb), dim3(kFB), 0, s, f, lb,
+      hipLaunchKernelGGL((front_panel_kernel<4, kFrontMaxM, kPanelBlock>), dim3(nb),
+                         dim3(kPanelBlock), 0, s, f, lb,
                          kb);
     // the largest trailing block at this panel position
     int rest = 0;
@@ -871,8 +879,8 @@ hipError_t launch_big_fronts(const FrontArgs &f, int lb, int nb, hipStream_t s) 
         rest = std::max(rest, f.m_host[sf] - std::min(kb + kBigW, f.k_host[sf]));
     }
     if (rest <= 0) continue;
-    hipLaunchKernelGGL(front_swap_trsm_kernel, dim3((rest + kFB - 1) / kFB, nb), dim3(kFB), 0,
-                       s, f, lb, kb);
+    hipLaunchKernelGGL(front_swap_trsm_kernel, dim3((rest + kSwapBlock - 1) / kSwapBlock, nb),
+                       dim3(kSwapBlock), 0, s, f, lb, kb);
     const unsigned tiles = (unsigned)((rest + kTile - 1) / kTile);
     hipLaunchKernelGGL(front_update_kernel, dim3(tiles, tiles, nb), dim3(kFB), 0, s, f, lb,
                        kb);
