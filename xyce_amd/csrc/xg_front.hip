@@ -38,6 +38,7 @@ namespace {
 
 constexpr int kFB = 256;           // threads per front workgroup
 constexpr int kFinishBlock = 1024;  // threads of the per-component check
+constexpr int kSolveBlock = 512;    // threads of a solve workgroup for fronts above 512 rows
 
 // (value, smallest index) maximum over the block: per wavefront, then
 // across the B / 64 wavefronts.
@@ -339,7 +340,9 @@ __global__ void __launch_bounds__(64) front_tiny_kernel(FrontArgs f, int lv0, in
 // sees the same operations in the same order as in the one-workgroup
 // form, so both give the same bits.
 constexpr int kBigW = 32;   // panel width
-constexpr int kPanelBlock = 512;  // threads of a panel workgroup (256: 5.8 ms, 512: 4.2 ms, 1024: 5.9 ms of panels per logic factorisation)
+// threads of a panel workgroup (per logic factorisation, 256: 5.8 ms of
+// panels, 512: 4.2 ms, 1024: 5.9 ms)
+constexpr int kPanelBlock = 512;
 constexpr int kTile = 64;   // trailing-update tile
 
 // The children's contribution blocks, one workgroup per receiving row of
@@ -492,8 +495,8 @@ __global__ void __launch_bounds__(kFB) front_left_swaps_kernel(FrontArgs f, int 
 // solution broadcast by lane shuffles), then every row below takes the
 // block's 64 terms.  Each element takes its terms in ascending column
 // order, as the column-by-column algorithm gives them.
-template <int ROWS>
-__global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const double *bp,
+template <int ROWS, int BS>
+__global__ void __launch_bounds__(BS) front_fwd_kernel(FrontArgs f, const double *bp,
                                                         int lv0) {
   __shared__ double ws[ROWS];
   const int s = f.lv[lv0 + blockIdx.x];
@@ -504,14 +507,14 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
   const double *A = f.vals + f.fo[s];
   const int32_t *rows = f.rows + f.ro[s];
   double *w = f.wrhs + f.ro[s];
-  for (int i = t; i < m; i += kFB) ws[i] = i < k ? bp[rows[i]] : 0.0;
+  for (int i = t; i < m; i += BS) ws[i] = i < k ? bp[rows[i]] : 0.0;
   __syncthreads();
   for (int q = f.ch_ptr[s]; q < f.ch_ptr[s + 1]; ++q) {
     const int ch = f.ch[q];
     const int kc = f.k[ch], wc = f.m[ch] - kc;
     const double *r = f.wrhs + f.ro[ch] + kc;
     const int32_t *map = f.cbmap + f.ro[ch] + kc;
-    for (int i = t; i < wc; i += kFB) ws[map[i]] += r[i];
+    for (int i = t; i < wc; i += BS) ws[map[i]] += r[i];
     __syncthreads();
   }
   // the row swaps in pivot order: one wavefront finds the pivots that
@@ -547,7 +550,7 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
       if (i < je) ws[i] = wi;
     }
     __syncthreads();
-    for (int i = je + t; i < m; i += kFB) {
+    for (int i = je + t; i < m; i += BS) {
       double r = ws[i];
       const double *a = A + (int64_t)i * m;
       for (int j = jb; j < je; ++j) r -= a[j] * ws[j];
@@ -555,7 +558,7 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
     }
     __syncthreads();
   }
-  for (int i = t; i < m; i += kFB) {
+  for (int i = t; i < m; i += BS) {
     w[i] = ws[i];
     if (i < k) f.v[rows[i]] = ws[i];
   }
@@ -567,8 +570,8 @@ __global__ void __launch_bounds__(kFB) front_fwd_kernel(FrontArgs f, const doubl
 // wavefront, then every row above takes the block's terms; each element
 // takes its terms in descending column order, as the column-by-column
 // algorithm gives them.
-template <int ROWS>
-__global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
+template <int ROWS, int BS>
+__global__ void __launch_bounds__(BS) front_bwd_kernel(FrontArgs f, int lv0) {
   __shared__ double tb[ROWS], xs[ROWS];
   const int s = f.lv[lv0 + blockIdx.x];
   const int c = f.comp[s];
@@ -579,10 +582,10 @@ __global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
   const int32_t *rows = f.rows + f.ro[s];
   // the right-hand side less the solved rows below: a wavefront per row,
   // its lanes strided over the row, summed by a fixed shuffle tree
-  for (int j = k + t; j < m; j += kFB) xs[j - k] = f.v[rows[j]];
+  for (int j = k + t; j < m; j += BS) xs[j - k] = f.v[rows[j]];
   __syncthreads();
   const int w = t >> 6, lane = t & 63;
-  for (int i = w; i < k; i += kFB / 64) {
+  for (int i = w; i < k; i += BS / 64) {
     const double *a = A + (int64_t)i * m;
     double p = 0.0;
     for (int j = k + lane; j < m; j += 64) p += a[j] * xs[j - k];
@@ -607,7 +610,7 @@ __global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
       if (i < je) tb[i] = ti;
     }
     __syncthreads();
-    for (int i = t; i < jb; i += kFB) {
+    for (int i = t; i < jb; i += BS) {
       double r = tb[i];
       const double *a = A + (int64_t)i * m;
       for (int j = je - 1; j >= jb; --j) r -= a[j] * tb[j];
@@ -615,7 +618,7 @@ __global__ void __launch_bounds__(kFB) front_bwd_kernel(FrontArgs f, int lv0) {
     }
     __syncthreads();
   }
-  for (int i = t; i < k; i += kFB) f.v[rows[i]] = tb[i];
+  for (int i = t; i < k; i += BS) f.v[rows[i]] = tb[i];
 }
 
 // The residual and |J| row-sum maxima of a component's long rows in
@@ -944,27 +947,29 @@ hipError_t launch_fronts(const FrontArgs &f, const SparseLuArgs &a, hipStream_t 
           return e;
   }
   // the solves' LDS holds a front's rows: the level's largest front picks
-  // the instance
+  // the instance (the largest fronts with 512 threads)
   for (int L = 0; L < f.n_levels; ++L) {
     const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
     const int mm = f.lv_maxm_host[L];
     const double *bp = a.bp;
     if (mm <= 64)
-      hipLaunchKernelGGL(front_fwd_kernel<64>, dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
+      hipLaunchKernelGGL((front_fwd_kernel<64, kFB>), dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
     else if (mm <= 512)
-      hipLaunchKernelGGL(front_fwd_kernel<512>, dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
+      hipLaunchKernelGGL((front_fwd_kernel<512, kFB>), dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
     else
-      hipLaunchKernelGGL(front_fwd_kernel<kFrontMaxM>, dim3(nfr), dim3(kFB), 0, s, f, bp, lv0);
+      hipLaunchKernelGGL((front_fwd_kernel<kFrontMaxM, kSolveBlock>), dim3(nfr), dim3(kSolveBlock),
+                         0, s, f, bp, lv0);
   }
   for (int L = f.n_levels - 1; L >= 0; --L) {
     const int lv0 = f.lv_ptr_host[L], nfr = f.lv_ptr_host[L + 1] - lv0;
     const int mm = f.lv_maxm_host[L];
     if (mm <= 64)
-      hipLaunchKernelGGL(front_bwd_kernel<64>, dim3(nfr), dim3(kFB), 0, s, f, lv0);
+      hipLaunchKernelGGL((front_bwd_kernel<64, kFB>), dim3(nfr), dim3(kFB), 0, s, f, lv0);
     else if (mm <= 512)
-      hipLaunchKernelGGL(front_bwd_kernel<512>, dim3(nfr), dim3(kFB), 0, s, f, lv0);
+      hipLaunchKernelGGL((front_bwd_kernel<512, kFB>), dim3(nfr), dim3(kFB), 0, s, f, lv0);
     else
-      hipLaunchKernelGGL(front_bwd_kernel<kFrontMaxM>, dim3(nfr), dim3(kFB), 0, s, f, lv0);
+      hipLaunchKernelGGL((front_bwd_kernel<kFrontMaxM, kSolveBlock>), dim3(nfr),
+                         dim3(kSolveBlock), 0, s, f, lv0);
   }
   hipLaunchKernelGGL(front_finish_kernel, dim3(f.n_comp), dim3(kFinishBlock), 0, s, f, a);
   if (f.n_fin_chunks > 0) {
