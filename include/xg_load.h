@@ -578,6 +578,24 @@ int xg_tran_uic(xg_tran *t);
 /* One step of dt (order 1 for the first two steps, then 2); a step whose
    Newton solve fails is replaced by two of dt/2, recursively. */
 int xg_tran_step(xg_tran *t, double dt);
+/* n_steps fixed steps of dt (the steps of xg_tran_step, bit-identical
+   results), probing every `every`-th: step g = first_step + s (s = 1 ..
+   n_steps) with g % every == 0 writes x_curr[ids] to row g / every - 1 of
+   out (device, n_ids per row) and its time to times[g / every - 1] (host,
+   may be NULL).  After xg_tran_prepare_run, later steps (order 2) run in
+   groups of 8 per graph launch, each step started on the device when the
+   one before converged within the graph's iterations, so the host reads
+   the control words once per group; a step that needs more iterations or
+   fails halts the group and continues as in xg_tran_step.  No counterpart
+   in the reference (its fixed-step loop is N_ANP_Transient's, one step per
+   host call). */
+int xg_tran_run(xg_tran *t, double dt, int64_t n_steps, const int32_t *ids, int32_t n_ids,
+                int32_t every, int64_t first_step, double *out, double *times);
+/* Captures xg_tran_run's graphs of 8 steps (1 to 4 iterations per step);
+   call it where graph captures are safe, as at xg_tran_create.  A circuit
+   with dense-front components (milliseconds and hundreds of launches per
+   iteration) gets none and keeps one step per launch. */
+int xg_tran_prepare_run(xg_tran *t);
 /* Variable-step ONESTEP (.TRAN TSTEP TSTOP with step control), after
    xg_tran_dcop or xg_tran_uic.  The scalar control of the reference's
    N_TIA_StepErrorControl and N_TIA_OneStep: local error estimate

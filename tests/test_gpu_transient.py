@@ -419,3 +419,41 @@ def test_uic_start_logic_parity():
     assert np.allclose(tg, tr_) and np.array_equal(tg, tp)
     assert wg.tobytes() == wp.tobytes()
     assert err <= 1e-3 * 1.2 + 1e-6
+
+
+@pytest.mark.parametrize("dt,every", [(1e-12, 1), (1e-12, 3), (5e-12, 7), (2e-11, 2)])
+def test_fused_steps_equal_one_step_per_launch(dt, every):
+    """Fixed steps in groups of 8 per graph launch (xg_tran_run after
+    xg_tran_prepare_run) against one step per launch: the same kernels in
+    the same order, so the waveforms, the time grid, the step and Newton
+    counts and the halvings agree bit for bit.  The larger steps need more
+    iterations than a group's graph holds (the group halts and the host
+    continues that step), and the largest fail and are halved; the run with
+    progress reports goes in calls of 50 steps.  Fewer control-word reads
+    with the groups."""
+    import ctypes
+    from xyce_amd import circuits
+    from xyce_amd._lib import lib
+    from xyce_amd.transient import Transient
+    mk = lambda: circuits.merge(circuits.monte_carlo_rings(6, seed=11, stages=(11, 61)))
+    probes = ["s0_out0", "s2_out3", "s5_out1", "s3_out0"]
+    tstop = 60 * dt * every + 5 * dt
+    res = []
+    for fused in (True, False):
+        tr = Transient(mk(), driver="native", fused_steps=fused)
+        seen = []
+        t, w = tr.run(tstop, dt, probes, every=every,
+                      progress=(lambda s, n: seen.append(s)) if dt == 1e-12 else None)
+        c = (ctypes.c_int64 * 4)()
+        assert lib().xg_tran_counters(tr._tr, c) == 0
+        res.append((t, w, tr.n_steps, tr.newton_total, tr.step_failures, c[0], seen))
+        print("fused %s: %d steps, %d Newton iterations, %d halvings, %d control reads"
+              % (fused, tr.n_steps, tr.newton_total, tr.step_failures, c[0]), flush=True)
+        tr.close()
+    (ta, wa, *ca, ra, sa), (tb, wb, *cb, rb, sb) = res
+    assert len(ta) == int(round(tstop / dt)) // every + 1
+    assert np.array_equal(ta, tb)
+    assert wa.tobytes() == wb.tobytes()
+    assert ca == cb
+    assert sa == sb
+    assert ra < rb

@@ -113,6 +113,10 @@ SIGNATURES = [
     ("xg_tran_dcop", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_tran_uic", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_tran_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("xg_tran_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    ("xg_tran_prepare_run", ctypes.c_int, [ctypes.c_void_p]),
     ("xg_tran_adaptive_begin", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
                                               ctypes.c_double]),

@@ -711,6 +711,7 @@ int32_t handle_n_comp(const xg_handle *H) {
   return (int32_t)H->h.plan.comp_start.size() - 1;
 }
 int64_t handle_n_nodes(const xg_handle *H) { return H->h.plan.n_nodes; }
+int32_t handle_n_fronts(const xg_handle *H) { return H->h.fr.n_fronts; }
 
 void handle_voltage_diagonal(const xg_handle *H, std::vector<int32_t> &ids,
                              std::vector<int32_t> &pos) {

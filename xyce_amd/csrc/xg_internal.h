@@ -362,6 +362,7 @@ hipStream_t handle_stream(const xg_handle *h);
 void handle_set_gate(xg_handle *h, const int32_t *gate);
 int32_t handle_n_comp(const xg_handle *h);
 int64_t handle_n_nodes(const xg_handle *h);
+int32_t handle_n_fronts(const xg_handle *h);
 // Voltage unknowns (every unknown but the voltage sources' branch currents)
 // with a diagonal Jacobian entry, and that entry's CSR position.
 void handle_voltage_diagonal(const xg_handle *h, std::vector<int32_t> &ids,

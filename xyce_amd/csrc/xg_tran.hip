@@ -46,6 +46,12 @@ struct xg_tran {
   int32_t k_step = 1;
   int64_t host_reads = 0;  // control-word reads (host round trips)
   int32_t *ctl_host = nullptr;  // pinned copy of the control words
+  // fused fixed steps (xg_tran_run): graphs of kRunSteps whole steps by
+  // iterations per step, their device descriptor and its pinned copies in
+  // and out
+  hipGraphExec_t run_exec[kMaxK + 1] = {};
+  hipGraph_t run_graph[kMaxK + 1] = {};
+  void *run_dev = nullptr, *run_in = nullptr, *run_out = nullptr;
   std::string err;
   // variable-step control (xg_tran_adaptive_*): the scalar state of the
   // reference's N_TIA_StepErrorControl and N_TIA_OneStep, and the device
@@ -105,6 +111,22 @@ enum { kIter = 0, kDone = 1, kFail = 2, kLeft = 3, kStop = 4, kCtlWords = 8 };
 // iterations per graph of the DC and the continuing-iteration loops
 constexpr int kDcIters = 4, kTranIters = 2;
 
+// Fused fixed steps: the words the host sets per group of steps (time,
+// step, probe output) and those the graph returns (steps done, iterations
+// per step, halt), in device memory so that one captured graph serves
+// every group.  Halt: a step of the group did not converge within the
+// graph's iterations, or failed; every later launch of the group does
+// nothing and the host takes that step over.
+constexpr int kRunSteps = 8;
+struct RunDesc {
+  double t, dt, inv_dt;
+  const int32_t *ids;
+  double *out;
+  int64_t s0;  // steps of the whole run before this group
+  int32_t every, n_ids, halt, done;
+  int32_t iters[kRunSteps];
+};
+
 // Step start: predictor (the rounding sequence of the predictor kernel),
 // iterate = predictor, step parameters, control words and masks reset.
 __global__ void __launch_bounds__(kBlock)
@@ -112,7 +134,13 @@ __global__ void __launch_bounds__(kBlock)
                       double beta2, const double *x_curr, const double *xh1,
                       const double *xh2, double *xpred, double *x, double *step,
                       double t, double bp_tol, double inv_dt, int32_t *conv,
-                      int32_t *active, int32_t *ctl, const int32_t *disabled) {
+                      int32_t *active, int32_t *ctl, const int32_t *disabled,
+                      const RunDesc *run) {
+  if (run) {  // a fused step: its time from the group's descriptor
+    if (run->halt) return;
+    t = run->t + run->dt;
+    inv_dt = run->inv_dt;
+  }
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n1) {
     double v = x_curr[i];
@@ -339,6 +367,43 @@ __global__ void __launch_bounds__(kBlock)
   xh2[i] = 0.0;
 }
 
+// End of fused step j: a converged step advances the group's time (the
+// host's t + dt, in the same order) and counts as done; otherwise the
+// group halts, its remaining launches stopped by the stop word.
+__global__ void run_end_kernel(RunDesc *run, int32_t *ctl, int32_t j) {
+  if (threadIdx.x != 0 || run->halt) return;
+  run->iters[j] = ctl[kIter];
+  if (ctl[kDone]) {
+    run->t = run->t + run->dt;
+    run->done = j + 1;
+  } else {
+    run->halt = 1;
+    ctl[kStop] = 1;
+  }
+}
+
+// The probes of fused step j when its step number is a multiple of
+// `every`: row s / every - 1 of the run's output.
+constexpr int kRunProbeBlocks = 4;
+__global__ void __launch_bounds__(kBlock)
+    run_probe_kernel(const RunDesc *run, const double *x, int64_t n_nodes, int32_t j) {
+  if (run->halt) return;
+  const int64_t s = run->s0 + j + 1;
+  if (s % run->every) return;
+  double *out = run->out + (s / run->every - 1) * (int64_t)run->n_ids;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < run->n_ids;
+       i += kRunProbeBlocks * kBlock) {
+    const int32_t id = run->ids[i];
+    out[i] = (id >= 0 && id < n_nodes) ? x[id] : __builtin_nan("");
+  }
+}
+
+// A step halted inside a fused group continues with the ordinary
+// iteration graphs: its stop word cleared unless it failed.
+__global__ void run_resume_kernel(int32_t *ctl) {
+  if (threadIdx.x == 0 && !ctl[kFail] && !ctl[kDone]) ctl[kStop] = 0;
+}
+
 // ids outside 0..n_nodes-1 (ground, a typo mapped to nothing) read no
 // memory: their output is NaN, so a bad probe is visible, never another
 // node's waveform.
@@ -489,6 +554,48 @@ int launch_iteration(xg_tran *T, int variant, int K) {
   return hipGraphLaunch(T->exec[variant][K], T->stream) == hipSuccess ? XG_OK : XG_E_HIP;
 }
 
+// kRunSteps whole later steps of a fixed step (order 2), K iterations
+// each: the launches of a step's graph (variant 4) between a fused step
+// start and the step's end and probe kernels.
+int enqueue_run(xg_tran *T, int K) {
+  const xg_tran_buffers &b = T->b;
+  const int64_t n = std::max(T->n1, (int64_t)T->n_comp);
+  auto *run = (xg::RunDesc *)T->run_dev;
+  for (int j = 0; j < xg::kRunSteps; ++j) {
+    hipLaunchKernelGGL(xg::tran_begin_kernel, dim3(xg::grid_for(n)), dim3(xg::kBlock), 0,
+                       T->stream, T->n1, T->n_comp, 2, 1.5, -0.5, (const double *)b.x_curr,
+                       (const double *)b.xh1, (const double *)b.xh2, b.xpred, b.x, b.step,
+                       0.0, T->bp_tol, 0.0, b.conv, b.active, b.ctl,
+                       (const int32_t *)T->disabled, (const xg::RunDesc *)run);
+    int rc = enqueue_variant(T, 4, K);
+    if (rc) return rc;
+    hipLaunchKernelGGL(xg::run_end_kernel, dim3(1), dim3(64), 0, T->stream, run, b.ctl, j);
+    hipLaunchKernelGGL(xg::run_probe_kernel, dim3(xg::kRunProbeBlocks), dim3(xg::kBlock), 0,
+                       T->stream, (const xg::RunDesc *)run, (const double *)b.x_curr,
+                       T->n1 - 1, j);
+    if (hipGetLastError() != hipSuccess) return XG_E_HIP;
+  }
+  return XG_OK;
+}
+
+int capture_run(xg_tran *T, int K) {
+  if (T->run_exec[K]) return XG_OK;
+  if (hipStreamBeginCapture(T->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return tfail(T, XG_E_HIP, "graph capture begin");
+  const int rc = enqueue_run(T, K);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(T->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return tfail(T, XG_E_HIP, "graph capture end");
+  T->run_graph[K] = g;
+  if (hipGraphInstantiate(&T->run_exec[K], g, nullptr, nullptr, 0) != hipSuccess)
+    return tfail(T, XG_E_HIP, "graph instantiate");
+  return XG_OK;
+}
+
 int read_ctl(xg_tran *T) {
   T->host_reads++;
   if (hipMemcpyAsync(T->ctl_host, T->b.ctl, sizeof(int32_t) * xg::kCtlWords,
@@ -634,7 +741,7 @@ int ad_attempt(xg_tran *T, double beta1, double beta2, bool *newton_ok) {
                      T->stream, T->n1, T->n_comp, a.order, beta1, beta2,
                      (const double *)b.x_curr, (const double *)b.xh1, (const double *)b.xh2,
                      b.xpred, b.x, b.step, a.nxt, a.bp_tol, 1.0 / a.h, b.conv, b.active,
-                     b.ctl, (const int32_t *)T->disabled);
+                     b.ctl, (const int32_t *)T->disabled, nullptr);
   if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "step begin");
   int rc = enqueue_first_load(T, 0, first);
   if (rc) return rc;
@@ -879,7 +986,8 @@ static int dc_solve(xg_tran *T, int init_jct, int32_t *iters) {
   hipLaunchKernelGGL(xg::tran_begin_kernel, dim3(xg::grid_for(n)), dim3(xg::kBlock), 0,
                      T->stream, T->n1, T->n_comp, 1, 0.0, 0.0, (const double *)b.x,
                      (const double *)b.xh1, (const double *)b.xh2, b.xpred, b.x, b.step, 0.0,
-                     T->bp_tol, 0.0, b.conv, b.active, b.ctl, (const int32_t *)T->disabled);
+                     T->bp_tol, 0.0, b.conv, b.active, b.ctl, (const int32_t *)T->disabled,
+                     nullptr);
   if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "dc restart");
   int rc = enqueue_first_load(T, 1, 0, init_jct);
   if (rc) return rc;
@@ -987,26 +1095,11 @@ int xg_tran_uic(xg_tran *T) {
   return XG_OK;
 }
 
-int xg_tran_step(xg_tran *T, double dt) {
-  if (!T || !(dt > 0.0)) return XG_E_ARG;
-  const int order = T->n_steps >= 2 ? 2 : 1;
-  const int first = T->n_steps == 0;
-  const double t_next = T->t + dt;
-  // constant-step ONESTEP predictor coefficients (N_TIA_OneStep.C:1380-1405)
-  const double b1 = order == 2 ? 1.5 : 1.0, b2 = order == 2 ? -0.5 : 0.0;
-  const xg_tran_buffers &b = T->b;
-  const int64_t n = std::max(T->n1, (int64_t)T->n_comp);
-  hipLaunchKernelGGL(xg::tran_begin_kernel, dim3(xg::grid_for(n)), dim3(xg::kBlock), 0,
-                     T->stream, T->n1, T->n_comp, order, b1, b2, (const double *)b.x_curr,
-                     (const double *)b.xh1, (const double *)b.xh2, b.xpred, b.x, b.step,
-                     t_next, T->bp_tol, 1.0 / dt, b.conv, b.active, b.ctl,
-                     (const int32_t *)T->disabled);
-  if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "step begin");
-  // first load, the first k_step iterations and (if converged) completion
-  // in one launch; k_step follows the iterations the last step took
-  int32_t iters = 0;
-  const int k_first = T->use_graphs ? T->k_step : 1;
-  int rc = newton_loop(T, first ? 1 : 2, xg::kTranIters, &iters, first ? 3 : 4, k_first);
+// After a step's Newton loop (rc as newton_loop's, iters its iterations):
+// the halving of a failed step, or the completion (unless the step's graph
+// completed it) and the time advance of a converged one.
+static int step_end(xg_tran *T, double dt, double t_next, int rc, int32_t iters,
+                    bool completed) {
   T->newton_total += iters;
   if (rc < 0) return rc;
   if (rc > 0) {  // no convergence: two half steps
@@ -1022,12 +1115,117 @@ int xg_tran_step(xg_tran *T, double dt) {
     return xg_tran_step(T, dt / 2);
   }
   T->k_step = std::max(1, std::min<int>(iters, xg_tran::kMaxK));
-  if (iters > k_first) {  // converged after the fused graph: complete now
+  if (!completed) {  // converged after the step's graph: complete now
     rc = complete(T, 0);
     if (rc) return rc;
   }
   T->t = t_next;
   T->n_steps++;
+  return XG_OK;
+}
+
+int xg_tran_step(xg_tran *T, double dt) {
+  if (!T || !(dt > 0.0)) return XG_E_ARG;
+  const int order = T->n_steps >= 2 ? 2 : 1;
+  const int first = T->n_steps == 0;
+  const double t_next = T->t + dt;
+  // constant-step ONESTEP predictor coefficients (N_TIA_OneStep.C:1380-1405)
+  const double b1 = order == 2 ? 1.5 : 1.0, b2 = order == 2 ? -0.5 : 0.0;
+  const xg_tran_buffers &b = T->b;
+  const int64_t n = std::max(T->n1, (int64_t)T->n_comp);
+  hipLaunchKernelGGL(xg::tran_begin_kernel, dim3(xg::grid_for(n)), dim3(xg::kBlock), 0,
+                     T->stream, T->n1, T->n_comp, order, b1, b2, (const double *)b.x_curr,
+                     (const double *)b.xh1, (const double *)b.xh2, b.xpred, b.x, b.step,
+                     t_next, T->bp_tol, 1.0 / dt, b.conv, b.active, b.ctl,
+                     (const int32_t *)T->disabled, nullptr);
+  if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "step begin");
+  // first load, the first k_step iterations and (if converged) completion
+  // in one launch; k_step follows the iterations the last step took
+  int32_t iters = 0;
+  const int k_first = T->use_graphs ? T->k_step : 1;
+  int rc = newton_loop(T, first ? 1 : 2, xg::kTranIters, &iters, first ? 3 : 4, k_first);
+  return step_end(T, dt, t_next, rc, iters, iters <= k_first);
+}
+
+int xg_tran_prepare_run(xg_tran *T) {
+  if (!T) return XG_E_ARG;
+  if (T->run_dev) return XG_OK;
+  // dense-front solves (components too large for the task lists) take
+  // milliseconds per iteration and hundreds of launches: their steps stay
+  // one per launch
+  if (xg::handle_n_fronts(T->h) > 0) return XG_OK;
+  if (hipMalloc(&T->run_dev, sizeof(xg::RunDesc)) != hipSuccess ||
+      hipHostMalloc(&T->run_in, sizeof(xg::RunDesc)) != hipSuccess ||
+      hipHostMalloc(&T->run_out, sizeof(xg::RunDesc)) != hipSuccess)
+    return XG_E_NOMEM;
+  std::memset(T->run_in, 0, sizeof(xg::RunDesc));
+  for (int K = 1; K <= xg_tran::kMaxK; ++K)
+    if (int rc = capture_run(T, K)) return rc;
+  return XG_OK;
+}
+
+int xg_tran_run(xg_tran *T, double dt, int64_t n_steps, const int32_t *ids, int32_t n_ids,
+                int32_t every, int64_t first_step, double *out, double *times) {
+  if (!T || !(dt > 0.0) || n_steps < 0 || n_ids < 0 || every < 1 || first_step < 0 ||
+      (n_ids > 0 && (!ids || !out)))
+    return XG_E_ARG;
+  const int32_t *ctl = T->ctl_host;
+  int64_t s = 0;
+  while (s < n_steps) {
+    const int K = T->k_step;
+    if (T->use_graphs && T->run_dev && T->n_steps >= 2 && n_steps - s >= xg::kRunSteps &&
+        T->run_exec[K]) {
+      auto *in = (xg::RunDesc *)T->run_in;
+      auto *res = (const xg::RunDesc *)T->run_out;
+      in->t = T->t;
+      in->dt = dt;
+      in->inv_dt = 1.0 / dt;
+      in->ids = ids;
+      in->out = out;
+      in->s0 = first_step + s;
+      in->every = every;
+      in->n_ids = n_ids;
+      in->halt = 0;
+      in->done = 0;
+      T->host_reads++;
+      if (hipMemcpyAsync(T->run_dev, in, sizeof(xg::RunDesc), hipMemcpyHostToDevice,
+                         T->stream) != hipSuccess ||
+          hipGraphLaunch(T->run_exec[K], T->stream) != hipSuccess ||
+          hipMemcpyAsync(T->run_out, T->run_dev, sizeof(xg::RunDesc), hipMemcpyDeviceToHost,
+                         T->stream) != hipSuccess ||
+          hipMemcpyAsync(T->ctl_host, T->b.ctl, sizeof(int32_t) * xg::kCtlWords,
+                         hipMemcpyDeviceToHost, T->stream) != hipSuccess ||
+          hipStreamSynchronize(T->stream) != hipSuccess)
+        return tfail(T, XG_E_HIP, "fused steps");
+      for (int j = 0; j < res->done; ++j) {
+        T->t = T->t + dt;
+        T->n_steps++;
+        T->newton_total += res->iters[j];
+        T->k_step = std::max(1, std::min<int>(res->iters[j], xg_tran::kMaxK));
+        ++s;
+        if (times && (first_step + s) % every == 0) times[(first_step + s) / every - 1] = T->t;
+      }
+      if (!res->halt) continue;
+      // the halted step: more iterations, or the failure's halving
+      int32_t iters = res->iters[res->done];
+      int rc = ctl[xg::kFail];
+      if (!rc) {
+        hipLaunchKernelGGL(xg::run_resume_kernel, dim3(1), dim3(64), 0, T->stream, T->b.ctl);
+        if (hipGetLastError() != hipSuccess) return tfail(T, XG_E_HIP, "fused steps");
+        rc = newton_loop(T, 2, xg::kTranIters, &iters);
+      }
+      if ((rc = step_end(T, dt, T->t + dt, rc, iters, false))) return rc;
+    } else if (int rc = xg_tran_step(T, dt)) {
+      return rc;
+    }
+    ++s;
+    const int64_t g = first_step + s;
+    if (g % every) continue;
+    if (times) times[g / every - 1] = T->t;
+    if (n_ids &&
+        xg_tran_probe(T, ids, n_ids, out + (g / every - 1) * (int64_t)n_ids) != XG_OK)
+      return tfail(T, XG_E_HIP, "probe");
+  }
   return XG_OK;
 }
 
@@ -1194,6 +1392,13 @@ int xg_tran_destroy(xg_tran *T) {
       if (T->exec[v][K]) (void)hipGraphExecDestroy(T->exec[v][K]);
       if (T->graph[v][K]) (void)hipGraphDestroy(T->graph[v][K]);
     }
+  for (int K = 0; K <= xg_tran::kMaxK; ++K) {
+    if (T->run_exec[K]) (void)hipGraphExecDestroy(T->run_exec[K]);
+    if (T->run_graph[K]) (void)hipGraphDestroy(T->run_graph[K]);
+  }
+  if (T->run_dev) (void)hipFree(T->run_dev);
+  if (T->run_in) (void)hipHostFree(T->run_in);
+  if (T->run_out) (void)hipHostFree(T->run_out);
   xg_set_step_source(T->h, nullptr);
   xg_set_active_mask(T->h, nullptr);
   xg_set_stream(T->h, T->old_stream);

@@ -66,14 +66,19 @@ _FAIL_WORDS = {1: "singular Jacobian", 2: "non-finite device values", 3: "no con
 
 class Transient:
     def __init__(self, nl, gmin=1e-12, limiter=True, bp_tol=None, device="cuda",
-                 solver="lu", driver=None, graphs=True, lu_fronts=False, lu_mode="auto"):
+                 solver="lu", driver=None, graphs=True, lu_fronts=False, lu_mode="auto",
+                 fused_steps=True):
         # the device load's host analysis runs unlocked; its upload, the
         # buffers and the driver's graph captures under DEVICE_LOCK.
         # lu_fronts: every component through dense fronts; lu_mode: the
-        # linear-solve form otherwise (loader.LU_MODES)
+        # linear-solve form otherwise (loader.LU_MODES); fused_steps: the
+        # native driver's fixed steps in groups per graph launch
+        # (xg_tran_run), with graphs
         dev = DeviceLoad(nl, gmin=gmin, limiter=limiter, lu_fronts=lu_fronts, lu_mode=lu_mode)
         with DEVICE_LOCK:
             self._init(nl, dev, bp_tol, device, solver, driver, graphs, limiter)
+            if self._tr is not None and graphs and fused_steps:
+                check(lib().xg_tran_prepare_run(self._tr), "fused-step graphs")
 
     def _init(self, nl, dev, bp_tol, device, solver, driver, graphs, limiter):
         """solver: "lu" (static-pivot sparse LU with pivoting fallback) or
@@ -378,26 +383,26 @@ class Transient:
         self.uic() if uic else self.dcop(dc)
         check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes), out[0].data_ptr()),
               "probe")
-        times = [0.0]
-        tnow = ctypes.c_double()
-        k = 1
-        for s in range(1, n + 1):
-            rc = L.xg_tran_step(self._tr, dt)
+        # the steps in calls of 50 when progress is reported, else one call
+        rec = n // every
+        times = np.zeros(rec + 1)
+        row1 = out.data_ptr() + out.element_size() * len(probes)
+        chunk = 50 if progress is not None else max(n, 1)
+        s = 0
+        while s < n:
+            m = min(chunk, n - s)
+            rc = L.xg_tran_run(self._tr, dt, m, ids.data_ptr(), len(probes), every, s, row1,
+                               times[1:].ctypes.data)
             if rc:
                 self._native(rc, "time step")
+            s += m
             if progress is not None and s % 50 == 0:
                 progress(s, n)
-            if s % every == 0:
-                check(L.xg_tran_probe(self._tr, ids.data_ptr(), len(probes),
-                                      out[k].data_ptr()), "probe")
-                check(L.xg_tran_info(self._tr, ctypes.byref(tnow), None), "driver info")
-                times.append(tnow.value)
-                k += 1
         check(L.xg_tran_sync(self._tr), "sync")
         self._sync_counters()
         with DEVICE_LOCK:
-            w = out[:k].T.cpu().numpy()
-        return np.array(times), w
+            w = out.T.cpu().numpy()
+        return times, w
 
     def run_adaptive(self, tstop, tstep, probes, uic=False, reltol=1e-3, abstol=1e-6,
                      max_points=1_000_000, dc="none", progress=None):
