@@ -205,6 +205,8 @@ def batched_transient_leg(torch, args):
         "batch_latency_s": b["seconds"],
         "batch_newton_iterations": int(b["newton"]),
         "batch_step_failures": int(b["step_failures"]),
+        "batch_host_reads": b["host_reads"],
+        "fixed_steps_per_graph_launch": 8,
         "batch_setup_s": b["setup_s"],
     }
     if args.mc_plain <= 0:
@@ -229,6 +231,8 @@ def batched_transient_leg(torch, args):
         "batch_newton_iterations": int(b["newton"]),
         "plain_newton_iterations_mean": float(p["newton"].mean()),
         "batch_step_failures": int(b["step_failures"]),
+        "batch_host_reads": b["host_reads"],
+        "fixed_steps_per_graph_launch": 8,
         "batch_setup_s": b["setup_s"],
         "max_waveform_diff_batch_vs_plain_V": diff,
     }

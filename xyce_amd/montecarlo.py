@@ -37,7 +37,7 @@ class BatchedTransient:
 
 def run_batched(parts, tstop, dt, probes, every=1, sync=None):
     """One batch of every sample.  Returns dict(times, waves, seconds,
-    setup_s, newton, step_failures)."""
+    setup_s, newton, step_failures, host_reads)."""
     b = BatchedTransient(parts)
     if sync:
         sync()
@@ -47,7 +47,8 @@ def run_batched(parts, tstop, dt, probes, every=1, sync=None):
         sync()
     el = time.perf_counter() - t0
     return dict(times=t, waves=w, seconds=el, setup_s=b.setup_s,
-                newton=b.tr.newton_total, step_failures=b.tr.step_failures)
+                newton=b.tr.newton_total, step_failures=b.tr.step_failures,
+                host_reads=b.tr.host_reads())
 
 
 def run_plain(parts, tstop, dt, probes, every=1, sync=None):

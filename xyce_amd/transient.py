@@ -166,6 +166,15 @@ class Transient:
         check(rc, what)
         self._sync_counters()
 
+    def host_reads(self):
+        """The native driver's control-word reads so far (host round trips
+        of the Newton loop); None without the native driver."""
+        if self._tr is None:
+            return None
+        c = (ctypes.c_int64 * 4)()
+        check(lib().xg_tran_counters(self._tr, c), "driver counters")
+        return int(c[0])
+
     def _sync_counters(self):
         t = ctypes.c_double()
         c = (ctypes.c_int64 * 4)()
